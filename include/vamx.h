@@ -1,0 +1,212 @@
+/*
+ * vamx.h -- C ABI of the MI355X (gfx950) light-propagation library for
+ * volumetric additive manufacturing (libvamx_hip.so).
+ *
+ * Plain pointers and sizes only.  Every device pointer is caller-owned
+ * (HIP device memory); every call enqueues work on the caller's HIP stream
+ * (passed as void*; NULL = the null stream) and returns immediately.  Every
+ * entry point returns VAMX_OK (0) or a negative status; the message of the
+ * last failure on the calling thread is returned by vamx_last_error().
+ *
+ * Layouts (float32, C order, the reference's NumPy layouts):
+ *   volume    (ny, nx, nz)        z contiguous     vamtoolbox/geometry.py:223-225
+ *   sinogram  (nu, n_angles, nz)  z contiguous     vamtoolbox/geometry.py:232-234
+ *   ray-trace medium (nx, ny, nz) z contiguous (axis 0 = x) and its sinogram
+ *             (nt, n_angles, nz)  flat C order    vamtoolbox/projector/pyTorchRayTrace.py:1522-1540
+ */
+#ifndef VAMX_H
+#define VAMX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VAMX_OK 0
+#define VAMX_ERR_ARG (-1)      /* invalid argument / shape                      */
+#define VAMX_ERR_HIP (-2)      /* a HIP runtime call failed                     */
+#define VAMX_ERR_STATE (-3)    /* plan of the wrong kind or not initialised     */
+
+typedef struct vamx_plan vamx_plan;
+
+/* ---------------------------------------------------------------------------
+ * Parallel-beam projector (reference discretisation "D1" plus extensions)
+ * ------------------------------------------------------------------------- */
+#define VAMX_PAR_NO_WARP_CLIP 1   /* forward: skip the per-slice output clip of
+                                     skimage.transform.warp(..., clip=True)    */
+#define VAMX_PAR_NO_CIRCLE_CLIP 2 /* backward: skip util.data.clipToCircle     */
+
+typedef struct vamx_par_geom {
+  int32_t n;                /* nx == ny == nu (square slices)                   */
+  int32_t nz;               /* slices == detector rows                          */
+  int32_t n_angles;
+  const double* angles_deg; /* host array [n_angles], copied into the plan      */
+  double det_offset;        /* detector-u offset in voxels (0: reference)       */
+  double incl_deg;          /* inclination phi (0: reference)                   */
+  double mu;                /* absorption per voxel inside the cylinder (0: none) */
+  double radius;            /* cylinder radius in voxels; <= 0 -> (n-1)/2        */
+  const uint8_t* outside_mask; /* host array [n*n], 1 where clipToCircle zeroes,
+                                  computed by the caller with the reference's
+                                  formula; copied into the plan                 */
+  int32_t flags;            /* VAMX_PAR_* bits                                   */
+} vamx_par_geom;
+
+/* Replaces the construction of Projector3DParallelPython / Projector3DParallelCUDAAstra
+ * (vamtoolbox/projector/Projector3DParallel.py:57-75,
+ *  vamtoolbox/projector/Projector3DParallelCUDA.py:13-38). */
+int vamx_par_create(const vamx_par_geom* geom, vamx_plan** out_plan);
+
+/* b = A x.  Replaces Projector3DParallelPython.forward
+ * (vamtoolbox/projector/Projector3DParallel.py:113-156) and the ASTRA call
+ * astra.create_sino3d_gpu (vamtoolbox/projector/Projector3DParallelCUDA.py:40-52). */
+int vamx_par_forward(vamx_plan* plan, const float* vol, float* sino, void* stream);
+
+/* x = B b (voxel-driven, np.interp gather, circle clip).  Replaces
+ * Projector3DParallelPython.backward (vamtoolbox/projector/Projector3DParallel.py:158-207)
+ * and astra.creators.create_backprojection3d_gpu
+ * (vamtoolbox/projector/Projector3DParallelCUDA.py:54-72). */
+int vamx_par_backward(vamx_plan* plan, const float* sino, float* vol, void* stream);
+
+/* x = A^T b, the exact transpose of the linear part of vamx_par_forward
+ * (autograd adjoint).  The reference has no such operator; it stands in for
+ * the transpose that vamtoolbox/optimizer/BCLP.py:372 approximates with
+ * P.forward. */
+int vamx_par_forward_adjoint(vamx_plan* plan, const float* sino, float* vol, void* stream);
+
+/* b = B^T x, the exact transpose of vamx_par_backward (autograd adjoint of
+ * the dose operator; BCLP.py:353-375 uses P.forward in its place). */
+int vamx_par_backward_adjoint(vamx_plan* plan, const float* vol, float* sino, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Fused BCLP iteration (vamtoolbox/optimizer/BCLP.py:277-386, 477-534)
+ * ------------------------------------------------------------------------- */
+#define VAMX_RESP_GLF 0       /* analytical generalised logistic, response.py:191-218 */
+#define VAMX_RESP_LINEAR 1    /* response.py:233-241 */
+#define VAMX_RESP_IDENTITY 2  /* response.py:245-253 */
+#define VAMX_RESP_TABLE 3     /* interpolation tables, response.py:256-279 */
+
+typedef struct vamx_response {
+  int32_t kind;               /* VAMX_RESP_*                                      */
+  double A, K, B, M, nu;      /* GLF parameters                                   */
+  double lin_M, lin_C;        /* linear: map = lin_M * f + lin_C                  */
+  int32_t n_pts;              /* table: nodes f_i = f0 + i*(f1-f0)/(n_pts-1)      */
+  const float* table_f;       /* device [n_pts] node positions                    */
+  const float* table_map;     /* device [n_pts]                                   */
+  const float* table_dmap;    /* device [n_pts]                                   */
+} vamx_response;
+
+typedef struct vamx_bclp_params {
+  double p, q;                /* Lp norm and outer power                          */
+  double eps;                 /* band half width (used when eps_arr == NULL)      */
+  double weight;              /* used when weight_arr == NULL                     */
+  double learning_rate;
+  double glb, gub;            /* sinogram bounds; use -inf / +inf for "none"      */
+  double dvol;                /* differential volume (BCLP.py:184)                */
+} vamx_bclp_params;
+
+/* Workspace bytes needed by vamx_bclp_step for this plan. */
+int vamx_bclp_workspace_bytes(vamx_plan* plan, int64_t* bytes);
+
+/* One BCLP gradient-descent iteration on sinogram g (in place):
+ *   dose = B g;  m = map(dose);  e = m - target;  b = |e| - eps;  v = b > 0
+ *   L = (sum v w |b|^p * dvol)^(q/p)      -> loss_log[*iter_counter]
+ *   operand = v w |b|^(p-1) sign(e) map'(dose)
+ *   g <- clip(g - lr * s * A operand, glb, gub),  s = q L^((q-p)/q) (1 if L == 0)
+ * and ++*iter_counter.  dose (device [V]) receives B g of the input g.
+ * eps_arr / weight_arr are optional device arrays of the volume's size.
+ * iter_counter is a device int32; workspace has vamx_bclp_workspace_bytes.
+ * No host synchronisation: the step can be captured in a HIP graph. */
+int vamx_bclp_step(vamx_plan* plan, const vamx_response* resp, const vamx_bclp_params* prm,
+                   float* g, const float* target, const float* eps_arr, const float* weight_arr,
+                   float* dose, double* loss_log, int32_t* iter_counter,
+                   void* workspace, void* stream);
+
+/* Loss, dose and operand of the current g without updating it
+ * (BCLP.py:277-351 for the final evaluation after the loop, 502-505). */
+int vamx_bclp_eval(vamx_plan* plan, const vamx_response* resp, const vamx_bclp_params* prm,
+                   const float* g, const float* target, const float* eps_arr,
+                   const float* weight_arr, float* dose, float* operand, double* loss_out,
+                   void* workspace, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Curved-ray tracer (vamtoolbox/projector/pyTorchRayTrace.py)
+ * ------------------------------------------------------------------------- */
+#define VAMX_RT_SYMPLECTIC_EULER 0   /* pyTorchRayTrace.py:738-775 */
+#define VAMX_RT_FORWARD_EULER 1      /* pyTorchRayTrace.py:777-814 */
+#define VAMX_RT_LEAPFROG 2           /* pyTorchRayTrace.py:816-828 + symplectic Euler */
+#define VAMX_RT_RK4 3                /* classical RK4 (placeholder in the reference, 289-292) */
+
+#define VAMX_RT_CANONICAL 0          /* dx = v,     dv = n grad n   (pyTorchRayTrace.py:695-705) */
+#define VAMX_RT_PHYSICAL 1           /* dx = v/n,   dv = grad n     (708-718) */
+#define VAMX_RT_OPTICAL 2            /* dx = v/n^2, dv = grad n / n (721-735) */
+
+#define VAMX_RT_ATT_NONE 0
+#define VAMX_RT_ATT_CYLINDER 1       /* medium.py:895-903 */
+#define VAMX_RT_ATT_BALL 2           /* medium.py:907-915 */
+#define VAMX_RT_ATT_GRID 3           /* grid_sample, zeros padding, medium.py:161-185 */
+
+typedef struct vamx_rt_geom {
+  int32_t nx, ny, nz;            /* medium grid (axis 0 = x)                           */
+  double xmin, ymin, zmin;       /* physical coordinates of voxel (0,0,0)              */
+  double dx, dy, dz;             /* voxel sizes (dz ignored when nz == 1)               */
+  int32_t nt, n_angles, nzr;     /* rays: (nt, n_angles, nzr), C order                  */
+  const double* angles_deg;      /* host [n_angles]                                     */
+  double incl_deg;
+  double step;                   /* parametrisation step (reference: dx / 2)           */
+  int32_t max_steps;             /* reference: 3 * nx                                   */
+  int32_t exit_every;            /* reference: 10                                       */
+  int32_t solver;                /* VAMX_RT_* solver                                    */
+  int32_t param;                 /* VAMX_RT_* parametrisation                           */
+  double ray_energy0;            /* 1 / ray_density^2 (3-D) or 1 / ray_density (2-D)    */
+  /* medium: float4 (n, dn/dx, dn/dy, dn/dz) per voxel, device, caller-owned,
+   * sampled trilinearly with border padding (medium.py:161-211)          */
+  const float* n_grad;
+  int32_t att_kind;              /* attenuation model (VAMX_RT_ATT_*)                   */
+  double att_alpha, att_radius;  /* analytic attenuation                                 */
+  const float* att_grid;         /* device [nx*ny*nz] for VAMX_RT_ATT_GRID               */
+  const float* abs_grid;         /* device [nx*ny*nz] absorption at grid points (alpha_abs) */
+} vamx_rt_geom;
+
+/* Replaces PyTorchRayTracingPropagator.__init__ / RayState.setupRaysParallel
+ * (vamtoolbox/projector/pyTorchRayTrace.py:33-85, 1515-1666). */
+int vamx_rt_create(const vamx_rt_geom* geom, vamx_plan** out_plan);
+
+/* g = W (alpha_abs * f): replaces PyTorchRayTracingPropagator.forward
+ * (pyTorchRayTrace.py:87-107 -> integrateEnergyUntilExit 471-571). */
+int vamx_rt_integrate(vamx_plan* plan, const float* f, float* g, void* stream);
+
+/* dose = alpha_abs * (W^T g): replaces PyTorchRayTracingPropagator.backward
+ * (pyTorchRayTrace.py:109-133 -> depositEnergyUntilExit 378-469).  The
+ * deposit is deterministic (64-bit fixed-point accumulation). */
+int vamx_rt_deposit(vamx_plan* plan, const float* g, float* dose, void* stream);
+
+/* Ray-step statistics of the last integrate/deposit call on this plan:
+ * steps[0] = total ray-steps executed, steps[1] = rays still active at
+ * max_steps (the reference logs an error for those, pyTorchRayTrace.py:566-569).
+ * Synchronises the plan's stream. */
+int vamx_rt_stats(vamx_plan* plan, int64_t* steps);
+
+/* ---------------------------------------------------------------------------
+ * Batched uneven parallel-beam problems in one launch
+ * ------------------------------------------------------------------------- */
+/* Forward-project n_plans independent problems (each a parallel plan created
+ * with vamx_par_create) in a single persistent launch; vols[i]/sinos[i] are
+ * device pointers of problem i.  Work is balanced by n*n_angles*n*nz. */
+int vamx_par_forward_batched(vamx_plan* const* plans, int32_t n_plans,
+                             const float* const* vols, float* const* sinos, void* stream);
+int vamx_par_backward_batched(vamx_plan* const* plans, int32_t n_plans,
+                              const float* const* sinos, float* const* vols, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * common
+ * ------------------------------------------------------------------------- */
+void vamx_destroy(vamx_plan* plan);
+const char* vamx_last_error(void);
+int vamx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VAMX_H */

@@ -1,0 +1,48 @@
+// Error reporting, plan lifetime and small helpers of libvamx_hip.so.
+#include "vamx_internal.h"
+
+namespace vamx {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int hip_check(hipError_t e, const char* what) {
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  return set_error(VAMX_ERR_HIP, m);
+}
+
+bool check_plan(vamx_plan* p, int kind) { return p != nullptr && p->kind == kind; }
+
+unsigned blocks_for(long long total, int block) {
+  long long b = (total + block - 1) / block;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
+}  // namespace vamx
+
+extern "C" void vamx_destroy(vamx_plan* p) {
+  if (!p) return;
+  int cur = 0;
+  hipGetDevice(&cur);
+  if (cur != p->device) hipSetDevice(p->device);
+  hipDeviceSynchronize();
+  if (p->d_mats) hipFree(p->d_mats);
+  if (p->d_cs) hipFree(p->d_cs);
+  if (p->d_outside) hipFree(p->d_outside);
+  if (p->d_minmax_part) hipFree(p->d_minmax_part);
+  if (p->d_minmax) hipFree(p->d_minmax);
+  if (p->d_rt_tables) hipFree(p->d_rt_tables);
+  if (p->d_accum) hipFree(p->d_accum);
+  if (p->d_gmax) hipFree(p->d_gmax);
+  if (p->d_stats) hipFree(p->d_stats);
+  if (cur != p->device) hipSetDevice(cur);
+  delete p;
+}
+
+extern "C" const char* vamx_last_error(void) { return vamx::g_last_error.c_str(); }
+
+extern "C" int vamx_version(void) { return 1; }

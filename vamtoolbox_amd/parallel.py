@@ -1,0 +1,108 @@
+"""Device-level parallel-beam projector plan (float32 torch tensors on the GPU).
+
+Thin owner of a ``vamx_plan`` created by ``vamx_par_create``; every method
+enqueues one HIP kernel sequence on torch's current stream through the C ABI.
+Layouts follow the reference: volume (n, n, nz), sinogram (n, n_angles, nz),
+z contiguous (vamtoolbox/geometry.py:223-234).
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+def outside_circle_mask(n: int) -> np.ndarray:
+    """clipToCircle's mask (vamtoolbox/util/data.py:29-38): True outside the
+    inscribed circle, computed with the reference's float64 formula."""
+    circle_y, circle_x = np.meshgrid(np.linspace(-1, 1, n), np.linspace(-1, 1, n))
+    return circle_x ** 2 + circle_y ** 2 > 1
+
+
+class ParallelPlan:
+    """Parallel-beam geometry bound to one GPU.
+
+    Parameters mirror ``vamx_par_geom``: ``n`` (square slice size), ``nz``,
+    ``angles_deg``, ``det_offset`` (voxels), ``incl_deg``, ``mu`` (absorption
+    per voxel), ``radius`` (cylinder radius, default (n-1)/2), and the two
+    clip switches (both on = reference behaviour).
+    """
+
+    def __init__(self, n: int, nz: int, angles_deg, det_offset: float = 0.0, incl_deg: float = 0.0,
+                 mu: float = 0.0, radius: float | None = None, warp_clip: bool = True,
+                 circle_clip: bool = True, device=None):
+        self.lib = _lib.load()
+        self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+        self.n, self.nz = int(n), int(nz)
+        self.angles_deg = np.ascontiguousarray(np.atleast_1d(np.asarray(angles_deg, dtype=np.float64)))
+        self.na = int(self.angles_deg.size)
+        self.det_offset, self.incl_deg, self.mu = float(det_offset), float(incl_deg), float(mu)
+        self.radius = float(radius) if radius is not None else (self.n - 1) / 2.0
+        mask = np.ascontiguousarray(outside_circle_mask(self.n).astype(np.uint8))
+        flags = (0 if warp_clip else _lib.PAR_NO_WARP_CLIP) | (0 if circle_clip else _lib.PAR_NO_CIRCLE_CLIP)
+        geom = _lib.ParGeom(
+            n=self.n, nz=self.nz, n_angles=self.na,
+            angles_deg=self.angles_deg.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+            det_offset=self.det_offset, incl_deg=self.incl_deg, mu=self.mu, radius=self.radius,
+            outside_mask=mask.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), flags=flags)
+        handle = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(self.lib.vamx_par_create(ctypes.byref(geom), ctypes.byref(handle)), "vamx_par_create")
+        self.handle = handle
+        self.vol_shape = (self.n, self.n, self.nz)
+        self.sino_shape = (self.n, self.na, self.nz)
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                self.lib.vamx_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
+
+    # -- helpers ---------------------------------------------------------
+    def _in(self, x: torch.Tensor, shape) -> torch.Tensor:
+        if not isinstance(x, torch.Tensor):
+            raise TypeError("expected a torch tensor")
+        if x.device != self.device:
+            raise ValueError("tensor on %s, plan on %s" % (x.device, self.device))
+        if tuple(x.shape) != tuple(shape):
+            if x.numel() == int(np.prod(shape)):
+                x = x.reshape(shape)
+            else:
+                raise ValueError("expected shape %s, got %s" % (shape, tuple(x.shape)))
+        if x.dtype != torch.float32:
+            x = x.float()
+        return x.contiguous()
+
+    def _run(self, name, src, dst):
+        with torch.cuda.device(self.device):
+            _lib.check(getattr(self.lib, name)(self.handle, src.data_ptr(), dst.data_ptr(),
+                                               _lib.stream_ptr(self.device)), name)
+        return dst
+
+    # -- operators -------------------------------------------------------
+    def forward(self, vol: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        vol = self._in(vol, self.vol_shape)
+        out = torch.empty(self.sino_shape, device=self.device, dtype=torch.float32) if out is None else out
+        return self._run("vamx_par_forward", vol, out)
+
+    def backward(self, sino: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        sino = self._in(sino, self.sino_shape)
+        out = torch.empty(self.vol_shape, device=self.device, dtype=torch.float32) if out is None else out
+        return self._run("vamx_par_backward", sino, out)
+
+    def forward_adjoint(self, sino: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        sino = self._in(sino, self.sino_shape)
+        out = torch.empty(self.vol_shape, device=self.device, dtype=torch.float32) if out is None else out
+        return self._run("vamx_par_forward_adjoint", sino, out)
+
+    def backward_adjoint(self, vol: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        vol = self._in(vol, self.vol_shape)
+        out = torch.empty(self.sino_shape, device=self.device, dtype=torch.float32) if out is None else out
+        return self._run("vamx_par_backward_adjoint", vol, out)
