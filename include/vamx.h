@@ -90,10 +90,10 @@ typedef struct vamx_response {
   int32_t kind;               /* VAMX_RESP_*                                      */
   double A, K, B, M, nu;      /* GLF parameters                                   */
   double lin_M, lin_C;        /* linear: map = lin_M * f + lin_C                  */
-  int32_t n_pts;              /* table: nodes f_i = f0 + i*(f1-f0)/(n_pts-1)      */
-  const float* table_f;       /* device [n_pts] node positions                    */
-  const float* table_map;     /* device [n_pts]                                   */
-  const float* table_dmap;    /* device [n_pts]                                   */
+  int32_t n_pts;              /* table: np.interp nodes (increasing)              */
+  const double* table_f;      /* device [n_pts] node positions (interp_f_0)       */
+  const double* table_map;    /* device [n_pts] (interp_map_0)                    */
+  const double* table_dmap;   /* device [n_pts] (interp_dmapdf_0)                 */
 } vamx_response;
 
 typedef struct vamx_bclp_params {
@@ -103,6 +103,9 @@ typedef struct vamx_bclp_params {
   double learning_rate;
   double glb, gub;            /* sinogram bounds; use -inf / +inf for "none"      */
   double dvol;                /* differential volume (BCLP.py:184)                */
+  int32_t grad_op;            /* 0: gradient through the forward projector, as
+                                    BCLP.py:372 does; 1: through the exact
+                                    transpose of the back projector            */
 } vamx_bclp_params;
 
 /* Workspace bytes needed by vamx_bclp_step for this plan. */
@@ -113,6 +116,7 @@ int vamx_bclp_workspace_bytes(vamx_plan* plan, int64_t* bytes);
  *   L = (sum v w |b|^p * dvol)^(q/p)      -> loss_log[*iter_counter]
  *   operand = v w |b|^(p-1) sign(e) map'(dose)
  *   g <- clip(g - lr * s * A operand, glb, gub),  s = q L^((q-p)/q) (1 if L == 0)
+ * (A = forward projector, or B^T when prm->grad_op == 1)
  * and ++*iter_counter.  dose (device [V]) receives B g of the input g.
  * eps_arr / weight_arr are optional device arrays of the volume's size.
  * iter_counter is a device int32; workspace has vamx_bclp_workspace_bytes.

@@ -1,0 +1,219 @@
+"""Float64 restatement of the reference's optimisation loops and helpers.
+
+TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
+
+* ``fourier_filter`` / ``filter_sinogram``  vamtoolbox/util/data.py:302-438
+* ``filter_target_osmo``                    vamtoolbox/util/data.py:43-113
+* ``Response``                              vamtoolbox/response.py:12-291
+* ``osmo``                                  vamtoolbox/optimizer/OSMO.py:9-178
+* ``cal``                                   vamtoolbox/optimizer/CAL.py:10-218 (d_h given)
+* ``bclp``                                  vamtoolbox/optimizer/BCLP.py:138-641
+* ``ver``                                   vamtoolbox/metrics.py:47-87
+
+The projector is passed in as a pair of callables (forward, backward) so the
+loops run on the D1 oracle or on an explicit matrix.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+from scipy.fft import fft, fft2, fftshift, ifft, ifft2, ifftshift
+
+
+def fourier_filter(size, name="ram-lak"):
+    n = np.concatenate((np.arange(1, size / 2 + 1, 2, dtype=int), np.arange(size / 2 - 1, 0, -2, dtype=int)))
+    f = np.zeros(size)
+    f[0] = 0.25
+    f[1::2] = -1 / (np.pi * n) ** 2
+    ff = 2 * np.real(fft(f))
+    if name == "ram-lak":
+        pass
+    elif name == "hamming":
+        ff *= fftshift(np.hamming(size))
+    elif name == "hanning":
+        ff *= fftshift(np.hanning(size))
+    elif name == "shepp-logan":
+        omega = np.pi * np.fft.fftfreq(size)[1:]
+        ff[1:] *= np.sin(omega) / omega
+    elif name == "cosine":
+        ff *= fftshift(np.sin(np.linspace(0, np.pi, size, endpoint=False)))
+    else:
+        raise ValueError(name)
+    return ff
+
+
+def filter_sinogram(s, name="ram-lak"):
+    if name is None:
+        return s
+    s = np.asarray(s, dtype=np.float64)
+    pX = s.shape[0]
+    P = max(64, int(2 ** np.ceil(np.log2(2 * pX))))
+    ff = fourier_filter(P, name)
+    out = np.zeros_like(s)
+    for z in range(s.shape[2]):
+        img = np.zeros((P, s.shape[1]))
+        img[:pX] = s[:, :, z]
+        out[:, :, z] = np.real(ifft(fft(img, axis=0) * ff[:, None], axis=0)[:pX, :])
+    return out
+
+
+def filter_target_osmo(x, name="ram-lak"):
+    if name is None:
+        return x
+    ny, nx = x.shape[0], x.shape[1]
+    fy, fx = np.meshgrid(np.linspace(-1, 1, ny), np.linspace(-1, 1, nx), indexing="ij")
+    f = np.sqrt(fx ** 2 + fy ** 2)
+    if name == "hamming":
+        w = np.abs(np.hamming(nx))
+        f = f * np.abs(np.sqrt(np.outer(w, w)))
+    elif name != "ram-lak":
+        raise ValueError(name)
+    X = fftshift(fft2(x, axes=(0, 1)), axes=(0, 1))
+    return np.real(ifft2(ifftshift(X * f[:, :, None], axes=(0, 1)), axes=(0, 1)))
+
+
+class Response:
+    """Generalised logistic response, analytical or as a 512-point table."""
+
+    def __init__(self, kind="interpolation", A=0.0, K=1.0, B=25.0, M=0.5, nu=1.0, n_pts=512):
+        self.kind = kind
+        self.A, self.K, self.B, self.M, self.nu = A, K, B, M, nu
+        if kind == "interpolation":
+            self.f0 = np.linspace(0, 1, n_pts)
+            self.m0 = self._map(self.f0)
+            self.d0 = self._dmap(self.f0)
+
+    def _map(self, f):
+        return self.A + (self.K - self.A) / (1 + np.exp(-self.B * (f - self.M))) ** (1 / self.nu)
+
+    def _dmap(self, f):
+        c1 = (1 / (self.K - self.A)) ** self.nu
+        c2 = self.B / self.nu
+        c3 = (self._map(f) - self.A) ** (self.nu + 1)
+        return c1 * c2 * c3 * np.exp(-self.B * (f - self.M))
+
+    def map(self, f):
+        if self.kind == "analytical":
+            return self._map(f)
+        return np.interp(f, self.f0, self.m0, left=self.m0[0], right=self.m0[-1])
+
+    def dmap(self, f):
+        if self.kind == "analytical":
+            return self._dmap(f)
+        return np.interp(f, self.f0, self.d0, left=self.d0[0], right=self.d0[-1])
+
+    def map_inv(self, m):
+        if self.kind == "analytical":
+            with np.errstate(divide="ignore", invalid="ignore"):
+                return -np.log(((self.K - self.A) / (m - self.A)) ** self.nu - 1) / self.B + self.M
+        return np.interp(m, self.m0, self.f0, left=self.f0[0], right=self.f0[-1])
+
+
+def circle_inds(target):
+    n0, n1, n2 = target.shape
+    cy, cx, _ = np.meshgrid(np.linspace(-1, 1, n0), np.linspace(-1, 1, n1), np.linspace(-1, 1, n2))
+    inside = cx ** 2 + cy ** 2 <= 1
+    return np.logical_and(target > 0, inside), np.logical_and(target == 0, inside)
+
+
+def ver(gel, void, x):
+    return np.sum(x[void] >= np.min(x[gel])) / (np.sum(gel) + np.sum(void))
+
+
+def osmo(target, fwd, bwd, n_iter, d_h=0.8, d_l=0.7, filt="ram-lak", inhibition=0.0):
+    """OSMO.py:111-178 (no bit depth, no exit_param)."""
+    gel, void = circle_inds(target)
+    err = np.full(n_iter, np.nan)
+    xm = np.real(filter_target_osmo(target, filt))
+    b = np.clip(fwd(xm), 0, None)
+    x = bwd(b)
+    x = x / np.amax(x)
+    err[0] = ver(gel, void, x)
+
+    def step(xm):
+        b = fwd(xm)
+        b = b / np.amax(b)
+        b[b < inhibition] = inhibition
+        xu = bwd(b)
+        xu = xu / np.amax(xu)
+        xu[xu < 0] = 0
+        return xu, b
+
+    for it in range(1, n_iter):
+        vd = x[void] - d_l
+        vd[vd < 0] = 0
+        xm = xm.copy()
+        xm[void] = xm[void] - vd
+        x, _ = step(xm)
+        gd = d_h - x[gel]
+        gd[gd < 0] = 0
+        xm[gel] = xm[gel] + gd
+        x, b = step(xm)
+        err[it] = ver(gel, void, x)
+    return b, x, err
+
+
+def cal(target, fwd, bwd, n_iter, d_h=0.8, filt="ram-lak", lr=0.01, momentum=0.0, sigmoid=0.01):
+    """CAL.py:116-218 with d_h given (sigmoid threshold)."""
+    gel, void = circle_inds(target)
+    err = np.full(n_iter, np.nan)
+    b = np.clip(filter_sinogram(fwd(target), filt), 0, None)
+    db_prev = np.zeros_like(b)
+    x = bwd(b)
+    x = x / np.max(x)
+    err[0] = ver(gel, void, x)
+    b_opt = b
+    for it in range(1, n_iter):
+        xt = 1 / (1 + np.exp(-(x - d_h) * (1 / sigmoid)))
+        db = fwd(xt - target)
+        grad = db * (1 - momentum) + db_prev * momentum / (1 - momentum ** (it + 1))
+        b = np.clip(b - lr * grad, 0, None)
+        x = bwd(b)
+        x = x / np.amax(x)
+        db_prev = db
+        b_opt = b
+        err[it] = ver(gel, void, x)
+    b_opt = np.clip(b_opt, 0, None)
+    b_opt = b_opt / np.amax(b_opt)
+    x_opt = bwd(b_opt)
+    return b_opt, x_opt / np.amax(x_opt), err
+
+
+def bclp_state(g, target, bwd, resp, eps, weight, p, q, dvol=1.0):
+    """BCLPNorm.updateVariables / computeLoss / operand (BCLP.py:277-367)."""
+    dose = bwd(g)
+    m = resp.map(dose)
+    eT = m - target
+    eb = np.abs(eT) - eps
+    v = eb > 0
+    loss = (np.sum(v * weight * np.abs(eb) ** p) * dvol) ** (q / p)
+    operand = v * weight * (np.abs(eb) ** (p - 1)) * np.sign(eT) * resp.dmap(dose)
+    return dose, m, loss, operand
+
+
+def bclp(target, fwd, bwd, resp, n_iter, g0, eps=0.1, weight=1.0, p=2.0, q=1.0, lr=0.01, glb=0.0, gub=None,
+         grad_op=None):
+    """BCLPNorm.gradientDescent (BCLP.py:477-526) from a given initial sinogram.
+
+    ``grad_op`` (default ``fwd``) maps the operand to sinogram space.
+    """
+    grad_op = fwd if grad_op is None else grad_op
+    g = np.clip(np.asarray(g0, dtype=np.float64), glb, gub)
+    losses = np.full(n_iter + 1, np.nan)
+    for k in range(n_iter):
+        dose, m, loss, operand = bclp_state(g, target, bwd, resp, eps, weight, p, q)
+        losses[k] = loss
+        G = grad_op(operand)
+        if loss != 0.0:
+            G = (q * loss ** ((q - p) / q)) * G
+        g = np.clip(g - lr * G, glb, gub)
+    dose, m, loss, _ = bclp_state(g, target, bwd, resp, eps, weight, p, q)
+    losses[n_iter] = loss
+    return g, dose, m, losses
+
+
+def bclp_init(target, fwd, resp, n_angles, filt="ram-lak", glb=0.0, gub=None):
+    """BCLP.py:258-271 (projector without inverseBackward)."""
+    g0 = fwd(resp.map_inv(target))
+    g0 = filter_sinogram(g0, filt) * (np.pi / (2 * n_angles))
+    return np.clip(g0, glb, gub)

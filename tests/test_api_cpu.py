@@ -1,0 +1,75 @@
+"""Host-side API of the drop-in (CPU): geometry, options, dispatch errors."""
+
+import numpy as np
+import pytest
+
+import vamtoolbox_amd as vam
+
+
+def test_target_geometry_matches_reference_semantics():
+    import oracle.d1 as d1
+    rng = np.random.default_rng(0)
+    t = (rng.random((20, 20)) > 0.5).astype(float)
+    tg = vam.geometry.TargetGeometry(target=t)
+    assert tg.array.shape == (20, 20, 1) and tg.n_dim == 3
+    np.testing.assert_array_equal(tg.array, d1.clip_to_circle(t[:, :, None]))
+    gel, void = vam.geometry.getInds(np.atleast_3d(t))
+    np.testing.assert_array_equal(tg.gel_inds, gel)
+    assert not np.any(gel & void)
+    assert tg.nY == 20 and tg.nX == 20 and tg.nZ == 1
+
+
+def test_input_not_mutated():
+    t = np.ones((8, 8, 2))
+    vam.geometry.TargetGeometry(target=t)
+    assert np.all(t == 1)
+    x = np.ones((8, 8, 2))
+    vam.util.data.clipToCircle(x)
+    assert np.all(x == 1)
+
+
+def test_options_defaults_do_not_leak():
+    a = vam.optimize.Options(method="CAL", learning_rate=0.5)
+    b = vam.optimize.Options(method="CAL")
+    assert a.learning_rate == 0.5 and b.learning_rate == 0.01
+    c = vam.optimize.Options(method="BCLP", eps=0.05, p=3)
+    d = vam.optimize.Options(method="BCLP")
+    assert (c.eps, c.p) == (0.05, 3) and (d.eps, d.p, d.q) == (0.1, 2, 1)
+    assert vam.optimize.Options(method="OSMO").inhibition == 0
+
+
+def test_projection_geometry_keywords():
+    pg = vam.geometry.ProjectionGeometry(np.arange(10), "parallel", CUDA=True, inclination_angle=5.0,
+                                         det_offset=2.0, absorption_per_voxel=0.01, n_rot=3)
+    assert pg.n_angles == 10 and pg.inclination_angle == 5.0 and pg.det_offset == 2.0
+    assert pg.absorption_per_voxel == 0.01 and pg.n_rot == 3 and pg.index_model is None
+
+
+def test_unsupported_paths_raise():
+    tg = vam.geometry.TargetGeometry(target=np.ones((8, 8, 2)))
+    with pytest.raises(NotImplementedError):
+        vam.projectorconstructor.projectorconstructor(tg, vam.geometry.ProjectionGeometry(np.arange(4), "algebraic"))
+    with pytest.raises(NotImplementedError):
+        vam.optimize.optimize(tg, vam.geometry.ProjectionGeometry(np.arange(4), "parallel"),
+                              vam.optimize.Options(method="PM"))
+
+
+def test_no_gpu_fails_loudly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    tg = vam.geometry.TargetGeometry(target=np.ones((8, 8, 2)))
+    with pytest.raises(RuntimeError, match="HIP GPU"):
+        vam.projectorconstructor.projectorconstructor(tg, vam.geometry.ProjectionGeometry(np.arange(4), "parallel"))
+
+
+def test_medium_models():
+    cv = vam.geometry.TargetGeometry(target=np.ones((12, 12, 6))).constructCoordVec()
+    im = vam.medium.IndexModel(cv, type="interpolation", form="luneburg_lens")
+    assert im.presampled_scalar_field_3D.shape == (12, 12, 6)
+    assert np.isclose(im.presampled_scalar_field_3D.max(), np.sqrt(2), atol=0.05)
+    am = vam.medium.AttenuationModel(cv, type="analytical", form="homogeneous_cylinder", alpha_internal=2.0)
+    a = am.alpha_at_grid()
+    assert a.max() == 2.0 and a.min() == 0.0
+    hm = vam.medium.IndexModel(cv, type="analytical", form="homogeneous", n_sur=1.4)
+    assert np.all(hm.presampled_scalar_field_3D == 1.4) and np.all(hm.presampled_vector_field_4D == 0)

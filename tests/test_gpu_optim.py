@@ -1,0 +1,190 @@
+"""OSMO / CAL / BCLP on the GPU against the float64 oracle loops (iterates within
+1e-4 relative L2), fused vs unfused vs graph-replayed BCLP, and the autograd
+adjoints (dot-product and finite differences)."""
+
+import numpy as np
+import pytest
+
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _ellipsoids(n, nz, seed, count=4, gray=False):
+    rng = np.random.default_rng(seed)
+    zz, yy, xx = np.meshgrid(np.arange(nz) - (nz - 1) / 2, np.arange(n) - (n - 1) / 2,
+                             np.arange(n) - (n - 1) / 2, indexing="ij")
+    vol = np.zeros((nz, n, n))
+    for _ in range(count):
+        a, b = rng.uniform(0.12, 0.3, 2) * n
+        c = rng.uniform(0.3, 0.6) * nz
+        cx, cy = rng.uniform(-0.15, 0.15, 2) * n
+        cz = rng.uniform(-0.2, 0.2) * nz
+        vol = np.maximum(vol, (((xx - cx) / a) ** 2 + ((yy - cy) / b) ** 2 + ((zz - cz) / c) ** 2 <= 1).astype(float))
+    vol = np.transpose(vol, (1, 2, 0)).copy()
+    if gray:
+        from scipy.ndimage import gaussian_filter
+        vol = np.clip(gaussian_filter(vol, 1.0), 0, 1)
+    return vol
+
+
+def _problem(n=32, nz=4, na=60, seed=0, gray=False, **pg_kw):
+    import vamtoolbox_amd as vam
+    t = _ellipsoids(n, nz, seed, gray=gray)
+    tg = vam.geometry.TargetGeometry(target=t)
+    ang = np.linspace(0, 360, na, endpoint=False)
+    pg = vam.geometry.ProjectionGeometry(ang, "parallel", CUDA=True, **pg_kw)
+    return tg, pg, ang
+
+
+def _oracle_ops(ang, **kw):
+    import oracle.d1 as d1
+    return (lambda v: d1.forward(v, ang, **kw)), (lambda s: d1.backward(s, ang, **kw))
+
+
+def test_osmo_matches_oracle(gpu):
+    import oracle.optim as oo
+    import vamtoolbox_amd as vam
+    tg, pg, ang = _problem()
+    opts = vam.optimize.Options(method="OSMO", n_iter=8, d_h=0.85, d_l=0.6, filter="ram-lak")
+    sino, recon, err = vam.optimize.optimize(tg, pg, opts)
+    fwd, bwd = _oracle_ops(ang)
+    b_ref, x_ref, err_ref = oo.osmo(tg.array, fwd, bwd, 8, d_h=0.85, d_l=0.6)
+    e_b, e_x = rel_l2(sino.array, b_ref), rel_l2(recon.array, x_ref)
+    print("OSMO rel L2 sinogram %.2e recon %.2e" % (e_b, e_x), err, err_ref)
+    assert e_b <= 1e-4 and e_x <= 1e-4
+    np.testing.assert_allclose(err, err_ref, atol=2e-3)
+
+
+def test_cal_matches_oracle(gpu):
+    import oracle.optim as oo
+    import vamtoolbox_amd as vam
+    tg, pg, ang = _problem(seed=1)
+    opts = vam.optimize.Options(method="CAL", n_iter=8, learning_rate=0.002, sigmoid=0.01)
+    sino, recon, err = vam.optimize.optimize(tg, pg, opts)
+    fwd, bwd = _oracle_ops(ang)
+    b_ref, x_ref, err_ref = oo.cal(tg.array, fwd, bwd, 8, lr=0.002, sigmoid=0.01)
+    e_b, e_x = rel_l2(sino.array, b_ref), rel_l2(recon.array, x_ref)
+    print("CAL rel L2 sinogram %.2e recon %.2e" % (e_b, e_x))
+    assert e_b <= 1e-4 and e_x <= 1e-4
+
+
+def _bclp_setup(gpu, kind="analytical", n_iter=20, **pg_kw):
+    import oracle.optim as oo
+    import vamtoolbox_amd as vam
+    tg, pg, ang = _problem(n=32, nz=8, na=45, seed=2, gray=True, **pg_kw)
+    if kind == "analytical":
+        rm = vam.response.ResponseModel(type="analytical", B=20, M=0.6)
+        ro = oo.Response("analytical", B=20, M=0.6)
+    else:
+        rm = vam.response.ResponseModel(B=25)
+        ro = oo.Response("interpolation", B=25)
+    # initial sinogram from the interpolation inverse (finite at zero response)
+    fwd, bwd = _oracle_ops(ang, **{k: v for k, v in (("mu", pg.absorption_per_voxel),
+                                                     ("det_offset", pg.det_offset)) if v})
+    g0 = oo.bclp_init(tg.array, fwd, oo.Response("interpolation", B=20, M=0.6), len(ang))
+    return tg, pg, ang, rm, ro, g0, fwd, bwd
+
+
+@pytest.mark.parametrize("kind", ["analytical", "interpolation"])
+def test_bclp_fused_matches_oracle(gpu, kind):
+    import oracle.optim as oo
+    import vamtoolbox_amd as vam
+    tg, pg, ang, rm, ro, g0, fwd, bwd = _bclp_setup(gpu, kind)
+    n_iter = 20
+    opts = vam.optimize.Options(method="BCLP", n_iter=n_iter, response_model=rm, eps=0.05, p=2, q=1,
+                                learning_rate=0.002, g0=g0, fused=True)
+    g, dose, mapped, logs = vam.optimize.optimize(tg, pg, opts, output="components")
+    g_ref, dose_ref, m_ref, loss_ref = oo.bclp(tg.array, fwd, bwd, ro, n_iter, g0, eps=0.05, lr=0.002)
+    e_g, e_d = rel_l2(g, g_ref), rel_l2(dose, dose_ref)
+    e_l = np.max(np.abs(logs.loss - loss_ref) / np.abs(loss_ref))
+    print("BCLP(%s) rel L2 g %.2e dose %.2e loss %.2e" % (kind, e_g, e_d, e_l))
+    assert e_g <= 1e-4 and e_d <= 1e-4 and e_l <= 1e-4
+
+
+def test_bclp_fused_unfused_graph_agree(gpu):
+    import vamtoolbox_amd as vam
+    tg, pg, ang, rm, ro, g0, fwd, bwd = _bclp_setup(gpu, "analytical", mu=0.01, det_offset=2.0)
+    res = {}
+    for name, kw in (("graph", dict(fused=True, hip_graph=True)), ("eager", dict(fused=True, hip_graph=False)),
+                     ("unfused", dict(fused=False))):
+        opts = vam.optimize.Options(method="BCLP", n_iter=12, response_model=rm, eps=0.05, learning_rate=0.002,
+                                    g0=g0, **kw)
+        res[name] = vam.optimize.optimize(tg, pg, opts, output="components")
+    assert np.array_equal(res["graph"][0], res["eager"][0]), "graph replay must equal eager bitwise"
+    assert np.array_equal(res["graph"][3].loss, res["eager"][3].loss)
+    e = rel_l2(res["unfused"][0], res["eager"][0])
+    print("fused vs unfused rel L2 g %.2e" % e)
+    assert e <= 1e-5
+
+
+def test_bclp_exact_gradient_matches_oracle(gpu):
+    import oracle.d1 as d1
+    import oracle.optim as oo
+    import vamtoolbox_amd as vam
+    tg, pg, ang, rm, ro, g0, fwd, bwd = _bclp_setup(gpu, "analytical")
+    opts = vam.optimize.Options(method="BCLP", n_iter=10, response_model=rm, eps=0.05, learning_rate=0.002,
+                                g0=g0, gradient="exact")
+    g, dose, mapped, logs = vam.optimize.optimize(tg, pg, opts, output="components")
+    g_ref, dose_ref, _, loss_ref = oo.bclp(tg.array, fwd, bwd, ro, 10, g0, eps=0.05, lr=0.002,
+                                           grad_op=lambda v: d1.backward_adjoint(v, ang))
+    e_g = rel_l2(g, g_ref)
+    print("BCLP exact-gradient rel L2 g %.2e" % e_g)
+    assert e_g <= 1e-4
+
+
+def test_bclp_autograd_gradient(gpu):
+    """BCLPLoss backward == oracle exact gradient, and == central finite differences
+    (eps = 1e-3) at 64 random sinogram pixels."""
+    import torch
+    import oracle.d1 as d1
+    import oracle.optim as oo
+    from vamtoolbox_amd.autograd import BCLPLoss
+    from vamtoolbox_amd.projectorconstructor import projectorconstructor
+    tg, pg, ang, rm, ro, g0, fwd, bwd = _bclp_setup(gpu, "analytical")
+    P = projectorconstructor(tg, pg)
+    T = torch.tensor(tg.array, dtype=torch.float32, device=gpu)
+    g = torch.tensor(g0, dtype=torch.float32, device=gpu, requires_grad=True)
+    L = BCLPLoss.apply(g, P, rm, T, 0.05, 1.0, 2.0, 1.0)
+    L.backward()
+    grad = g.grad.double().cpu().numpy()
+    _, _, Lr, operand = oo.bclp_state(g0.astype(np.float32).astype(np.float64), tg.array, bwd, ro, 0.05, 1.0, 2.0, 1.0)
+    grad_ref = (1.0 * Lr ** ((1.0 - 2.0) / 1.0)) * d1.backward_adjoint(operand, ang)
+    print("loss %.6e vs %.6e; grad rel L2 %.2e" % (float(L), Lr, rel_l2(grad, grad_ref)))
+    assert abs(float(L) - Lr) / Lr <= 1e-5
+    assert rel_l2(grad, grad_ref) <= 1e-4
+    rng = np.random.default_rng(0)
+    h = 1e-3
+    g64 = g0.astype(np.float64)
+    worst = 0.0
+    for _ in range(64):
+        ii = tuple(rng.integers(0, s) for s in g0.shape)
+        gp, gm = g64.copy(), g64.copy()
+        gp[ii] += h
+        gm[ii] -= h
+        fd = (oo.bclp_state(gp, tg.array, bwd, ro, 0.05, 1.0, 2.0, 1.0)[2] -
+              oo.bclp_state(gm, tg.array, bwd, ro, 0.05, 1.0, 2.0, 1.0)[2]) / (2 * h)
+        worst = max(worst, abs(fd - grad[ii]) / max(np.abs(grad).max(), 1e-12))
+    print("finite-difference worst error relative to max|grad|: %.2e" % worst)
+    assert worst <= 1e-3
+
+
+def test_autograd_dot_products(gpu):
+    import torch
+    from vamtoolbox_amd.autograd import ParallelBackproject, ParallelProject
+    from vamtoolbox_amd.projectorconstructor import projectorconstructor
+    tg, pg, ang = _problem(n=24, nz=4, na=30, seed=3)
+    P = projectorconstructor(tg, pg)
+    rng = np.random.default_rng(1)
+    x = torch.tensor(rng.random(P.vol_shape), dtype=torch.float32, device=gpu, requires_grad=True)
+    y = torch.tensor(rng.random(P.sino_shape), dtype=torch.float32, device=gpu)
+    (ParallelProject.apply(x, P) * y).sum().backward()
+    lhs = float((P.plan.forward(x.detach()).double() * y.double()).sum())
+    rhs = float((x.detach().double() * x.grad.double()).sum())
+    assert abs(lhs - rhs) / abs(lhs) <= 1e-6
+    s = torch.tensor(rng.random(P.sino_shape), dtype=torch.float32, device=gpu, requires_grad=True)
+    v = torch.tensor(rng.random(P.vol_shape), dtype=torch.float32, device=gpu)
+    (ParallelBackproject.apply(s, P) * v).sum().backward()
+    lhs = float((P.plan.backward(s.detach()).double() * v.double()).sum())
+    rhs = float((s.detach().double() * s.grad.double()).sum())
+    assert abs(lhs - rhs) / abs(lhs) <= 1e-6

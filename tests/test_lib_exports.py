@@ -1,0 +1,53 @@
+"""The C-ABI library builds for gfx950, loads, and exports every symbol include/vamx.h declares (CPU)."""
+
+import os
+import re
+import subprocess
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "vamx.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(vamx_\w+)\s*\(", src, flags=re.M)))
+
+
+def test_header_declares_entry_points():
+    names = _declared()
+    assert "vamx_par_forward" in names and "vamx_rt_deposit" in names and "vamx_bclp_step" in names
+    assert len(names) >= 17
+
+
+def test_library_exports_every_declared_symbol():
+    from vamtoolbox_amd import _lib
+    lib_path = _lib.LIB_PATH
+    assert os.path.exists(lib_path), "run python -m vamtoolbox_amd.build first"
+    out = subprocess.run(["nm", "-D", "--defined-only", lib_path], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\s[TW]\s(vamx_\w+)$", out, flags=re.M))
+    missing = [n for n in _declared() if n not in exported]
+    assert not missing, missing
+    # the ctypes binding knows every declared symbol and the library satisfies it
+    assert set(_lib.SIGNATURES) == set(_declared())
+    _lib.load()
+    assert _lib.MISSING == []
+    assert _lib.load().vamx_version() == 1
+
+
+def test_library_targets_gfx950():
+    from vamtoolbox_amd import _lib
+    out = subprocess.run(["strings", _lib.LIB_PATH], capture_output=True, text=True).stdout
+    assert "gfx950" in out
+
+
+def test_error_path_without_gpu_work():
+    """Argument validation returns a status and a message (no device work needed)."""
+    import ctypes
+    from vamtoolbox_amd import _lib
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    rc = lib.vamx_par_create(None, ctypes.byref(h))
+    assert rc == -1
+    assert b"null" in lib.vamx_last_error()
+    rc = lib.vamx_par_forward(None, None, None, None)
+    assert rc == -3

@@ -1,0 +1,128 @@
+"""Pinning of the filter / response / optimiser oracle and of the host helpers (CPU)."""
+
+import numpy as np
+import pytest
+
+import oracle.optim as oo
+
+
+def test_ramlak_filter_closed_form():
+    """The spatial Ram-Lak kernel's DFT: H(0) = 2*(1/4 - 2*sum_{odd n<P/2} 1/(pi n)^2 - ...) ~ 0
+    and H approaches 2|f| (the ideal ramp) away from DC."""
+    P = 512
+    H = oo.fourier_filter(P)
+    f = np.fft.fftfreq(P)
+    assert abs(H[0]) < 2e-3
+    mid = (np.abs(f) > 0.05) & (np.abs(f) < 0.45)
+    np.testing.assert_allclose(H[mid], 2 * np.abs(f[mid]), rtol=0.02, atol=2e-3)
+    # the kernel is real and even -> the filter is real and symmetric
+    np.testing.assert_allclose(H[1:], H[1:][::-1], atol=1e-12)
+
+
+def test_filter_sinogram_matches_direct_convolution():
+    """Filtering = linear convolution (zero padded) with the spatial kernel h[n]."""
+    rng = np.random.default_rng(0)
+    nR = 20
+    s = rng.random((nR, 3, 2))
+    out = oo.filter_sinogram(s, "ram-lak")
+    P = max(64, int(2 ** np.ceil(np.log2(2 * nR))))
+    h = np.zeros(P)
+    h[0] = 0.25
+    n = np.arange(P)
+    odd = n % 2 == 1
+    m = np.where(n < P // 2, n, P - n)
+    h[odd] = -1 / (np.pi * m[odd]) ** 2
+    h *= 2
+    direct = np.zeros_like(s)
+    for k in range(nR):
+        for j in range(nR):
+            direct[k] += h[(k - j) % P] * s[j]
+    np.testing.assert_allclose(out, direct, atol=1e-12)
+
+
+def test_product_filters_match_oracle_on_host():
+    from vamtoolbox_amd.util import data
+    rng = np.random.default_rng(1)
+    s = rng.random((24, 7, 3))
+    for name in ("ram-lak", "hamming", "hanning", "shepp-logan", "cosine"):
+        np.testing.assert_allclose(data.filterSinogram(s, name), oo.filter_sinogram(s, name), atol=1e-12)
+    x = rng.random((16, 16, 2))
+    np.testing.assert_allclose(data.filterTargetOSMO(x, "ram-lak"), oo.filter_target_osmo(x, "ram-lak"), atol=1e-12)
+
+
+def test_response_closed_forms():
+    from vamtoolbox_amd.response import ResponseModel
+    f = np.linspace(-0.2, 1.2, 301)
+    ra = ResponseModel(type="analytical", B=20, M=0.6)
+    o = oo.Response("analytical", B=20, M=0.6)
+    np.testing.assert_allclose(ra.map(f), 1 / (1 + np.exp(-20 * (f - 0.6))), atol=1e-15)
+    # derivative against central finite differences
+    h = 1e-6
+    fd = (ra.map(f + h) - ra.map(f - h)) / (2 * h)
+    np.testing.assert_allclose(ra.dmapdf(f), fd, rtol=1e-6, atol=1e-9)
+    m = ra.map(f[(f > 0.05) & (f < 1.0)])
+    np.testing.assert_allclose(ra.map_inv(m), f[(f > 0.05) & (f < 1.0)], atol=1e-9)
+    np.testing.assert_allclose(o.map(f), ra.map(f), atol=1e-15)
+    ri = ResponseModel()  # reference default: 512-point table, B=25, M=0.5
+    oi = oo.Response("interpolation")
+    np.testing.assert_allclose(ri.map(f), oi.map(f), atol=0)
+    np.testing.assert_allclose(ri.dmapdf(f), oi.dmap(f), atol=0)
+    np.testing.assert_allclose(ri.map_inv(ri.map(f)), oi.map_inv(oi.map(f)), atol=0)
+    # nearest extrapolation outside [0, 1]
+    assert ri.map(np.array([-5.0]))[0] == ri.interp_map_0[0]
+
+
+def test_response_torch_matches_numpy():
+    import torch
+    from vamtoolbox_amd.response import ResponseModel
+    f = np.linspace(-0.3, 1.3, 1001)
+    for rm in (ResponseModel(), ResponseModel(type="analytical", B=20, M=0.6),
+               ResponseModel(type="analytical", form="linear", M=2.0, C=0.1)):
+        t = torch.tensor(f, dtype=torch.float64)
+        np.testing.assert_allclose(rm.map(t).numpy(), rm.map(f), atol=1e-15)
+        np.testing.assert_allclose(rm.dmapdf(t).numpy(), rm.dmapdf(f), atol=1e-15)
+
+
+def test_bclp_gradient_finite_differences_oracle():
+    """The BCLP operand pushed through the exact transpose of the dose operator is the
+    gradient of the loss: check against central finite differences (eps = 1e-3) on the
+    float64 oracle with an explicit projector matrix."""
+    import oracle.d1 as d1
+    rng = np.random.default_rng(2)
+    n, nz = 8, 2
+    ang = np.arange(0.0, 180.0, 20.0)
+    sh_v, sh_s = (n, n, nz), (n, len(ang), nz)
+    target = np.clip(rng.random(sh_v), 0, 1) * 0.8
+    resp = oo.Response("analytical", B=20, M=0.6)
+    bwd = lambda s: d1.backward(s, ang)
+    bT = lambda v: d1.backward_adjoint(v, ang)
+    g = rng.uniform(0.0, 0.2, sh_s)
+    p, q, eps = 2.0, 1.0, 0.05
+    _, _, L, operand = oo.bclp_state(g, target, bwd, resp, eps, 1.0, p, q)
+    grad = (q * L ** ((q - p) / q)) * bT(operand)
+    idx = [tuple(rng.integers(0, s) for s in sh_s) for _ in range(12)]
+    h = 1e-3
+    for ii in idx:
+        gp, gm = g.copy(), g.copy()
+        gp[ii] += h
+        gm[ii] -= h
+        Lp = oo.bclp_state(gp, target, bwd, resp, eps, 1.0, p, q)[2]
+        Lm = oo.bclp_state(gm, target, bwd, resp, eps, 1.0, p, q)[2]
+        fd = (Lp - Lm) / (2 * h)
+        assert abs(fd - grad[ii]) <= 2e-3 * max(1.0, abs(grad[ii])), (ii, fd, grad[ii])
+
+
+def test_osmo_cal_oracle_tiny_run():
+    """The oracle loops run and reduce VER on a tiny problem (explicit projector)."""
+    import oracle.d1 as d1
+    n = 16
+    yy, xx = np.mgrid[:n, :n] - (n - 1) / 2
+    t = ((xx / 5) ** 2 + (yy / 3.5) ** 2 <= 1).astype(float)[:, :, None]
+    t = d1.clip_to_circle(t)
+    ang = np.arange(0.0, 360.0, 10.0)
+    fwd = lambda v: d1.forward(v, ang)
+    bwd = lambda s: d1.backward(s, ang)
+    _, _, err = oo.osmo(t, fwd, bwd, 6)
+    assert err[-1] <= err[0]
+    _, _, err2 = oo.cal(t, fwd, bwd, 6, lr=0.002)
+    assert np.all(np.isfinite(err2))

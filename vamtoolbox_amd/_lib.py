@@ -63,6 +63,7 @@ class BclpParams(ctypes.Structure):
         ("glb", c_double),
         ("gub", c_double),
         ("dvol", c_double),
+        ("grad_op", c_int32),
     ]
 
 

@@ -1,0 +1,181 @@
+// Batched parallel-beam projection of many independent problems of uneven
+// size in one persistent launch (gfx950).
+//
+// Each problem is a parallel plan (vamx_par_create) with its own volume and
+// sinogram.  The flattened work of every problem is cut into tiles of 256
+// (ray or voxel column, z-vector) items; tiles are ordered largest problem
+// first and handed out by a device-side counter, so the resident blocks of
+// the single launch drain one queue until it is empty (every block exits
+// when the counter passes the tile count).  The counter is reset by a
+// one-thread kernel on the same stream, so the sequence is graph-capturable.
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+#include "par_core.h"
+
+namespace vamx {
+
+struct BatchItem {
+  ParDev g;
+  const float* in;
+  float* out;
+  const float* minmax;
+  long long tile_begin;  // first tile of this problem
+  long long items;       // work items of this problem
+  int zv;                // 4 or 1
+  int clip;              // forward: warp clip; backward: circle clip
+};
+
+__global__ void batch_reset(unsigned long long* counter) { *counter = 0ull; }
+
+template <bool FORWARD>
+__global__ __launch_bounds__(256) void batch_kernel(const BatchItem* __restrict__ items, int nitems,
+                                                    long long ntiles, unsigned long long* counter) {
+  __shared__ long long tile_sh;
+  for (;;) {
+    if (threadIdx.x == 0) tile_sh = (long long)atomicAdd(counter, 1ull);
+    __syncthreads();
+    const long long tile = tile_sh;
+    __syncthreads();
+    if (tile >= ntiles) return;
+    // problem owning this tile (items are ordered by tile_begin)
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (items[mid].tile_begin <= tile) lo = mid; else hi = mid - 1;
+    }
+    const BatchItem& it = items[lo];
+    const ParDev& g = it.g;
+    const long long w = (tile - it.tile_begin) * 256 + threadIdx.x;
+    if (w >= it.items) continue;
+    const int nzv = g.nz / it.zv;
+    const int z = (int)(w % nzv) * it.zv;
+    const long long outer = w / nzv;
+    if (FORWARD) {
+      const int k = (int)(outer % g.n), i = (int)(outer / g.n);
+      float* dst = it.out + ((size_t)k * g.na + i) * g.nz + z;
+      if (g.incl) {
+        *dst = fp_ray_incl(g, it.in, k, i, z);
+      } else if (it.zv == 4) {
+        vstore<4>(dst, it.clip ? fp_ray<4, true>(g, it.in, it.minmax, k, i, z)
+                               : fp_ray<4, false>(g, it.in, it.minmax, k, i, z));
+      } else {
+        *dst = it.clip ? fp_ray<1, true>(g, it.in, it.minmax, k, i, z)
+                       : fp_ray<1, false>(g, it.in, it.minmax, k, i, z);
+      }
+    } else {
+      const int pix = (int)outer;
+      float* dst = it.out + (size_t)pix * g.nz + z;
+      const bool zero = it.clip && g.outside[pix];
+      if (g.incl) {
+        *dst = zero ? 0.0f : bp_voxel_incl(g, it.in, pix, z);
+      } else if (it.zv == 4) {
+        vstore<4>(dst, zero ? vzero<4>() : bp_voxel<4>(g, it.in, pix, z));
+      } else {
+        *dst = zero ? 0.0f : bp_voxel<1>(g, it.in, pix, z);
+      }
+    }
+  }
+}
+
+namespace {
+
+struct BatchScratch {
+  BatchItem* d_items = nullptr;
+  size_t cap = 0;
+  unsigned long long* d_counter = nullptr;
+  int device = -1;
+  std::vector<char> last;
+};
+
+thread_local BatchScratch g_scratch;
+
+int run_batched(vamx_plan* const* plans, int n, const float* const* ins, float* const* outs, hipStream_t s,
+                bool forward) {
+  if (!plans || n < 1 || !ins || !outs) return set_error(VAMX_ERR_ARG, "batched: empty or null batch");
+  int dev = 0;
+  VAMX_HIP(hipGetDevice(&dev));
+  std::vector<BatchItem> items((size_t)n);
+  std::vector<int> order((size_t)n);
+  for (int i = 0; i < n; ++i) {
+    vamx_plan* p = plans[i];
+    if (!check_plan(p, PLAN_PARALLEL)) return set_error(VAMX_ERR_STATE, "batched: not a parallel plan");
+    if (p->device != dev) return set_error(VAMX_ERR_ARG, "batched: plans must live on the current device");
+    if (!ins[i] || !outs[i]) return set_error(VAMX_ERR_ARG, "batched: null buffer");
+    BatchItem& it = items[(size_t)i];
+    it.g = p->par;
+    it.in = ins[i];
+    it.out = outs[i];
+    it.minmax = p->d_minmax;
+    it.zv = par_use_vec4(p, ins[i], outs[i]) ? 4 : 1;
+    it.clip = forward ? !(p->flags & VAMX_PAR_NO_WARP_CLIP) : !(p->flags & VAMX_PAR_NO_CIRCLE_CLIP);
+    const long long outer = forward ? (long long)it.g.na * it.g.n : (long long)it.g.n * it.g.n;
+    it.items = outer * (it.g.nz / it.zv);
+    if (forward && it.clip && !it.g.incl) {
+      int rc = par_slice_minmax(p, ins[i], s);
+      if (rc) return rc;
+    }
+  }
+  // largest work first (cost ~ items * inner loop length)
+  std::iota(order.begin(), order.end(), 0);
+  auto cost = [&](int i) {
+    const BatchItem& it = items[(size_t)i];
+    return (double)it.items * it.zv * (forward ? it.g.n : it.g.na);
+  };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+  std::vector<BatchItem> sorted((size_t)n);
+  long long tiles = 0;
+  for (int r = 0; r < n; ++r) {
+    sorted[(size_t)r] = items[(size_t)order[(size_t)r]];
+    sorted[(size_t)r].tile_begin = tiles;
+    tiles += (sorted[(size_t)r].items + 255) / 256;
+  }
+  BatchScratch& sc = g_scratch;
+  if (sc.device != dev || sc.cap < (size_t)n) {
+    if (sc.d_items) hipFree(sc.d_items);
+    if (sc.d_counter) hipFree(sc.d_counter);
+    sc.d_items = nullptr;
+    sc.d_counter = nullptr;
+    sc.cap = std::max<size_t>((size_t)n, 64);
+    VAMX_HIP(hipMalloc((void**)&sc.d_items, sc.cap * sizeof(BatchItem)));
+    VAMX_HIP(hipMalloc((void**)&sc.d_counter, sizeof(unsigned long long)));
+    sc.device = dev;
+    sc.last.clear();
+  }
+  // the item table is uploaded only when it differs from the previous call
+  // (a blocking copy); repeated calls with the same problems therefore enqueue
+  // only the two kernels below and can be captured in a HIP graph
+  const size_t bytes = (size_t)n * sizeof(BatchItem);
+  if (sc.last.size() != bytes || std::memcmp(sc.last.data(), sorted.data(), bytes) != 0) {
+    VAMX_HIP(hipStreamSynchronize(s));
+    VAMX_HIP(hipMemcpy(sc.d_items, sorted.data(), bytes, hipMemcpyHostToDevice));
+    sc.last.assign((const char*)sorted.data(), (const char*)sorted.data() + bytes);
+  }
+  hipLaunchKernelGGL(batch_reset, dim3(1), dim3(1), 0, s, sc.d_counter);
+  int cus = 0;
+  VAMX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  long long grid = (long long)std::max(cus, 1) * 8;
+  if (grid > tiles) grid = tiles;
+  if (forward)
+    hipLaunchKernelGGL(batch_kernel<true>, dim3((unsigned)grid), dim3(256), 0, s, sc.d_items, n, tiles,
+                       sc.d_counter);
+  else
+    hipLaunchKernelGGL(batch_kernel<false>, dim3((unsigned)grid), dim3(256), 0, s, sc.d_items, n, tiles,
+                       sc.d_counter);
+  VAMX_HIP(hipGetLastError());
+  return VAMX_OK;
+}
+
+}  // namespace
+}  // namespace vamx
+
+extern "C" int vamx_par_forward_batched(vamx_plan* const* plans, int32_t n_plans, const float* const* vols,
+                                        float* const* sinos, void* stream) {
+  return vamx::run_batched(plans, n_plans, vols, sinos, (hipStream_t)stream, true);
+}
+
+extern "C" int vamx_par_backward_batched(vamx_plan* const* plans, int32_t n_plans, const float* const* sinos,
+                                         float* const* vols, void* stream) {
+  return vamx::run_batched(plans, n_plans, sinos, vols, (hipStream_t)stream, false);
+}

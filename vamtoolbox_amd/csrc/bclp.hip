@@ -1,0 +1,370 @@
+// Fused BCLP iteration (band-constraint Lp-norm sinogram optimisation) for gfx950.
+//
+// Reference: vamtoolbox/optimizer/BCLP.py, BCLPNorm.updateVariables /
+// computeLoss / computeLossGradient (277-386) and gradientDescent (477-534);
+// material response vamtoolbox/response.py:191-291.
+//
+// One iteration is three kernels on one stream and no host synchronisation:
+//   A  back projection of g fused with the response map, band error, Lp
+//      weighting and a per-block partial of the loss (fixed-order tree sum);
+//   B  one block reduces the partials in a fixed order, logs the loss, and
+//      derives the gradient scale q L^((q-p)/q);
+//   C  forward projection of the operand (or the exact transpose of the back
+//      projection) fused with g <- clip(g - lr * scale * grad, glb, gub).
+// The per-voxel epilogue runs in float64, as the reference does in NumPy.
+#include "par_core.h"
+
+namespace vamx {
+
+struct RespDev {
+  int kind;
+  double A, K, B, M, nu, lin_M, lin_C;
+  int n_pts;
+  const double* tf;
+  const double* tm;
+  const double* td;
+};
+
+struct BclpDev {
+  double p, q, eps, weight, dvol;
+  const float* eps_arr;
+  const float* w_arr;
+};
+
+// np.interp(x, xp, fp, left, right) for increasing xp (numpy's branch order)
+__device__ double np_interp(double x, const double* __restrict__ xp, const double* __restrict__ fp, int n,
+                            double left, double right) {
+  if (x > xp[n - 1]) return right;
+  if (x < xp[0]) return left;
+  if (x == xp[n - 1]) return fp[n - 1];
+  int lo = 0, hi = n - 1;  // xp[lo] <= x < xp[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (xp[mid] <= x) lo = mid; else hi = mid;
+  }
+  if (xp[lo] == x) return fp[lo];
+  const double slope = (fp[lo + 1] - fp[lo]) / (xp[lo + 1] - xp[lo]);
+  return slope * (x - xp[lo]) + fp[lo];
+}
+
+// response map and derivative (response.py:191-218, 233-253, 256-279)
+__device__ void response_eval(const RespDev& r, double f, double& m, double& dm) {
+  if (r.kind == VAMX_RESP_GLF) {
+    const double e = exp(-r.B * (f - r.M));
+    m = r.A + (r.K - r.A) / pow(1.0 + e, 1.0 / r.nu);
+    const double c1 = pow(1.0 / (r.K - r.A), r.nu);
+    const double c2 = r.B / r.nu;
+    const double c3 = pow(m - r.A, r.nu + 1.0);
+    dm = c1 * c2 * c3 * e;
+  } else if (r.kind == VAMX_RESP_LINEAR) {
+    m = r.lin_M * f + r.lin_C;
+    dm = r.lin_M;
+  } else if (r.kind == VAMX_RESP_IDENTITY) {
+    m = f;
+    dm = 1.0;
+  } else {
+    const int n = r.n_pts;
+    m = np_interp(f, r.tf, r.tm, n, r.tm[0], r.tm[n - 1]);
+    dm = np_interp(f, r.tf, r.td, n, r.td[0], r.td[n - 1]);
+  }
+}
+
+// BCLP epilogue for one voxel: returns the loss integrand, writes the operand
+__device__ __forceinline__ double bclp_voxel(const RespDev& r, const BclpDev& b, float dose, float target,
+                                             size_t idx, float& operand) {
+  double m, dm;
+  response_eval(r, (double)dose, m, dm);
+  const double eT = m - (double)target;                       // BCLP.py:298-300
+  const double eps = b.eps_arr ? (double)b.eps_arr[idx] : b.eps;
+  const double eb = fabs(eT) - eps;                            // 305-307
+  const bool v = eb > 0.0;                                     // 311
+  if (!v) {
+    operand = 0.0f;
+    return 0.0;
+  }
+  const double w = b.w_arr ? (double)b.w_arr[idx] : b.weight;
+  const double sgn = eT > 0.0 ? 1.0 : (eT < 0.0 ? -1.0 : 0.0);
+  operand = (float)(w * pow(fabs(eb), b.p - 1.0) * sgn * dm);  // 361-367
+  return w * pow(fabs(eb), b.p);                               // 340-342
+}
+
+__device__ __forceinline__ double block_sum_256(double v, double* sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) sh[t] += sh[t + s];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+// A: back projection + response + band/Lp epilogue + loss partials
+template <int ZV, bool INCL>
+__global__ __launch_bounds__(256) void bclp_bp_epilogue(ParDev g, int clipc, RespDev r, BclpDev b,
+                                                        const float* __restrict__ gsino,
+                                                        const float* __restrict__ target,
+                                                        float* __restrict__ dose, float* __restrict__ operand,
+                                                        double* __restrict__ partials) {
+  __shared__ double sh[256];
+  const int nzv = g.nz / ZV;
+  const long long total = (long long)g.n * g.n * nzv;
+  const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
+  const long long tid = (long long)lb * blockDim.x + threadIdx.x;
+  double part = 0.0;
+  if (tid < total) {
+    const int z = (int)(tid % nzv) * ZV;
+    const int pix = (int)(tid / nzv);
+    const size_t base = (size_t)pix * g.nz + z;
+    typename VecT<ZV>::T d = vzero<ZV>();
+    if (!(clipc && g.outside[pix])) {
+      if constexpr (INCL) d = bp_voxel_incl(g, gsino, pix, z);
+      else d = bp_voxel<ZV>(g, gsino, pix, z);
+    }
+    vstore<ZV>(dose + base, d);
+    typename VecT<ZV>::T op = vzero<ZV>();
+    const typename VecT<ZV>::T tg = vload<ZV>(target + base);
+#pragma unroll
+    for (int q = 0; q < ZV; ++q) {
+      float o;
+      part += bclp_voxel(r, b, vget(d, q), vget(tg, q), base + q, o);
+      vset(op, q, o);
+    }
+    vstore<ZV>(operand + base, op);
+  }
+  const double s = block_sum_256(part, sh);
+  if (threadIdx.x == 0) partials[lb] = s;
+}
+
+// B: fixed-order reduction of the partials, loss log and gradient scale
+__global__ __launch_bounds__(256) void bclp_reduce(const double* __restrict__ partials, int nparts, double dvol,
+                                                   double p, double q, double* loss_dst, int32_t* counter,
+                                                   double* loss_log, double* scale_out) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += 256) acc += partials[i];
+  const double s = block_sum_256(acc, sh);
+  if (threadIdx.x == 0) {
+    const double L = pow(s * dvol, q / p);                     // BCLP.py:345-347
+    if (counter) {
+      loss_log[*counter] = L;
+      *counter = *counter + 1;
+    } else {
+      *loss_dst = L;
+    }
+    *scale_out = (L != 0.0) ? q * pow(L, (q - p) / q) : 1.0;    // 370-375
+  }
+}
+
+// C: forward projection of the operand fused with the sinogram update
+template <int ZV, bool INCL, bool CLIP>
+__global__ __launch_bounds__(256) void bclp_fp_update(ParDev g, const float* __restrict__ operand,
+                                                      float* __restrict__ gsino, const float* __restrict__ minmax,
+                                                      const double* __restrict__ scale_p, float lr, float glb,
+                                                      float gub) {
+  const int nzv = g.nz / ZV;
+  const long long total = (long long)g.na * g.n * nzv;
+  const long long tid = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (tid >= total) return;
+  const int z = (int)(tid % nzv) * ZV;
+  const long long ray = tid / nzv;
+  const int k = (int)(ray % g.n), i = (int)(ray / g.n);
+  typename VecT<ZV>::T fp;
+  if constexpr (INCL) fp = fp_ray_incl(g, operand, k, i, z);
+  else fp = fp_ray<ZV, CLIP>(g, operand, minmax, k, i, z);
+  const float scale = (float)(*scale_p);
+  float* gp = gsino + ((size_t)k * g.na + i) * g.nz + z;
+  typename VecT<ZV>::T gv = vload<ZV>(gp);
+#pragma unroll
+  for (int q = 0; q < ZV; ++q) {
+    const float grad = scale * vget(fp, q);
+    vset(gv, q, clampf(vget(gv, q) - lr * grad, glb, gub));  // BCLP.py:486-487, 532
+  }
+  vstore<ZV>(gp, gv);
+}
+
+// C': exact transpose of the back projection fused with the sinogram update
+template <int ZV, bool INCL>
+__global__ __launch_bounds__(256) void bclp_bpT_update(ParDev g, int clipc, const float* __restrict__ operand,
+                                                       float* __restrict__ gsino, const double* __restrict__ scale_p,
+                                                       float lr, float glb, float gub) {
+  const int nzv = g.nz / ZV;
+  const long long total = (long long)g.n * g.na * nzv;
+  const long long tid = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (tid >= total) return;
+  const int z = (int)(tid % nzv) * ZV;
+  const long long bin = tid / nzv;
+  const int i = (int)(bin % g.na), k = (int)(bin / g.na);
+  typename VecT<ZV>::T gr;
+  if constexpr (INCL) gr = bpT_bin_incl(g, operand, k, i, z, clipc);
+  else gr = bpT_bin<ZV>(g, operand, k, i, z, clipc);
+  const float scale = (float)(*scale_p);
+  float* gp = gsino + ((size_t)k * g.na + i) * g.nz + z;
+  typename VecT<ZV>::T gv = vload<ZV>(gp);
+#pragma unroll
+  for (int q = 0; q < ZV; ++q) vset(gv, q, clampf(vget(gv, q) - lr * (scale * vget(gr, q)), glb, gub));
+  vstore<ZV>(gp, gv);
+}
+
+}  // namespace vamx
+
+using namespace vamx;
+
+namespace {
+
+struct WsLayout {
+  size_t partials_off, scale_off, operand_off, total;
+  int nparts;
+};
+
+WsLayout ws_layout(const vamx_plan* p) {
+  const ParDev& g = p->par;
+  WsLayout w;
+  const long long items = (long long)g.n * g.n * g.nz;  // worst case ZV = 1
+  w.nparts = (int)blocks_for(items, 256);
+  w.partials_off = 0;
+  w.scale_off = ((size_t)w.nparts * sizeof(double) + 255) / 256 * 256;
+  w.operand_off = w.scale_off + 256;
+  w.total = w.operand_off + (size_t)items * sizeof(float);
+  return w;
+}
+
+int fill_devs(const vamx_response* resp, const vamx_bclp_params* prm, const float* eps_arr, const float* w_arr,
+              RespDev& r, BclpDev& b) {
+  if (!resp || !prm) return set_error(VAMX_ERR_ARG, "bclp: null response or params");
+  if (resp->kind < VAMX_RESP_GLF || resp->kind > VAMX_RESP_TABLE)
+    return set_error(VAMX_ERR_ARG, "bclp: unknown response kind");
+  if (resp->kind == VAMX_RESP_TABLE && (resp->n_pts < 2 || !resp->table_f || !resp->table_map || !resp->table_dmap))
+    return set_error(VAMX_ERR_ARG, "bclp: interpolation response needs n_pts >= 2 and three device tables");
+  if (!(prm->p > 0.0) || !(prm->q > 0.0)) return set_error(VAMX_ERR_ARG, "bclp: p and q must be > 0");
+  r.kind = resp->kind;
+  r.A = resp->A; r.K = resp->K; r.B = resp->B; r.M = resp->M; r.nu = resp->nu;
+  r.lin_M = resp->lin_M; r.lin_C = resp->lin_C;
+  r.n_pts = resp->n_pts;
+  r.tf = resp->table_f; r.tm = resp->table_map; r.td = resp->table_dmap;
+  b.p = prm->p; b.q = prm->q; b.eps = prm->eps; b.weight = prm->weight; b.dvol = prm->dvol;
+  b.eps_arr = eps_arr;
+  b.w_arr = w_arr;
+  return VAMX_OK;
+}
+
+int launch_epilogue(vamx_plan* p, const RespDev& r, const BclpDev& b, const float* g, const float* target,
+                    float* dose, float* operand, double* partials, int& nparts, hipStream_t s) {
+  const ParDev& gd = p->par;
+  const int clipc = (p->flags & VAMX_PAR_NO_CIRCLE_CLIP) ? 0 : 1;
+  const bool v4 = !gd.incl && gd.nz % 4 == 0 && ((uintptr_t)g | (uintptr_t)target | (uintptr_t)dose |
+                                                 (uintptr_t)operand) % 16 == 0 &&
+                  (!b.eps_arr || (uintptr_t)b.eps_arr % 16 == 0) && (!b.w_arr || (uintptr_t)b.w_arr % 16 == 0);
+  const long long total = (long long)gd.n * gd.n * (gd.nz / (v4 ? 4 : 1));
+  nparts = (int)blocks_for(total, 256);
+  const dim3 grid(nparts), block(256);
+  if (gd.incl)
+    hipLaunchKernelGGL((bclp_bp_epilogue<1, true>), grid, block, 0, s, gd, clipc, r, b, g, target, dose, operand,
+                       partials);
+  else if (v4)
+    hipLaunchKernelGGL((bclp_bp_epilogue<4, false>), grid, block, 0, s, gd, clipc, r, b, g, target, dose, operand,
+                       partials);
+  else
+    hipLaunchKernelGGL((bclp_bp_epilogue<1, false>), grid, block, 0, s, gd, clipc, r, b, g, target, dose, operand,
+                       partials);
+  VAMX_HIP(hipGetLastError());
+  return VAMX_OK;
+}
+
+}  // namespace
+
+extern "C" int vamx_bclp_workspace_bytes(vamx_plan* plan, int64_t* bytes) {
+  if (!check_plan(plan, PLAN_PARALLEL)) return set_error(VAMX_ERR_STATE, "vamx_bclp_workspace_bytes: not a parallel plan");
+  if (!bytes) return set_error(VAMX_ERR_ARG, "vamx_bclp_workspace_bytes: null output");
+  *bytes = (int64_t)ws_layout(plan).total;
+  return VAMX_OK;
+}
+
+extern "C" int vamx_bclp_step(vamx_plan* plan, const vamx_response* resp, const vamx_bclp_params* prm, float* g,
+                              const float* target, const float* eps_arr, const float* weight_arr, float* dose,
+                              double* loss_log, int32_t* iter_counter, void* workspace, void* stream) {
+  if (!check_plan(plan, PLAN_PARALLEL)) return set_error(VAMX_ERR_STATE, "vamx_bclp_step: not a parallel plan");
+  if (!g || !target || !dose || !loss_log || !iter_counter || !workspace)
+    return set_error(VAMX_ERR_ARG, "vamx_bclp_step: null buffer");
+  RespDev r;
+  BclpDev b;
+  int rc = fill_devs(resp, prm, eps_arr, weight_arr, r, b);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const WsLayout w = ws_layout(plan);
+  char* ws = (char*)workspace;
+  double* partials = (double*)(ws + w.partials_off);
+  double* scale = (double*)(ws + w.scale_off);
+  float* operand = (float*)(ws + w.operand_off);
+  int nparts = 0;
+  rc = launch_epilogue(plan, r, b, g, target, dose, operand, partials, nparts, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(bclp_reduce, dim3(1), dim3(256), 0, s, partials, nparts, b.dvol, b.p, b.q, (double*)nullptr,
+                     iter_counter, loss_log, scale);
+  VAMX_HIP(hipGetLastError());
+  const ParDev& gd = plan->par;
+  const float lr = (float)prm->learning_rate, glb = (float)prm->glb, gub = (float)prm->gub;
+  const bool v4 = !gd.incl && gd.nz % 4 == 0 && ((uintptr_t)g | (uintptr_t)operand) % 16 == 0;
+  const int zv = v4 ? 4 : 1;
+  if (prm->grad_op == 0) {
+    const bool clip = !(plan->flags & VAMX_PAR_NO_WARP_CLIP);
+    if (clip && !gd.incl) {
+      rc = par_slice_minmax(plan, operand, s);
+      if (rc) return rc;
+    }
+    const long long total = (long long)gd.na * gd.n * (gd.nz / zv);
+    const dim3 grid(blocks_for(total, 256)), block(256);
+    if (gd.incl)
+      hipLaunchKernelGGL((bclp_fp_update<1, true, false>), grid, block, 0, s, gd, operand, g, plan->d_minmax, scale,
+                         lr, glb, gub);
+    else if (v4 && clip)
+      hipLaunchKernelGGL((bclp_fp_update<4, false, true>), grid, block, 0, s, gd, operand, g, plan->d_minmax, scale,
+                         lr, glb, gub);
+    else if (v4)
+      hipLaunchKernelGGL((bclp_fp_update<4, false, false>), grid, block, 0, s, gd, operand, g, plan->d_minmax, scale,
+                         lr, glb, gub);
+    else if (clip)
+      hipLaunchKernelGGL((bclp_fp_update<1, false, true>), grid, block, 0, s, gd, operand, g, plan->d_minmax, scale,
+                         lr, glb, gub);
+    else
+      hipLaunchKernelGGL((bclp_fp_update<1, false, false>), grid, block, 0, s, gd, operand, g, plan->d_minmax, scale,
+                         lr, glb, gub);
+  } else {
+    const int clipc = (plan->flags & VAMX_PAR_NO_CIRCLE_CLIP) ? 0 : 1;
+    const long long total = (long long)gd.n * gd.na * (gd.nz / zv);
+    const dim3 grid(blocks_for(total, 256)), block(256);
+    if (gd.incl)
+      hipLaunchKernelGGL((bclp_bpT_update<1, true>), grid, block, 0, s, gd, clipc, operand, g, scale, lr, glb, gub);
+    else if (v4)
+      hipLaunchKernelGGL((bclp_bpT_update<4, false>), grid, block, 0, s, gd, clipc, operand, g, scale, lr, glb, gub);
+    else
+      hipLaunchKernelGGL((bclp_bpT_update<1, false>), grid, block, 0, s, gd, clipc, operand, g, scale, lr, glb, gub);
+  }
+  VAMX_HIP(hipGetLastError());
+  return VAMX_OK;
+}
+
+extern "C" int vamx_bclp_eval(vamx_plan* plan, const vamx_response* resp, const vamx_bclp_params* prm,
+                              const float* g, const float* target, const float* eps_arr, const float* weight_arr,
+                              float* dose, float* operand, double* loss_out, void* workspace, void* stream) {
+  if (!check_plan(plan, PLAN_PARALLEL)) return set_error(VAMX_ERR_STATE, "vamx_bclp_eval: not a parallel plan");
+  if (!g || !target || !dose || !loss_out || !workspace)
+    return set_error(VAMX_ERR_ARG, "vamx_bclp_eval: null buffer");
+  RespDev r;
+  BclpDev b;
+  int rc = fill_devs(resp, prm, eps_arr, weight_arr, r, b);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const WsLayout w = ws_layout(plan);
+  char* ws = (char*)workspace;
+  double* partials = (double*)(ws + w.partials_off);
+  double* scale = (double*)(ws + w.scale_off);
+  float* op = operand ? operand : (float*)(ws + w.operand_off);
+  int nparts = 0;
+  rc = launch_epilogue(plan, r, b, g, target, dose, op, partials, nparts, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(bclp_reduce, dim3(1), dim3(256), 0, s, partials, nparts, b.dvol, b.p, b.q, loss_out,
+                     (int32_t*)nullptr, (double*)nullptr, scale);
+  VAMX_HIP(hipGetLastError());
+  return VAMX_OK;
+}

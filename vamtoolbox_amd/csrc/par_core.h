@@ -1,0 +1,438 @@
+// Device-level cores of the parallel-beam operators, shared by the plain
+// kernels (parallel.hip), the fused BCLP kernels (bclp.hip) and the batched
+// launch (batched.hip).
+//
+// Discretisation: the reference's CPU projector "D1"
+// (vamtoolbox/projector/Projector3DParallel.py:113-207) plus the detector
+// offset / absorption / inclination extensions.  Layouts: volume (n, n, nz)
+// and sinogram (n, n_angles, nz), both z-contiguous; a lane owns one ray or
+// voxel column and ZV consecutive z values, so sample weights are identical
+// across a wavefront and every tap is a coalesced load.
+//
+// Geometry (sample coordinates, floor/ceil decisions, weights) is evaluated
+// in float64 with the reference's operation order (translation units are
+// compiled with -ffp-contract=off) so that every discontinuous decision
+// matches the reference; data arithmetic is float32 with explicit fused
+// multiply-adds.  No core uses atomics.
+#pragma once
+
+#include "vamx_internal.h"
+
+namespace vamx {
+
+template <int ZV> struct VecT;
+template <> struct VecT<1> { using T = float; };
+template <> struct VecT<4> { using T = float4; };
+
+template <int ZV> __device__ __forceinline__ typename VecT<ZV>::T vzero();
+template <> __device__ __forceinline__ float vzero<1>() { return 0.0f; }
+template <> __device__ __forceinline__ float4 vzero<4>() { return make_float4(0.f, 0.f, 0.f, 0.f); }
+
+template <int ZV> __device__ __forceinline__ typename VecT<ZV>::T vload(const float* p);
+template <> __device__ __forceinline__ float vload<1>(const float* p) { return *p; }
+template <> __device__ __forceinline__ float4 vload<4>(const float* p) { return *reinterpret_cast<const float4*>(p); }
+
+template <int ZV> __device__ __forceinline__ void vstore(float* p, typename VecT<ZV>::T v);
+template <> __device__ __forceinline__ void vstore<1>(float* p, float v) { *p = v; }
+template <> __device__ __forceinline__ void vstore<4>(float* p, float4 v) { *reinterpret_cast<float4*>(p) = v; }
+
+__device__ __forceinline__ float vfma(float a, float x, float y) { return __builtin_fmaf(a, x, y); }
+__device__ __forceinline__ float4 vfma(float a, float4 x, float4 y) {
+  return make_float4(__builtin_fmaf(a, x.x, y.x), __builtin_fmaf(a, x.y, y.y), __builtin_fmaf(a, x.z, y.z),
+                     __builtin_fmaf(a, x.w, y.w));
+}
+__device__ __forceinline__ float vmul(float a, float x) { return a * x; }
+__device__ __forceinline__ float4 vmul(float a, float4 x) { return make_float4(a * x.x, a * x.y, a * x.z, a * x.w); }
+__device__ __forceinline__ float vsub(float a, float b) { return a - b; }
+__device__ __forceinline__ float4 vsub(float4 a, float4 b) {
+  return make_float4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w);
+}
+__device__ __forceinline__ float vadd(float a, float b) { return a + b; }
+__device__ __forceinline__ float4 vadd(float4 a, float4 b) {
+  return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+}
+__device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+__device__ __forceinline__ float vclamp(float v, float lo, float hi) { return clampf(v, lo, hi); }
+__device__ __forceinline__ float4 vclamp(float4 v, float4 lo, float4 hi) {
+  return make_float4(clampf(v.x, lo.x, hi.x), clampf(v.y, lo.y, hi.y), clampf(v.z, lo.z, hi.z),
+                     clampf(v.w, lo.w, hi.w));
+}
+__device__ __forceinline__ float vget(float v, int) { return v; }
+__device__ __forceinline__ float vget(float4 v, int q) { return q == 0 ? v.x : (q == 1 ? v.y : (q == 2 ? v.z : v.w)); }
+__device__ __forceinline__ void vset(float& v, int, float x) { v = x; }
+__device__ __forceinline__ void vset(float4& v, int q, float x) {
+  if (q == 0) v.x = x; else if (q == 1) v.y = x; else if (q == 2) v.z = x; else v.w = x;
+}
+
+// absorption weight of a sample at lateral coordinate t and along-ray s:
+// exp(-mu * max(0, sqrt(R^2 - t^2) - s) / cos(phi)) inside |t| < R, else 0
+// (per-ray form of astra3Dabsorption.getExpabs, astra3Dabsorption.py:129-145)
+__device__ __forceinline__ float abs_weight(const ParDev& g, double t, double s) {
+  if (g.mu == 0.0) return 1.0f;
+  if (!(fabs(t) < g.radius)) return 0.0f;
+  const double half = sqrt(g.radius * g.radius - t * t);
+  const double path = fmax(0.0, half - s) * g.inv_cos;
+  return expf((float)(-g.mu * path));
+}
+
+// np.interp(t, xp, fp, left=0, right=0) with xp[k] = (k - c) - det_offset:
+// value = fp[j] + w (fp[j1] - fp[j]); numpy compares t with the end nodes and
+// returns fp[-1] exactly at the last node.
+__device__ __forceinline__ bool interp_node(const ParDev& g, double t, int& j, int& j1, float& w) {
+  if (!(t >= g.xp0 && t <= g.xp_last)) return false;
+  const double u = t - g.xp0;
+  int jj = (int)floor(u);
+  jj = jj < 0 ? 0 : (jj > g.n - 1 ? g.n - 1 : jj);
+  j = jj;
+  j1 = jj + 1 < g.n ? jj + 1 : g.n - 1;
+  w = (jj == g.n - 1) ? 0.0f : (float)(t - ((double)(jj - g.c) - g.det_offset));
+  return true;
+}
+
+// weight that a sample at `coord` gives to integer position idx
+// (skimage bilinear: floor tap 1-d, ceil tap d; floor == ceil -> 1 + 0)
+__device__ __forceinline__ float tap_weight_axis(double coord, int idx) {
+  const double f = floor(coord), cl = ceil(coord);
+  const double d = coord - f;
+  double w = 0.0;
+  if ((int)f == idx) w += 1.0 - d;
+  if ((int)cl == idx) w += d;
+  return (float)w;
+}
+
+// ---------------------------------------------------------------------------
+// forward projection of ray (k, i) at z .. z+ZV-1, zero inclination
+// (Projector3DParallel.py:129-156: warp with matrix R, clip, column sum)
+// ---------------------------------------------------------------------------
+template <int ZV, bool CLIP>
+__device__ __forceinline__ typename VecT<ZV>::T fp_ray(const ParDev& g, const float* __restrict__ vol,
+                                                        const float* __restrict__ minmax, int k, int i, int z) {
+  using V = typename VecT<ZV>::T;
+  const int n = g.n;
+  const double* M = g.mats + 6 * i;
+  const double M01 = M[1], M02 = M[2], M11 = M[4], M12 = M[5];
+  const double kp = (double)k - g.det_offset;
+  const double a0 = M[0] * kp, a1 = M[3] * kp;
+  const double t = (double)(k - g.c) - g.det_offset;
+  V lo = vzero<ZV>(), hi = vzero<ZV>();
+  if constexpr (CLIP) {
+    lo = vload<ZV>(minmax + z);
+    hi = vload<ZV>(minmax + g.nz + z);
+  }
+  V acc = vzero<ZV>();
+  const size_t zstride = (size_t)g.nz;
+  for (int r = 0; r < n; ++r) {
+    const double rr = (double)r;
+    const double col = (a0 + M01 * rr) + M02;
+    const double row = (a1 + M11 * rr) + M12;
+    if (!(row > -1.0 && row < (double)n && col > -1.0 && col < (double)n)) continue;
+    const double fr = floor(row), fc = floor(col);
+    const double dr = row - fr, dc = col - fc;
+    const int r0 = (int)fr, c0 = (int)fc;
+    const int r1 = (int)ceil(row), c1 = (int)ceil(col);
+    const bool vr0 = r0 >= 0, vr1 = r1 < n, vc0 = c0 >= 0, vc1 = c1 < n;
+    const float f1dc = (float)(1.0 - dc), fdc = (float)dc;
+    const float f1dr = (float)(1.0 - dr), fdr = (float)dr;
+    V tl = vzero<ZV>(), tr = vzero<ZV>(), bl = vzero<ZV>(), br = vzero<ZV>();
+    if (vr0 && vc0) tl = vload<ZV>(vol + ((size_t)r0 * n + c0) * zstride + z);
+    if (vr0 && vc1) tr = vload<ZV>(vol + ((size_t)r0 * n + c1) * zstride + z);
+    if (vr1 && vc0) bl = vload<ZV>(vol + ((size_t)r1 * n + c0) * zstride + z);
+    if (vr1 && vc1) br = vload<ZV>(vol + ((size_t)r1 * n + c1) * zstride + z);
+    const V top = vfma(fdc, tr, vmul(f1dc, tl));
+    const V bottom = vfma(fdc, br, vmul(f1dc, bl));
+    V v = vfma(fdr, bottom, vmul(f1dr, top));
+    if constexpr (CLIP) {
+      // skimage keeps exact-cval (0) outputs when 0 lies outside the slice
+      // range; an output is exactly 0 iff no in-range tap has positive weight.
+      const bool colw = vc0 || (dc > 0.0 && vc1);
+      const bool anyin = (vr0 && colw) || (dr > 0.0 && vr1 && colw);
+      if (anyin) v = vclamp(v, lo, hi);
+    }
+    acc = vfma(abs_weight(g, t, (double)(r - g.c)), v, acc);
+  }
+  return acc;
+}
+
+// forward projection with inclination, one detector row j
+__device__ __forceinline__ float fp_ray_incl(const ParDev& g, const float* __restrict__ vol, int k, int i, int j) {
+  const int n = g.n, nz = g.nz;
+  const double* M = g.mats + 6 * i;
+  const double kp = (double)k - g.det_offset;
+  const double a0 = M[0] * kp, a1 = M[3] * kp;
+  const double t = (double)(k - g.c) - g.det_offset;
+  float acc = 0.0f;
+  for (int r = 0; r < n; ++r) {
+    const double rr = (double)r;
+    const double col = (a0 + M[1] * rr) + M[2];
+    const double row = (a1 + M[4] * rr) + M[5];
+    if (!(row > -1.0 && row < (double)n && col > -1.0 && col < (double)n)) continue;
+    const double s = (double)(r - g.c);
+    const double zpos = (double)j + s * g.tan_phi;
+    const double fz = floor(zpos);
+    const double wz1 = zpos - fz;
+    const int z0 = (int)fz;
+    const double fr = floor(row), fc = floor(col);
+    const double dr = row - fr, dc = col - fc;
+    const int r0 = (int)fr, c0 = (int)fc, r1 = (int)ceil(row), c1 = (int)ceil(col);
+    const bool vr0 = r0 >= 0, vr1 = r1 < n, vc0 = c0 >= 0, vc1 = c1 < n;
+    const float f1dc = (float)(1.0 - dc), fdc = (float)dc;
+    const float f1dr = (float)(1.0 - dr), fdr = (float)dr;
+    float v = 0.0f;
+    for (int q = 0; q < 2; ++q) {
+      const int zz = z0 + q;
+      if (zz < 0 || zz >= nz) continue;
+      const float wz = (float)(q == 0 ? 1.0 - wz1 : wz1);
+      float tl = 0.f, tr = 0.f, bl = 0.f, br = 0.f;
+      if (vr0 && vc0) tl = vol[((size_t)r0 * n + c0) * nz + zz];
+      if (vr0 && vc1) tr = vol[((size_t)r0 * n + c1) * nz + zz];
+      if (vr1 && vc0) bl = vol[((size_t)r1 * n + c0) * nz + zz];
+      if (vr1 && vc1) br = vol[((size_t)r1 * n + c1) * nz + zz];
+      const float top = __builtin_fmaf(fdc, tr, f1dc * tl);
+      const float bottom = __builtin_fmaf(fdc, br, f1dc * bl);
+      v = __builtin_fmaf(wz, __builtin_fmaf(fdr, bottom, f1dr * top), v);
+    }
+    acc = __builtin_fmaf(abs_weight(g, t, s), v, acc);
+  }
+  return acc;
+}
+
+// ---------------------------------------------------------------------------
+// back projection at voxel column pix, z .. z+ZV-1 (no circle clip here)
+// (Projector3DParallel.py:175-205)
+// ---------------------------------------------------------------------------
+template <int ZV>
+__device__ __forceinline__ typename VecT<ZV>::T bp_voxel(const ParDev& g, const float* __restrict__ sino, int pix,
+                                                          int z) {
+  using V = typename VecT<ZV>::T;
+  const int row = pix / g.n, col = pix % g.n;
+  const double x = (double)(col - g.c), y = (double)(row - g.c);
+  const size_t zstride = (size_t)g.nz;
+  V acc = vzero<ZV>();
+  for (int i = 0; i < g.na; ++i) {
+    const double co = g.cs[2 * i], sn = g.cs[2 * i + 1];
+    const double t = x * co - y * sn;
+    int j, j1;
+    float w;
+    if (!interp_node(g, t, j, j1, w)) continue;
+    const V b0 = vload<ZV>(sino + ((size_t)j * g.na + i) * zstride + z);
+    const V b1 = vload<ZV>(sino + ((size_t)j1 * g.na + i) * zstride + z);
+    V val = vfma(w, vsub(b1, b0), b0);
+    if (g.mu != 0.0) val = vmul(abs_weight(g, t, x * sn + y * co), val);
+    acc = vadd(acc, val);
+  }
+  return acc;
+}
+
+__device__ __forceinline__ float bp_voxel_incl(const ParDev& g, const float* __restrict__ sino, int pix, int z) {
+  const int row = pix / g.n, col = pix % g.n;
+  const double x = (double)(col - g.c), y = (double)(row - g.c);
+  float acc = 0.0f;
+  for (int i = 0; i < g.na; ++i) {
+    const double co = g.cs[2 * i], sn = g.cs[2 * i + 1];
+    const double t = x * co - y * sn;
+    int j, j1;
+    float w;
+    if (!interp_node(g, t, j, j1, w)) continue;
+    const double s = x * sn + y * co;
+    const double vpos = (double)z - s * g.tan_phi;
+    const double fz = floor(vpos);
+    const double wz1 = vpos - fz;
+    const int q0 = (int)fz;
+    float val = 0.0f;
+    for (int q = 0; q < 2; ++q) {
+      const int jz = q0 + q;
+      if (jz < 0 || jz >= g.nz) continue;
+      const float wz = (float)(q == 0 ? 1.0 - wz1 : wz1);
+      const float b0 = sino[((size_t)j * g.na + i) * g.nz + jz];
+      const float b1 = sino[((size_t)j1 * g.na + i) * g.nz + jz];
+      val = __builtin_fmaf(wz, __builtin_fmaf(w, b1 - b0, b0), val);
+    }
+    acc = __builtin_fmaf(abs_weight(g, t, s), val, acc);
+  }
+  return acc;
+}
+
+// ---------------------------------------------------------------------------
+// exact transpose of the forward projection (linear part): voxel-driven
+// gather over the <= 4x4 candidate samples (k, r) per angle whose bilinear
+// footprint covers the voxel
+// ---------------------------------------------------------------------------
+template <int ZV>
+__device__ __forceinline__ typename VecT<ZV>::T fpT_voxel(const ParDev& g, const float* __restrict__ sino, int pix,
+                                                           int z) {
+  using V = typename VecT<ZV>::T;
+  const int n = g.n;
+  const int row = pix / n, col = pix % n;
+  const double X = (double)(col - g.c), Y = (double)(row - g.c);
+  const size_t zstride = (size_t)g.nz;
+  V acc = vzero<ZV>();
+  for (int i = 0; i < g.na; ++i) {
+    const double* M = g.mats + 6 * i;
+    const double co = M[0], sn = M[1];
+    const double kstar = co * X - sn * Y + (double)g.c + g.det_offset;
+    const double rstar = sn * X + co * Y + (double)g.c;
+    const int kb = (int)floor(kstar) - 1, rb = (int)floor(rstar) - 1;
+    for (int kk = kb; kk < kb + 4; ++kk) {
+      if (kk < 0 || kk >= n) continue;
+      const double kp = (double)kk - g.det_offset;
+      const double a0 = M[0] * kp, a1 = M[3] * kp;
+      const double t = (double)(kk - g.c) - g.det_offset;
+      float wk = 0.0f;
+      for (int r = rb; r < rb + 4; ++r) {
+        if (r < 0 || r >= n) continue;
+        const double rr = (double)r;
+        const double colS = (a0 + M[1] * rr) + M[2];
+        const double rowS = (a1 + M[4] * rr) + M[5];
+        if (!(fabs(rowS - (double)row) < 1.0 && fabs(colS - (double)col) < 1.0)) continue;
+        const float wr = tap_weight_axis(rowS, row);
+        const float wc = tap_weight_axis(colS, col);
+        wk = __builtin_fmaf(wr * wc, abs_weight(g, t, (double)(r - g.c)), wk);
+      }
+      if (wk != 0.0f) acc = vfma(wk, vload<ZV>(sino + ((size_t)kk * g.na + i) * zstride + z), acc);
+    }
+  }
+  return acc;
+}
+
+__device__ __forceinline__ float fpT_voxel_incl(const ParDev& g, const float* __restrict__ sino, int pix, int z) {
+  const int n = g.n, nz = g.nz;
+  const int row = pix / n, col = pix % n;
+  const double X = (double)(col - g.c), Y = (double)(row - g.c);
+  float acc = 0.0f;
+  for (int i = 0; i < g.na; ++i) {
+    const double* M = g.mats + 6 * i;
+    const double co = M[0], sn = M[1];
+    const double kstar = co * X - sn * Y + (double)g.c + g.det_offset;
+    const double rstar = sn * X + co * Y + (double)g.c;
+    const int kb = (int)floor(kstar) - 1, rb = (int)floor(rstar) - 1;
+    for (int kk = kb; kk < kb + 4; ++kk) {
+      if (kk < 0 || kk >= n) continue;
+      const double kp = (double)kk - g.det_offset;
+      const double a0 = M[0] * kp, a1 = M[3] * kp;
+      const double t = (double)(kk - g.c) - g.det_offset;
+      for (int r = rb; r < rb + 4; ++r) {
+        if (r < 0 || r >= n) continue;
+        const double rr = (double)r;
+        const double colS = (a0 + M[1] * rr) + M[2];
+        const double rowS = (a1 + M[4] * rr) + M[5];
+        if (!(fabs(rowS - (double)row) < 1.0 && fabs(colS - (double)col) < 1.0)) continue;
+        const float wxy = tap_weight_axis(rowS, row) * tap_weight_axis(colS, col);
+        if (wxy == 0.0f) continue;
+        const double s = (double)(r - g.c);
+        const float wa = abs_weight(g, t, s) * wxy;
+        // detector rows j whose sample z position j + s tan(phi) has a tap on z
+        const int jc = (int)floor((double)z - s * g.tan_phi);
+        for (int j = jc - 1; j <= jc + 2; ++j) {
+          if (j < 0 || j >= nz) continue;
+          const float wz = tap_weight_axis((double)j + s * g.tan_phi, z);
+          if (wz == 0.0f) continue;
+          acc = __builtin_fmaf(wa * wz, sino[((size_t)kk * g.na + i) * nz + j], acc);
+        }
+      }
+    }
+  }
+  return acc;
+}
+
+// ---------------------------------------------------------------------------
+// exact transpose of the back projection: sinogram-driven gather over the
+// voxels whose detector coordinate lies within one bin of bin k
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void bpT_range(const ParDev& g, double co, double sn, double xk, int a, bool rows_major,
+                                          int& b0, int& b1) {
+  double lo, hi;
+  if (rows_major) {
+    const double Y = (double)(a - g.c);
+    const double e0 = (xk - 1.0 + Y * sn) / co, e1 = (xk + 1.0 + Y * sn) / co;
+    lo = fmin(e0, e1);
+    hi = fmax(e0, e1);
+  } else {
+    const double X = (double)(a - g.c);
+    const double e0 = (X * co - (xk - 1.0)) / sn, e1 = (X * co - (xk + 1.0)) / sn;
+    lo = fmin(e0, e1);
+    hi = fmax(e0, e1);
+  }
+  b0 = (int)floor(lo) - 1 + g.c;
+  b1 = (int)ceil(hi) + 1 + g.c;
+  if (b0 < 0) b0 = 0;
+  if (b1 > g.n - 1) b1 = g.n - 1;
+}
+
+template <int ZV>
+__device__ __forceinline__ typename VecT<ZV>::T bpT_bin(const ParDev& g, const float* __restrict__ vol, int k, int i,
+                                                         int z, int clipc) {
+  using V = typename VecT<ZV>::T;
+  const int n = g.n;
+  const size_t zstride = (size_t)g.nz;
+  const double co = g.cs[2 * i], sn = g.cs[2 * i + 1];
+  const double xk = (double)(k - g.c) - g.det_offset;
+  const bool rows_major = fabs(co) >= fabs(sn);
+  V acc = vzero<ZV>();
+  for (int a = 0; a < n; ++a) {
+    int b0, b1;
+    bpT_range(g, co, sn, xk, a, rows_major, b0, b1);
+    for (int bb = b0; bb <= b1; ++bb) {
+      const int row = rows_major ? a : bb;
+      const int col = rows_major ? bb : a;
+      const int pix = row * n + col;
+      if (clipc && g.outside[pix]) continue;
+      const double x = (double)(col - g.c), y = (double)(row - g.c);
+      const double t = x * co - y * sn;
+      int j, j1;
+      float w;
+      if (!interp_node(g, t, j, j1, w)) continue;
+      float wk = 0.0f;
+      if (j == k) wk += 1.0f - w;
+      if (j1 == k) wk += w;
+      if (wk == 0.0f) continue;
+      if (g.mu != 0.0) wk *= abs_weight(g, t, x * sn + y * co);
+      acc = vfma(wk, vload<ZV>(vol + (size_t)pix * zstride + z), acc);
+    }
+  }
+  return acc;
+}
+
+__device__ __forceinline__ float bpT_bin_incl(const ParDev& g, const float* __restrict__ vol, int k, int i, int jz,
+                                              int clipc) {
+  const int n = g.n, nz = g.nz;
+  const double co = g.cs[2 * i], sn = g.cs[2 * i + 1];
+  const double xk = (double)(k - g.c) - g.det_offset;
+  const bool rows_major = fabs(co) >= fabs(sn);
+  float acc = 0.0f;
+  for (int a = 0; a < n; ++a) {
+    int b0, b1;
+    bpT_range(g, co, sn, xk, a, rows_major, b0, b1);
+    for (int bb = b0; bb <= b1; ++bb) {
+      const int row = rows_major ? a : bb;
+      const int col = rows_major ? bb : a;
+      const int pix = row * n + col;
+      if (clipc && g.outside[pix]) continue;
+      const double x = (double)(col - g.c), y = (double)(row - g.c);
+      const double t = x * co - y * sn;
+      int j, j1;
+      float w;
+      if (!interp_node(g, t, j, j1, w)) continue;
+      float wk = 0.0f;
+      if (j == k) wk += 1.0f - w;
+      if (j1 == k) wk += w;
+      if (wk == 0.0f) continue;
+      const double s = x * sn + y * co;
+      wk *= abs_weight(g, t, s);
+      // voxels z whose detector-row position z - s tan(phi) has a tap on jz
+      const int zc = (int)floor((double)jz + s * g.tan_phi);
+      for (int z = zc - 1; z <= zc + 2; ++z) {
+        if (z < 0 || z >= nz) continue;
+        const float wz = tap_weight_axis((double)z - s * g.tan_phi, jz);
+        if (wz == 0.0f) continue;
+        acc = __builtin_fmaf(wk * wz, vol[(size_t)pix * nz + z], acc);
+      }
+    }
+  }
+  return acc;
+}
+
+// host helpers implemented in parallel.hip
+int par_slice_minmax(vamx_plan* p, const float* vol, hipStream_t s);
+bool par_use_vec4(const vamx_plan* p, const void* a, const void* b);
+
+}  // namespace vamx

@@ -1,0 +1,227 @@
+"""Geometry containers of the light-propagation path.
+
+Mirrors ``vamtoolbox/geometry.py``: ``ProjectionGeometry`` (30-193),
+``Volume`` (196-430), ``TargetGeometry`` (433-613), ``Sinogram`` (616-636),
+``Reconstruction`` (639-659), ``getCircleMask`` / ``getInds`` (682-733), with
+the same constructor keywords and array layouts (volume (nY, nX, nZ),
+sinogram (nR, nTheta, nZ)).  Differences: inputs are never mutated in place;
+STL voxelisation and dill pickling are outside this package (targets are
+arrays or images); ``save``/``load`` use ``.npz``.
+"""
+
+from __future__ import annotations
+
+from typing import Literal
+
+import numpy as np
+
+RayType = Literal["parallel", "cone", "algebraic", "ray_trace"]
+
+_PROJ_KWARGS = (
+    "projector_pixel_size", "absorption_coeff", "container_radius", "attenuation_field", "occlusion",
+    "inclination_angle", "zero_dose_sino", "index_model", "attenuation_model", "absorption_model",
+    "ray_trace_method", "eikonal_parametrization", "ray_trace_ode_solver", "ray_trace_ray_config",
+    "tensor_dtype", "ray_density",
+)
+
+
+class ProjectionGeometry:
+    """Projection geometry (geometry.py:30-155).
+
+    Keywords as the reference, plus the build's parallel-beam extensions:
+    ``det_offset`` (detector-u offset in voxels), ``absorption_per_voxel``
+    (exponential absorption inside the container cylinder, per voxel) and
+    ``absorption_radius`` (cylinder radius in voxels, default (N-1)/2).
+    """
+
+    def __init__(self, angles, ray_type: RayType, CUDA: bool | None = False, **kwargs):
+        self.angles = np.asarray(angles)
+        self.n_angles = np.size(self.angles)
+        self.ray_type = ray_type
+        self.CUDA = CUDA
+        for k in _PROJ_KWARGS:
+            setattr(self, k, kwargs.get(k, None))
+        self.loading_path_for_matrix = kwargs.get("loading_path_for_matrix", True)
+        self.det_offset = float(kwargs.get("det_offset", 0.0))
+        self.absorption_per_voxel = float(kwargs.get("absorption_per_voxel", 0.0))
+        self.absorption_radius = kwargs.get("absorption_radius", None)
+        self.absorption_mask = None
+        unknown = set(kwargs) - set(_PROJ_KWARGS) - {"loading_path_for_matrix", "det_offset",
+                                                     "absorption_per_voxel", "absorption_radius"}
+        for k in unknown:  # the reference ignores unknown keywords (e.g. n_rot, rot_vel)
+            setattr(self, k, kwargs[k])
+
+    def calcZeroDoseSinogram(self, A, target_geo):
+        """geometry.py:157-159."""
+        b = A.forward(target_geo.zero_dose)
+        self.zero_dose_sino = np.where(np.asarray(b) != 0, True, False)
+
+    def calcAbsorptionMask(self, target_geo):
+        """Radial absorption mask of geometry.py:161-193 (the reference's own
+        approximation; the exponential per-ray absorption of this build is
+        ``absorption_per_voxel``)."""
+        if self.container_radius is None or self.projector_pixel_size is None:
+            raise Exception(
+                "container_radius and projector_pixel_size must be specified in ProjectorGeometry if "
+                "absoption_coeff is used to calculate an absorption mask.")
+        x = target_geo.array
+        r = x.shape[0] / 2 * self.projector_pixel_size
+        R = self.container_radius
+        if R < r:
+            raise Exception(
+                "container radius is smaller than the simulation radius. container radius must be larger "
+                "than simulation radius for valid reconstruction.")
+        circle_y, circle_x = np.meshgrid(np.linspace(-r, r, x.shape[0]), np.linspace(-r, r, x.shape[1]))
+        z = R - np.sqrt(circle_x ** 2 + circle_y ** 2)
+        mask = np.exp(-self.absorption_coeff * z)
+        mask[z < R - r] = 0
+        if x.ndim == 3:
+            mask = np.broadcast_to(mask[..., np.newaxis], x.shape)
+        self.absorption_mask = mask
+
+
+class Volume:
+    """geometry.py:196-430."""
+
+    def __init__(self, array, proj_geo: ProjectionGeometry | None = None, **kwargs):
+        self.array = array
+        self.proj_geo = proj_geo
+        self.file_extension = kwargs.get("file_extension", None)
+        self.vol_type = kwargs.get("vol_type", None)
+        self.spatial_sampling_rate = kwargs.get("spatial_sampling_rate", None)
+        self.options = kwargs.get("options", None)
+        self.n_dim = np.ndim(self.array)
+        if self.vol_type in ("recon", "target"):
+            if self.n_dim == 2:
+                self.nY, self.nX = np.shape(self.array)
+                self.nZ = 0
+                self.resolution = None
+            elif self.n_dim == 3:
+                self.nY, self.nX, self.nZ = np.shape(self.array)
+                self.resolution = self.nZ
+        elif self.vol_type == "sino":
+            if self.n_dim == 2:
+                self.nR, self.nTheta = np.shape(self.array)
+                self.nZ = 0
+                self.resolution = None
+            elif self.n_dim == 3:
+                self.nR, self.nTheta, self.nZ = np.shape(self.array)
+                self.resolution = self.nZ
+
+    def segmentZ(self, slices):
+        if isinstance(slices, int) or (isinstance(slices, list) and len(slices) == 1):
+            self.array = self.array[:, :, slices]
+            self.n_dim = 2
+            self.nZ = 0
+            self.resolution = None
+        if isinstance(slices, list):
+            self.array = self.array[:, :, slices[0]:slices[1]]
+            self.nZ = slices[1] - slices[0] + 1
+            self.resolution = self.nZ
+
+    def save(self, name: str):
+        """Save the array (and its type) as ``name + file_extension + '.npz'``."""
+        if self.file_extension:
+            name = name + self.file_extension
+        np.savez(name + ".npz", array=np.asarray(self.array), vol_type=str(self.vol_type))
+
+    def constructCoordVec(self, spatial_sampling_rate=None, device=None):
+        """Coordinate vectors centred on the object, geometry.py:365-430."""
+        if self.vol_type == "sino":
+            print("Coordinate system of sinogram is defined by propagator, using both target_geo and proj_geo.")
+            return None
+        if spatial_sampling_rate is not None:
+            self.spatial_sampling_rate = spatial_sampling_rate
+        if self.spatial_sampling_rate is None:
+            self.spatial_sampling_rate = 500
+        rate = self.spatial_sampling_rate
+        xv = np.linspace(-(self.nX - 1) / (2 * rate), (self.nX - 1) / (2 * rate), self.nX)
+        yv = np.linspace(-(self.nY - 1) / (2 * rate), (self.nY - 1) / (2 * rate), self.nY)
+        if self.n_dim == 2:
+            zv = np.atleast_1d(0.0)
+        else:
+            zv = np.linspace(-(self.nZ - 1) / (2 * rate), (self.nZ - 1) / (2 * rate), self.nZ)
+        self.coord_vec_list = [xv, yv, zv]
+        return self.coord_vec_list
+
+    def constructCoordGrid(self, spatial_sampling_rate=None, device=None):
+        cv = self.constructCoordVec(spatial_sampling_rate, device)
+        if cv is None:
+            return None
+        self.coord_grid_list = list(np.meshgrid(cv[0], cv[1], cv[2], indexing="ij"))
+        return self.coord_grid_list
+
+
+class TargetGeometry(Volume):
+    """geometry.py:433-542: array or image targets (STL voxelisation is not part
+    of this package; pass the voxelised array)."""
+
+    def __init__(self, target=None, stlfilename=None, resolution=None, imagefilename=None, pixels=None,
+                 rot_angles=(0, 0, 0), bodies="all", binarize_image=True, clip_to_circle=True, options=None,
+                 insert=None, zero_dose=None):
+        self.insert = insert
+        self.zero_dose = zero_dose
+        if target is not None:
+            array = np.atleast_3d(np.array(target, copy=True))
+        elif imagefilename is not None and stlfilename is None:
+            from PIL import Image, ImageOps
+            image = Image.open(imagefilename).convert("L")
+            if image.size[0] != image.size[1]:
+                sq = np.max(image.size)
+                dw, dh = sq - image.size[0], sq - image.size[1]
+                image = ImageOps.expand(image, (dw // 2, dh // 2, dw - dw // 2, dh - dh // 2))
+            if pixels is not None:
+                image = image.resize(size=(pixels, pixels))
+            image = np.array(image).astype(np.float32)
+            image = image / np.max(image)
+            array = np.where(image >= 0.5, 1.0, 0.0) if binarize_image else image
+            array = np.atleast_3d(array)
+        elif stlfilename is not None:
+            raise NotImplementedError(
+                "STL voxelisation is not part of vamtoolbox_amd; voxelise the STL and pass target=array")
+        else:
+            raise ValueError("TargetGeometry needs target= or imagefilename=")
+        self.gel_inds, self.void_inds = getInds(array)
+        if clip_to_circle:
+            from .util.data import clipToCircle
+            array = clipToCircle(array)
+        super().__init__(array=array, options=options, file_extension=".target", vol_type="target")
+
+
+class Sinogram(Volume):
+    def __init__(self, sinogram, proj_geo: ProjectionGeometry, options=None):
+        super().__init__(array=sinogram, proj_geo=proj_geo, options=options, file_extension=".sino",
+                         vol_type="sino")
+
+
+class Reconstruction(Volume):
+    def __init__(self, reconstruction, proj_geo: ProjectionGeometry, options=None):
+        super().__init__(array=reconstruction, proj_geo=proj_geo, options=options, file_extension=".recon",
+                         vol_type="recon")
+
+
+def loadVolume(file_name: str):
+    """Load a Volume saved with ``Volume.save`` (npz; no pickles)."""
+    z = np.load(file_name, allow_pickle=False)
+    vt = str(z["vol_type"])
+    cls = {"target": Reconstruction, "recon": Reconstruction, "sino": Sinogram}.get(vt, Reconstruction)
+    return cls(z["array"], None)
+
+
+def getCircleMask(target):
+    """geometry.py:682-711: True inside the inscribed circle."""
+    target = np.asarray(target)
+    if np.ndim(target) == 2:
+        circle_y, circle_x = np.meshgrid(np.linspace(-1, 1, target.shape[0]), np.linspace(-1, 1, target.shape[1]))
+    else:
+        circle_y, circle_x, _ = np.meshgrid(np.linspace(-1, 1, target.shape[0]),
+                                            np.linspace(-1, 1, target.shape[1]),
+                                            np.linspace(-1, 1, target.shape[2]))
+    return np.array(circle_x ** 2 + circle_y ** 2 <= 1 ** 2, dtype=bool)
+
+
+def getInds(target):
+    """geometry.py:714-733: gel (target > 0) and void (target == 0) inside the circle."""
+    target = np.asarray(target)
+    circle_mask = getCircleMask(target)
+    return np.logical_and(target > 0, circle_mask), np.logical_and(target == 0, circle_mask)
