@@ -1,0 +1,413 @@
+"""Benchmark: fused BCLP iterations per second at 256^3 voxels x 360 angles on one MI355X.
+
+Workload (the BCLP configuration): 256^3 float32 grayscale target (union of 8
+seeded random ellipsoids, semi-axes 15-70 voxels, Gaussian-smoothed with
+sigma = 2 voxels, clipped to [0, 1] and to the inscribed cylinder), 360
+angles (0..359 deg), 256 x 256 detector, exponential absorption 0.01 per voxel
+inside the cylinder, 10-voxel detector offset, zero inclination; analytical
+sigmoid response (threshold 0.6, steepness 20), band +-0.05, p = 2, q = 1,
+sinogram >= 0.  A step is one BCLP iteration (back projection of the
+sinogram, response / band / Lp epilogue and loss, forward projection of the
+operand fused with the sinogram update), replayed from a captured HIP graph.
+
+Besides the JSON line's required fields it reports: the same iteration as a
+plain unfused GPU loop (HIP projectors + torch element-wise ops + the
+reference's host-side loss branch) and as a plain PyTorch loop without custom
+kernels, the relative sinogram / dose error of the fused loop against the
+unfused one, the forward / back projection kernel times, and the float64
+NumPy port of the reference CPU projector timed on a bounded sample.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+N, NZ, NA = 256, 256, 360
+MU, OFFSET = 0.01, 10.0
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------------------
+# synthetic target
+# ---------------------------------------------------------------------------
+def make_target(device, seed=3):
+    import torch
+    rng = np.random.default_rng(seed)
+    c = (N - 1) / 2
+    ax = torch.arange(N, device=device, dtype=torch.float32) - c
+    zz = (torch.arange(NZ, device=device, dtype=torch.float32) - (NZ - 1) / 2)
+    Y, X, Z = torch.meshgrid(ax, ax, zz, indexing="ij")
+    vol = torch.zeros((N, N, NZ), device=device)
+    for _ in range(8):
+        a, b, cc = rng.uniform(15, 70, 3)
+        ctr = rng.uniform(-50, 50, 3)
+        R = _rotation(rng)
+        d = torch.stack([X - ctr[0], Y - ctr[1], Z - ctr[2]], dim=-1)
+        Rt = torch.tensor(R, device=device, dtype=torch.float32)
+        u = d @ Rt.T
+        inside = (u[..., 0] / a) ** 2 + (u[..., 1] / b) ** 2 + (u[..., 2] / cc) ** 2 <= 1
+        vol = torch.maximum(vol, inside.float())
+    # Gaussian smoothing, sigma = 2 voxels (separable, truncated at 4 sigma)
+    r = torch.arange(-8, 9, device=device, dtype=torch.float32)
+    k = torch.exp(-r ** 2 / (2 * 2.0 ** 2))
+    k = k / k.sum()
+    v = vol[None, None]
+    for dim in range(3):
+        shape = [1, 1, 1, 1, 1]
+        shape[2 + dim] = 17
+        pad = [0, 0, 0, 0, 0, 0]
+        pad[2 * (2 - dim)] = 8
+        pad[2 * (2 - dim) + 1] = 8
+        v = torch.nn.functional.conv3d(torch.nn.functional.pad(v, pad), k.reshape(shape))
+    vol = v[0, 0].clamp(0, 1)
+    from vamtoolbox_amd.util.data import clipToCircle
+    return clipToCircle(vol).contiguous()
+
+
+def _rotation(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+# ---------------------------------------------------------------------------
+# plain PyTorch projector (no custom kernels): the baseline GPU port
+# ---------------------------------------------------------------------------
+class TorchD1:
+    """Rotate-and-sum forward / np.interp back projection with torch ops only."""
+
+    def __init__(self, n, nz, angles_deg, device, mu=0.0, offset=0.0, chunk=6):
+        import torch
+        self.n, self.nz, self.dev, self.chunk = n, nz, device, chunk
+        self.c = n // 2
+        th = torch.tensor(np.deg2rad(angles_deg), device=device, dtype=torch.float64)
+        self.co, self.sn = torch.cos(th), torch.sin(th)
+        self.mu, self.off = mu, offset
+        self.R = (n - 1) / 2.0
+        kk = torch.arange(n, device=device, dtype=torch.float64)
+        self.t = (kk - self.c) - offset
+        self.s = kk - self.c
+        yy, xx = torch.meshgrid(kk - self.c, kk - self.c, indexing="ij")
+        self.x, self.y = xx, yy
+        from vamtoolbox_amd.util.data import outside_mask
+        self.out_mask = torch.as_tensor(outside_mask(n, n), device=device)
+
+    def _abs(self, t, s):
+        import torch
+        if self.mu == 0:
+            return torch.ones_like(t + s)
+        inside = t.abs() < self.R
+        half = torch.sqrt(torch.clamp(self.R ** 2 - t * t, min=0))
+        return torch.where(inside, torch.exp(-self.mu * torch.clamp(half - s, min=0)), torch.zeros_like(t + s))
+
+    def forward(self, vol):
+        import torch
+        n, nz = self.n, self.nz
+        flat = vol.reshape(n * n, nz)
+        out = torch.empty((n, self.co.numel(), nz), device=self.dev)
+        T, S = self.t[None, :], self.s[:, None]          # (r, k)
+        W = self._abs(T, S).float()
+        for a0 in range(0, self.co.numel(), self.chunk):
+            co, sn = self.co[a0:a0 + self.chunk, None, None], self.sn[a0:a0 + self.chunk, None, None]
+            row = self.c - sn * T + co * S
+            col = self.c + co * T + sn * S
+            r0, c0 = torch.floor(row), torch.floor(col)
+            dr, dc = (row - r0).float(), (col - c0).float()
+            acc = 0
+            for rr, wr in ((r0, 1 - dr), (r0 + 1, dr)):
+                for cc, wc in ((c0, 1 - dc), (c0 + 1, dc)):
+                    ok = (rr >= 0) & (rr < n) & (cc >= 0) & (cc < n)
+                    idx = (rr.clamp(0, n - 1) * n + cc.clamp(0, n - 1)).long()
+                    vals = flat.index_select(0, idx.reshape(-1)).reshape(idx.shape + (nz,))
+                    acc = acc + (wr * wc * ok)[..., None] * vals
+            out[:, a0:a0 + self.chunk, :] = (W[None, ..., None] * acc).sum(dim=1).permute(1, 0, 2)
+        return out
+
+    def backward(self, sino):
+        import torch
+        n, nz = self.n, self.nz
+        out = torch.zeros((n, n, nz), device=self.dev)
+        for i in range(self.co.numel()):
+            t = self.x * self.co[i] - self.y * self.sn[i]
+            s = self.x * self.sn[i] + self.y * self.co[i]
+            u = t - (-self.c - self.off)
+            ok = (u >= 0) & (u <= n - 1)
+            j = torch.clamp(torch.floor(u), 0, n - 1).long()
+            j1 = torch.clamp(j + 1, max=n - 1)
+            w = torch.where(j == n - 1, torch.zeros_like(u), u - j).float()
+            b = sino[:, i, :]
+            v = b[j] + w[..., None] * (b[j1] - b[j])
+            v = v * (ok * self._abs(t, s)).float()[..., None]
+            out += v
+        return torch.where(self.out_mask[..., None], torch.zeros((), device=self.dev), out)
+
+
+# ---------------------------------------------------------------------------
+# CPU baseline: float64 NumPy port of the reference's per-(z, angle) loops
+# ---------------------------------------------------------------------------
+def cpu_baseline(target_np, g_np, angles, k_slices=2):
+    import oracle.d1 as d1
+    import oracle.optim as oo
+    resp = oo.Response("analytical", B=20, M=0.6)
+    t0 = time.perf_counter()
+    sl = slice(0, k_slices)
+    dose = d1.backward(g_np[:, :, sl].astype(np.float64), angles, det_offset=OFFSET, mu=MU)
+    m = resp.map(dose)
+    eT = m - target_np[:, :, sl]
+    eb = np.abs(eT) - 0.05
+    v = eb > 0
+    operand = v * np.abs(eb) * np.sign(eT) * resp.dmap(dose)
+    d1.forward(operand, angles, det_offset=OFFSET, mu=MU)
+    dt = time.perf_counter() - t0
+    per_iter = dt * (NZ / k_slices)
+    return {"value": 1.0 / per_iter, "unit": "iters/s", "cores": 1, "kind": "port",
+            "sample": "float64 NumPy port of Projector3DParallelPython back+forward projection and the BCLP "
+                      "epilogue on %d of %d z-slices (all %d angles), %.1f s measured, scaled x%d to one "
+                      "iteration" % (k_slices, NZ, NA, dt, NZ // k_slices)}
+
+
+# ---------------------------------------------------------------------------
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-torch-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the light-propagation path runs on a single MI355X; "
+              "multi-GPU runs are not part of this build (see DESIGN.md for how it would shard)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import vamtoolbox_amd as vam
+    from vamtoolbox_amd import _lib
+    from vamtoolbox_amd.optimizer.BCLP import BCLPNorm
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    angles = np.arange(NA, dtype=np.float64)
+
+    log("building target")
+    T = make_target(dev)
+    tg = vam.geometry.TargetGeometry(target=T.cpu().numpy())
+    pg = vam.geometry.ProjectionGeometry(angles, "parallel", CUDA=True, absorption_per_voxel=MU, det_offset=OFFSET)
+    rm = vam.response.ResponseModel(type="analytical", B=20, M=0.6)
+    inv = vam.response.ResponseModel(type="interpolation", B=20, M=0.6)
+    P = vam.projectorconstructor.projectorconstructor(tg, pg)
+    log("initial sinogram")
+    g0 = P.forward(inv.map_inv(T.double()).float())
+    g0 = vam.util.data.filterSinogram(g0, "ram-lak") * (np.pi / (2 * NA))
+    g0 = torch.clamp(0.8 * g0, min=0).float().contiguous()
+    lr = 2e-5
+
+    def options(**kw):
+        return vam.optimize.Options(method="BCLP", n_iter=kw.pop("n_iter"), response_model=rm, eps=0.05, p=2, q=1,
+                                    learning_rate=lr, g0=g0, **kw)
+
+    # ---- fused, graph-replayed iteration ------------------------------------
+    log("fused BCLP: warmup %d, timed %d" % (args.warmup, args.steps))
+    bc = BCLPNorm(tg, pg, options(n_iter=args.steps, fused=True, hip_graph=True))
+    lib = _lib.load()
+    plan = bc.P.plan
+    g = bc.g0.reshape(plan.sino_shape).contiguous().clone()
+    dose = torch.empty(plan.vol_shape, device=dev)
+    loss_log = torch.zeros(args.steps + args.warmup + 2, device=dev, dtype=torch.float64)
+    counter = torch.zeros(1, device=dev, dtype=torch.int32)
+    nb = ctypes.c_int64()
+    _lib.check(lib.vamx_bclp_workspace_bytes(plan.handle, ctypes.byref(nb)), "workspace")
+    ws = torch.empty(int(nb.value), device=dev, dtype=torch.uint8)
+    resp, prm = bc._vamx_args()
+
+    def step():
+        _lib.check(lib.vamx_bclp_step(plan.handle, ctypes.byref(resp), ctypes.byref(prm), g.data_ptr(),
+                                      bc.T.data_ptr(), 0, 0, dose.data_ptr(), loss_log.data_ptr(),
+                                      counter.data_ptr(), ws.data_ptr(), _lib.stream_ptr(dev)), "vamx_bclp_step")
+
+    step()
+    graph = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream(device=dev)
+    s.wait_stream(torch.cuda.current_stream(dev))
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(graph, stream=s):
+            step()
+    torch.cuda.current_stream(dev).wait_stream(s)
+    for _ in range(max(args.warmup - 1, 0)):
+        graph.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        graph.replay()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_per_step = dt / args.steps * 1e3
+    iters_s = args.steps / dt
+    log("fused: %.3f ms/iter, %.1f iters/s" % (ms_per_step, iters_s))
+    n_fused = args.warmup + args.steps
+    g_fused = g.clone()
+    dose_fused = dose.clone()
+
+    # ---- unfused GPU loop (same projector kernels + torch element-wise) -----
+    log("unfused GPU loop")
+    bu = BCLPNorm(tg, pg, options(n_iter=n_fused, fused=False, log_metrics=False))
+    gu = bu.g0.reshape(-1).clone()
+    t_unf = []
+    for k in range(n_fused):
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        d_, m_, eT, eb, v, loss, operand = bu._eval_state(gu)
+        L = float(loss.item())
+        G = bu._grad(operand, L)
+        gu = bu.imposeSinogramConstraints(bu._cyl(gu - lr * G.reshape(-1))).reshape(-1)
+        torch.cuda.synchronize()
+        t_unf.append(time.perf_counter() - ta)
+    unf_ms = 1e3 * float(np.mean(t_unf[args.warmup:])) if n_fused > args.warmup else 1e3 * float(np.mean(t_unf))
+    dose_u = bu.P.backward(bu._cyl(gu)).reshape(plan.vol_shape)
+    # the fused dose buffer holds B g of the last step's input; compare like with like
+    rel_g = float(torch.linalg.norm((g_fused.reshape(-1) - gu).double()) / torch.linalg.norm(gu.double()))
+    dose_f_last = bu.P.backward(g_fused.reshape(plan.sino_shape))
+    rel_d = float(torch.linalg.norm((dose_f_last - dose_u).double()) / torch.linalg.norm(dose_u.double()))
+    log("unfused: %.3f ms/iter; fused vs unfused rel L2 g %.2e dose %.2e" % (unf_ms, rel_g, rel_d))
+
+    # ---- kernel times with HIP events on the launch stream -------------------
+    def time_kernel(fn, reps=10):
+        fn()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    op = torch.clamp(bc.T, 0, 1)  # a volume of the workload's shape and sign structure
+    sino_out = torch.empty(plan.sino_shape, device=dev)
+    vol_out = torch.empty(plan.vol_shape, device=dev)
+    fp_ms = time_kernel(lambda: plan.forward(op, out=sino_out))
+    bp_ms = time_kernel(lambda: plan.backward(g, out=vol_out))
+    V, S = N * N * NZ, N * NA * NZ
+    fp_bytes = 4.0 * (NA * V + S)
+    bp_bytes = 4.0 * (NA * V + V)
+    log("FP %.3f ms (%.0f GB/s algorithmic)  BP %.3f ms (%.0f GB/s algorithmic)" % (
+        fp_ms, fp_bytes / fp_ms / 1e6, bp_ms, bp_bytes / bp_ms / 1e6))
+    dom_name, dom_ms, dom_bytes = ("par_fp_kernel", fp_ms, fp_bytes) if fp_ms >= bp_ms else \
+        ("par_bp_kernel", bp_ms, bp_bytes)
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(HERE, "profiles", "traffic.txt")
+    if os.path.exists(tfile):
+        for line in open(tfile):
+            parts = line.split()
+            if len(parts) >= 2 and parts[0] == dom_name:
+                try:
+                    traffic = float(parts[1])
+                except ValueError:
+                    pass
+
+    # ---- plain PyTorch loop (no custom kernels) -------------------------------
+    torch_ms = None
+    if not args.no_torch_baseline:
+        log("plain PyTorch BCLP iteration")
+        tp = TorchD1(N, NZ, angles, dev, mu=MU, offset=OFFSET)
+        gt = g0.clone()
+        Td = bc.T
+        times = []
+        for k in range(2):
+            torch.cuda.synchronize()
+            ta = time.perf_counter()
+            dd = tp.backward(gt)
+            mm = rm.map(dd)
+            eT = mm - Td
+            eb = torch.abs(eT) - 0.05
+            vv = eb > 0
+            loss = torch.sum(vv * eb ** 2) ** 0.5
+            Lh = float(loss.item())
+            opd = (vv * torch.abs(eb) * torch.sign(eT) * rm.dmapdf(dd)).float()
+            G = tp.forward(opd)
+            if Lh != 0:
+                G = G / Lh
+            gt = torch.clamp(gt - lr * G, min=0)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - ta)
+        torch_ms = 1e3 * times[-1]
+        log("plain torch: %.1f ms/iter" % torch_ms)
+
+    # ---- CPU baseline ----------------------------------------------------------
+    cpu = None
+    if not args.no_cpu:
+        log("CPU baseline (NumPy port, bounded sample)")
+        cpu = cpu_baseline(bc.T.cpu().numpy().astype(np.float64), g.cpu().numpy(), angles)
+        cpu["cores"] = 1
+        cpu["host"] = "%d host cores visible (%s)" % (os.cpu_count() or 0, _cpu_model())
+        log("CPU: %.3e iters/s" % cpu["value"])
+
+    out = {
+        "metric": "BCLP iters/s at 256³×360 angles (rel err ≤1e-4); fwd/back HBM GB/s vs peak",
+        "value": iters_s,
+        "unit": "iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "BCLP 256^3 voxels x 360 angles, 256x256 detector, smoothed ellipsoid target, "
+                               "sigmoid response (M=0.6, B=20), eps=0.05, p=2, q=1, absorption 0.01/voxel, "
+                               "detector offset 10 voxels, fused HIP-graph iteration",
+                   "n": N, "nz": NZ, "n_angles": NA},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom_name,
+                     "kernel_ms": dom_ms, "algorithmic_bytes": dom_bytes},
+        "cpu_baseline": cpu,
+        "forward_ms": fp_ms,
+        "backward_ms": bp_ms,
+        "forward_gbs": fp_bytes / fp_ms / 1e6,
+        "backward_gbs": bp_bytes / bp_ms / 1e6,
+        "unfused_ms_per_iter": unf_ms,
+        "speedup_vs_unfused_hip": unf_ms / ms_per_step,
+        "plain_torch_ms_per_iter": torch_ms,
+        "speedup_vs_plain_torch": (torch_ms / ms_per_step) if torch_ms else None,
+        "rel_err_sino_vs_unfused": rel_g,
+        "rel_err_dose_vs_unfused": rel_d,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+if __name__ == "__main__":
+    main()
