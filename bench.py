@@ -265,6 +265,8 @@ def main():
     iters_s = args.steps / dt
     log("fused: %.3f ms/iter, %.1f iters/s" % (ms_per_step, iters_s))
     n_fused = args.warmup + args.steps
+    losses = loss_log[:n_fused].cpu().numpy()
+    log("loss first %.6g last %.6g" % (losses[0], losses[-1]))
     g_fused = g.clone()
     dose_fused = dose.clone()
 
@@ -395,6 +397,8 @@ def main():
         "speedup_vs_plain_torch": (torch_ms / ms_per_step) if torch_ms else None,
         "rel_err_sino_vs_unfused": rel_g,
         "rel_err_dose_vs_unfused": rel_d,
+        "loss_first": float(losses[0]),
+        "loss_last": float(losses[-1]),
     }
     print(json.dumps(out), flush=True)
 
