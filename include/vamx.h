@@ -36,6 +36,9 @@ typedef struct vamx_plan vamx_plan;
 #define VAMX_PAR_NO_WARP_CLIP 1   /* forward: skip the per-slice output clip of
                                      skimage.transform.warp(..., clip=True)    */
 #define VAMX_PAR_NO_CIRCLE_CLIP 2 /* backward: skip util.data.clipToCircle     */
+#define VAMX_PAR_PER_LANE_GEOMETRY 4 /* use the per-lane-geometry kernels instead of
+                                        the wavefront-uniform ones (same results
+                                        to float32 rounding; for cross-checks)  */
 
 typedef struct vamx_par_geom {
   int32_t n;                /* nx == ny == nu (square slices)                   */

@@ -193,3 +193,23 @@ def test_autograd_dot_products(gpu):
     lhs = float((P.plan.backward(s.detach()).double() * v.double()).sum())
     rhs = float((s.detach().double() * s.grad.double()).sum())
     assert abs(lhs - rhs) / abs(lhs) <= 1e-6
+
+
+def test_bclp_fused_wave_path_matches_oracle(gpu):
+    """nz = 64: the fused BCLP kernels take the wavefront-uniform path."""
+    import oracle.optim as oo
+    import vamtoolbox_amd as vam
+    tg, pg, ang = _problem(n=24, nz=64, na=30, seed=4, gray=True, absorption_per_voxel=0.01, det_offset=1.0)
+    fwd, bwd = _oracle_ops(ang, mu=0.01, det_offset=1.0)
+    g0 = 0.8 * oo.bclp_init(tg.array, fwd, oo.Response("interpolation", B=20, M=0.6), len(ang))
+    rm = vam.response.ResponseModel(type="analytical", B=20, M=0.6)
+    ro = oo.Response("analytical", B=20, M=0.6)
+    lr = 1e-4
+    opts = vam.optimize.Options(method="BCLP", n_iter=10, response_model=rm, eps=0.05, learning_rate=lr, g0=g0)
+    g, dose, _, logs = vam.optimize.optimize(tg, pg, opts, output="components")
+    g_ref, dose_ref, _, loss_ref = oo.bclp(tg.array, fwd, bwd, ro, 10, g0, eps=0.05, lr=lr)
+    e_g, e_d = rel_l2(g, g_ref), rel_l2(dose, dose_ref)
+    e_l = np.max(np.abs(logs.loss - loss_ref) / np.abs(loss_ref))
+    print("BCLP wave path rel L2 g %.2e dose %.2e loss %.2e" % (e_g, e_d, e_l), loss_ref[[0, -1]])
+    assert loss_ref[-1] < loss_ref[0]
+    assert e_g <= 1e-4 and e_d <= 1e-4 and e_l <= 1e-4

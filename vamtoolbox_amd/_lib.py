@@ -93,6 +93,7 @@ class RtGeom(ctypes.Structure):
 # constants mirrored from include/vamx.h
 PAR_NO_WARP_CLIP = 1
 PAR_NO_CIRCLE_CLIP = 2
+PAR_PER_LANE_GEOMETRY = 4
 RESP_GLF, RESP_LINEAR, RESP_IDENTITY, RESP_TABLE = 0, 1, 2, 3
 RT_SYMPLECTIC_EULER, RT_FORWARD_EULER, RT_LEAPFROG, RT_RK4 = 0, 1, 2, 3
 RT_CANONICAL, RT_PHYSICAL, RT_OPTICAL = 0, 1, 2

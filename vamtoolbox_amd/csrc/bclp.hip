@@ -51,11 +51,18 @@ __device__ double np_interp(double x, const double* __restrict__ xp, const doubl
 __device__ void response_eval(const RespDev& r, double f, double& m, double& dm) {
   if (r.kind == VAMX_RESP_GLF) {
     const double e = exp(-r.B * (f - r.M));
-    m = r.A + (r.K - r.A) / pow(1.0 + e, 1.0 / r.nu);
-    const double c1 = pow(1.0 / (r.K - r.A), r.nu);
-    const double c2 = r.B / r.nu;
-    const double c3 = pow(m - r.A, r.nu + 1.0);
-    dm = c1 * c2 * c3 * e;
+    if (r.nu == 1.0) {
+      m = r.A + (r.K - r.A) / (1.0 + e);
+      const double c1 = 1.0 / (r.K - r.A);
+      const double d = m - r.A;
+      dm = c1 * r.B * (d * d) * e;
+    } else {
+      m = r.A + (r.K - r.A) / pow(1.0 + e, 1.0 / r.nu);
+      const double c1 = pow(1.0 / (r.K - r.A), r.nu);
+      const double c2 = r.B / r.nu;
+      const double c3 = pow(m - r.A, r.nu + 1.0);
+      dm = c1 * c2 * c3 * e;
+    }
   } else if (r.kind == VAMX_RESP_LINEAR) {
     m = r.lin_M * f + r.lin_C;
     dm = r.lin_M;
@@ -84,8 +91,13 @@ __device__ __forceinline__ double bclp_voxel(const RespDev& r, const BclpDev& b,
   }
   const double w = b.w_arr ? (double)b.w_arr[idx] : b.weight;
   const double sgn = eT > 0.0 ? 1.0 : (eT < 0.0 ? -1.0 : 0.0);
-  operand = (float)(w * pow(fabs(eb), b.p - 1.0) * sgn * dm);  // 361-367
-  return w * pow(fabs(eb), b.p);                               // 340-342
+  const double ab = fabs(eb);
+  if (b.p == 2.0) {
+    operand = (float)(w * ab * sgn * dm);
+    return w * (ab * ab);
+  }
+  operand = (float)(w * pow(ab, b.p - 1.0) * sgn * dm);  // 361-367
+  return w * pow(ab, b.p);                               // 340-342
 }
 
 __device__ __forceinline__ double block_sum_256(double v, double* sh) {
@@ -134,6 +146,67 @@ __global__ __launch_bounds__(256) void bclp_bp_epilogue(ParDev g, int clipc, Res
   }
   const double s = block_sum_256(part, sh);
   if (threadIdx.x == 0) partials[lb] = s;
+}
+
+// A (wavefront-uniform variant): one wavefront = one voxel column x 64*ZV z
+template <int ZV>
+__global__ __launch_bounds__(256) void bclp_bp_epilogue_wave(ParDev g, int clipc, RespDev r, BclpDev b,
+                                                             const float* __restrict__ gsino,
+                                                             const float* __restrict__ target,
+                                                             float* __restrict__ dose, float* __restrict__ operand,
+                                                             double* __restrict__ partials) {
+  __shared__ double sh[256];
+  const int lane = threadIdx.x & 63;
+  const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
+  const long long wave = ((long long)lb * blockDim.x + threadIdx.x) >> 6;
+  const int nch = g.nz / (64 * ZV);
+  double part = 0.0;
+  if (wave < (long long)g.n * g.n * nch) {
+    const int zc = (int)(wave % nch);
+    const int pix = (int)(wave / nch);
+    const int z = zc * 64 * ZV + lane * ZV;
+    const size_t base = (size_t)pix * g.nz + z;
+    typename VecT<ZV>::T d = vzero<ZV>();
+    if (!(clipc && g.outside[pix])) d = bp_voxel_wave<ZV>(g, gsino, pix, z, lane);
+    vstore<ZV>(dose + base, d);
+    typename VecT<ZV>::T op = vzero<ZV>();
+    const typename VecT<ZV>::T tg = vload<ZV>(target + base);
+#pragma unroll
+    for (int q = 0; q < ZV; ++q) {
+      float o;
+      part += bclp_voxel(r, b, vget(d, q), vget(tg, q), base + q, o);
+      vset(op, q, o);
+    }
+    vstore<ZV>(operand + base, op);
+  }
+  const double s = block_sum_256(part, sh);
+  if (threadIdx.x == 0) partials[lb] = s;
+}
+
+// C (wavefront-uniform variant)
+template <int ZV>
+__global__ __launch_bounds__(256) void bclp_fp_update_wave(ParDev g, const float* __restrict__ operand,
+                                                           float* __restrict__ gsino,
+                                                           const float* __restrict__ minmax, int clip,
+                                                           const double* __restrict__ scale_p, float lr, float glb,
+                                                           float gub) {
+  const int lane = threadIdx.x & 63;
+  const long long wave = ((long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
+  const int nch = g.nz / (64 * ZV);
+  if (wave >= (long long)g.na * g.n * nch) return;
+  const int zc = (int)(wave % nch);
+  const long long ray = wave / nch;
+  const int k = (int)(ray % g.n), i = (int)(ray / g.n);
+  const int z = zc * 64 * ZV + lane * ZV;
+  const bool clip_on = clip && reinterpret_cast<const int*>(minmax)[2 * g.nz] != 0;
+  const typename VecT<ZV>::T fp = clip_on ? fp_ray_wave<ZV, true>(g, operand, minmax, k, i, z, lane)
+                                          : fp_ray_wave<ZV, false>(g, operand, minmax, k, i, z, lane);
+  const float scale = (float)(*scale_p);
+  float* gp = gsino + ((size_t)k * g.na + i) * g.nz + z;
+  typename VecT<ZV>::T gv = vload<ZV>(gp);
+#pragma unroll
+  for (int q = 0; q < ZV; ++q) vset(gv, q, clampf(vget(gv, q) - lr * (scale * vget(fp, q)), glb, gub));
+  vstore<ZV>(gp, gv);
 }
 
 // B: fixed-order reduction of the partials, loss log and gradient scale
@@ -255,6 +328,28 @@ int launch_epilogue(vamx_plan* p, const RespDev& r, const BclpDev& b, const floa
   const bool v4 = !gd.incl && gd.nz % 4 == 0 && ((uintptr_t)g | (uintptr_t)target | (uintptr_t)dose |
                                                  (uintptr_t)operand) % 16 == 0 &&
                   (!b.eps_arr || (uintptr_t)b.eps_arr % 16 == 0) && (!b.w_arr || (uintptr_t)b.w_arr % 16 == 0);
+  int wz = par_wave_zv(p, g, dose);
+  if (wz) {
+    const uintptr_t al = (uintptr_t)target | (uintptr_t)operand | (uintptr_t)b.eps_arr | (uintptr_t)b.w_arr;
+    while (wz > 1 && al % (4 * wz) != 0) wz >>= 1;
+    if ((wz == 1 && gd.nz % 64) || (wz == 2 && gd.nz % 128)) wz = 0;
+  }
+  if (wz) {
+    const long long threads = (long long)gd.n * gd.n * (gd.nz / (64 * wz)) * 64;
+    nparts = (int)blocks_for(threads, 256);
+    const dim3 grid(nparts), block(256);
+    if (wz == 4)
+      hipLaunchKernelGGL(bclp_bp_epilogue_wave<4>, grid, block, 0, s, gd, clipc, r, b, g, target, dose, operand,
+                         partials);
+    else if (wz == 2)
+      hipLaunchKernelGGL(bclp_bp_epilogue_wave<2>, grid, block, 0, s, gd, clipc, r, b, g, target, dose, operand,
+                         partials);
+    else
+      hipLaunchKernelGGL(bclp_bp_epilogue_wave<1>, grid, block, 0, s, gd, clipc, r, b, g, target, dose, operand,
+                         partials);
+    VAMX_HIP(hipGetLastError());
+    return VAMX_OK;
+  }
   const long long total = (long long)gd.n * gd.n * (gd.nz / (v4 ? 4 : 1));
   nparts = (int)blocks_for(total, 256);
   const dim3 grid(nparts), block(256);
@@ -311,6 +406,23 @@ extern "C" int vamx_bclp_step(vamx_plan* plan, const vamx_response* resp, const 
     if (clip && !gd.incl) {
       rc = par_slice_minmax(plan, operand, s);
       if (rc) return rc;
+    }
+    const int wz = par_wave_zv(plan, g, operand);
+    if (wz) {
+      const long long threads = (long long)gd.na * gd.n * (gd.nz / (64 * wz)) * 64;
+      const dim3 grid(blocks_for(threads, 256)), block(256);
+      const int cl = clip ? 1 : 0;
+      if (wz == 4)
+        hipLaunchKernelGGL(bclp_fp_update_wave<4>, grid, block, 0, s, gd, operand, g, plan->d_minmax, cl, scale, lr,
+                           glb, gub);
+      else if (wz == 2)
+        hipLaunchKernelGGL(bclp_fp_update_wave<2>, grid, block, 0, s, gd, operand, g, plan->d_minmax, cl, scale, lr,
+                           glb, gub);
+      else
+        hipLaunchKernelGGL(bclp_fp_update_wave<1>, grid, block, 0, s, gd, operand, g, plan->d_minmax, cl, scale, lr,
+                           glb, gub);
+      VAMX_HIP(hipGetLastError());
+      return VAMX_OK;
     }
     const long long total = (long long)gd.na * gd.n * (gd.nz / zv);
     const dim3 grid(blocks_for(total, 256)), block(256);

@@ -22,18 +22,22 @@ namespace vamx {
 
 template <int ZV> struct VecT;
 template <> struct VecT<1> { using T = float; };
+template <> struct VecT<2> { using T = float2; };
 template <> struct VecT<4> { using T = float4; };
 
 template <int ZV> __device__ __forceinline__ typename VecT<ZV>::T vzero();
 template <> __device__ __forceinline__ float vzero<1>() { return 0.0f; }
+template <> __device__ __forceinline__ float2 vzero<2>() { return make_float2(0.f, 0.f); }
 template <> __device__ __forceinline__ float4 vzero<4>() { return make_float4(0.f, 0.f, 0.f, 0.f); }
 
 template <int ZV> __device__ __forceinline__ typename VecT<ZV>::T vload(const float* p);
 template <> __device__ __forceinline__ float vload<1>(const float* p) { return *p; }
+template <> __device__ __forceinline__ float2 vload<2>(const float* p) { return *reinterpret_cast<const float2*>(p); }
 template <> __device__ __forceinline__ float4 vload<4>(const float* p) { return *reinterpret_cast<const float4*>(p); }
 
 template <int ZV> __device__ __forceinline__ void vstore(float* p, typename VecT<ZV>::T v);
 template <> __device__ __forceinline__ void vstore<1>(float* p, float v) { *p = v; }
+template <> __device__ __forceinline__ void vstore<2>(float* p, float2 v) { *reinterpret_cast<float2*>(p) = v; }
 template <> __device__ __forceinline__ void vstore<4>(float* p, float4 v) { *reinterpret_cast<float4*>(p) = v; }
 
 __device__ __forceinline__ float vfma(float a, float x, float y) { return __builtin_fmaf(a, x, y); }
@@ -41,25 +45,38 @@ __device__ __forceinline__ float4 vfma(float a, float4 x, float4 y) {
   return make_float4(__builtin_fmaf(a, x.x, y.x), __builtin_fmaf(a, x.y, y.y), __builtin_fmaf(a, x.z, y.z),
                      __builtin_fmaf(a, x.w, y.w));
 }
+__device__ __forceinline__ float2 vfma(float a, float2 x, float2 y) {
+  return make_float2(__builtin_fmaf(a, x.x, y.x), __builtin_fmaf(a, x.y, y.y));
+}
 __device__ __forceinline__ float vmul(float a, float x) { return a * x; }
+__device__ __forceinline__ float2 vmul(float a, float2 x) { return make_float2(a * x.x, a * x.y); }
 __device__ __forceinline__ float4 vmul(float a, float4 x) { return make_float4(a * x.x, a * x.y, a * x.z, a * x.w); }
 __device__ __forceinline__ float vsub(float a, float b) { return a - b; }
 __device__ __forceinline__ float4 vsub(float4 a, float4 b) {
   return make_float4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w);
 }
+__device__ __forceinline__ float2 vsub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float vadd(float a, float b) { return a + b; }
+__device__ __forceinline__ float2 vadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float4 vadd(float4 a, float4 b) {
   return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
 }
 __device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
 __device__ __forceinline__ float vclamp(float v, float lo, float hi) { return clampf(v, lo, hi); }
+__device__ __forceinline__ float2 vclamp(float2 v, float2 lo, float2 hi) {
+  return make_float2(clampf(v.x, lo.x, hi.x), clampf(v.y, lo.y, hi.y));
+}
 __device__ __forceinline__ float4 vclamp(float4 v, float4 lo, float4 hi) {
   return make_float4(clampf(v.x, lo.x, hi.x), clampf(v.y, lo.y, hi.y), clampf(v.z, lo.z, hi.z),
                      clampf(v.w, lo.w, hi.w));
 }
 __device__ __forceinline__ float vget(float v, int) { return v; }
+__device__ __forceinline__ float vget(float2 v, int q) { return q == 0 ? v.x : v.y; }
 __device__ __forceinline__ float vget(float4 v, int q) { return q == 0 ? v.x : (q == 1 ? v.y : (q == 2 ? v.z : v.w)); }
 __device__ __forceinline__ void vset(float& v, int, float x) { v = x; }
+__device__ __forceinline__ void vset(float2& v, int q, float x) {
+  if (q == 0) v.x = x; else v.y = x;
+}
 __device__ __forceinline__ void vset(float4& v, int q, float x) {
   if (q == 0) v.x = x; else if (q == 1) v.y = x; else if (q == 2) v.z = x; else v.w = x;
 }
@@ -431,8 +448,164 @@ __device__ __forceinline__ float bpT_bin_incl(const ParDev& g, const float* __re
   return acc;
 }
 
+
+// ---------------------------------------------------------------------------
+// wavefront-uniform cores
+//
+// When a wavefront's 64 lanes cover 64*ZV consecutive z values of ONE ray (or
+// voxel column), the geometry of a sample (or angle) is the same for every
+// lane.  Lane l then evaluates the geometry of sample r0 + l (or angle
+// i0 + l) -- 64 samples in parallel, float64, the reference's operation order
+// -- and the inner loop walks the samples with an in-range tap (a ballot
+// mask), broadcasting each sample's tap base and weights with readlane into
+// scalar registers: per sample the vector units only issue the tap loads and
+// the fused multiply-adds.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int rl_i(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+__device__ __forceinline__ float rl_f(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// slot weights of one axis: the taps floor / floor+1 (skimage's floor / ceil
+// with weights 1-d / d) mapped onto the pair (b, b+1) with b in [0, n-2]
+__device__ __forceinline__ void axis_slots(double coord, int n, int& b, double& s0, double& s1) {
+  const double f = floor(coord);
+  const double d = coord - f;
+  const int fi = (int)f;
+  if (fi < 0) {          // floor = -1: only floor+1 = 0 is a tap
+    b = 0; s0 = d; s1 = 0.0;
+  } else if (fi >= n - 1) {  // floor = n-1: floor+1 = n is outside
+    b = n - 2; s0 = 0.0; s1 = 1.0 - d;
+  } else {
+    b = fi; s0 = 1.0 - d; s1 = d;
+  }
+}
+
+// forward projection of ray (k, i), lanes = z; requires n >= 2
+template <int ZV, bool CLIP>
+__device__ __forceinline__ typename VecT<ZV>::T fp_ray_wave(const ParDev& g, const float* __restrict__ vol,
+                                                             const float* __restrict__ minmax, int k, int i, int z,
+                                                             int lane) {
+  using V = typename VecT<ZV>::T;
+  const int n = g.n;
+  const double* M = g.mats + 6 * i;
+  const double M01 = M[1], M02 = M[2], M11 = M[4], M12 = M[5];
+  const double kp = (double)k - g.det_offset;
+  const double a0 = M[0] * kp, a1 = M[3] * kp;
+  const double t = (double)(k - g.c) - g.det_offset;
+  V lo = vzero<ZV>(), hi = vzero<ZV>();
+  if constexpr (CLIP) {
+    lo = vload<ZV>(minmax + z);
+    hi = vload<ZV>(minmax + g.nz + z);
+  }
+  const size_t nzs = (size_t)g.nz;
+  const size_t step_c = nzs, step_r = (size_t)n * nzs;
+  V acc = vzero<ZV>();
+  for (int r0 = 0; r0 < n; r0 += 64) {
+    const int r = r0 + lane;
+    int base = 0;
+    float w00 = 0.f, w01 = 0.f, w10 = 0.f, w11 = 0.f, aw = 0.f;
+    bool valid = false, anyin = false;
+    if (r < n) {
+      const double rr = (double)r;
+      const double col = (a0 + M01 * rr) + M02;
+      const double row = (a1 + M11 * rr) + M12;
+      if (row > -1.0 && row < (double)n && col > -1.0 && col < (double)n) {
+        int br, bc;
+        double r0w, r1w, c0w, c1w;
+        axis_slots(row, n, br, r0w, r1w);
+        axis_slots(col, n, bc, c0w, c1w);
+        base = br * n + bc;
+        anyin = (r0w > 0.0 || r1w > 0.0) && (c0w > 0.0 || c1w > 0.0);
+        aw = abs_weight(g, t, (double)(r - g.c));
+        valid = anyin && (CLIP || aw != 0.0f);
+        const float f = CLIP ? 1.0f : aw;   // without the clip the absorption folds into the weights
+        w00 = f * (float)(r0w * c0w);
+        w01 = f * (float)(r0w * c1w);
+        w10 = f * (float)(r1w * c0w);
+        w11 = f * (float)(r1w * c1w);
+      }
+    }
+    unsigned long long m = __ballot(valid);
+    while (m) {
+      const int j = __builtin_ctzll(m);
+      m &= m - 1;
+      const int b = rl_i(base, j);
+      const float q00 = rl_f(w00, j), q01 = rl_f(w01, j), q10 = rl_f(w10, j), q11 = rl_f(w11, j);
+      const float* p = vol + (size_t)b * nzs + z;
+      const V t00 = vload<ZV>(p), t01 = vload<ZV>(p + step_c);
+      const V t10 = vload<ZV>(p + step_r), t11 = vload<ZV>(p + step_r + step_c);
+      if constexpr (CLIP) {
+        V v = vfma(q11, t11, vfma(q10, t10, vfma(q01, t01, vmul(q00, t00))));
+        v = vclamp(v, lo, hi);  // every enumerated sample has an in-range tap of positive weight
+        acc = vfma(rl_f(aw, j), v, acc);
+      } else {
+        acc = vfma(q11, t11, vfma(q10, t10, vfma(q01, t01, vfma(q00, t00, acc))));
+      }
+    }
+  }
+  return acc;
+}
+
+// back projection of voxel column pix, lanes = z (no circle clip here); n >= 2
+template <int ZV>
+__device__ __forceinline__ typename VecT<ZV>::T bp_voxel_wave(const ParDev& g, const float* __restrict__ sino,
+                                                               int pix, int z, int lane) {
+  using V = typename VecT<ZV>::T;
+  const int row = pix / g.n, col = pix % g.n;
+  const double x = (double)(col - g.c), y = (double)(row - g.c);
+  const size_t nzs = (size_t)g.nz;
+  const size_t step_j = (size_t)g.na * nzs;
+  V acc = vzero<ZV>();
+  for (int i0 = 0; i0 < g.na; i0 += 64) {
+    const int i = i0 + lane;
+    int base = 0;
+    float w0 = 0.f, w1 = 0.f;
+    bool valid = false;
+    if (i < g.na) {
+      const double co = g.cs[2 * i], sn = g.cs[2 * i + 1];
+      const double t = x * co - y * sn;
+      int j, j1;
+      float w;
+      if (interp_node(g, t, j, j1, w)) {
+        const float a = (g.mu != 0.0) ? abs_weight(g, t, x * sn + y * co) : 1.0f;
+        valid = a != 0.0f;
+        if (j == g.n - 1) {        // exactly at the last node: fp[n-1]
+          base = (j - 1) * g.na + i;
+          w0 = 0.0f;
+          w1 = a;
+        } else {
+          base = j * g.na + i;
+          w0 = a * (1.0f - w);
+          w1 = a * w;
+        }
+      }
+    }
+    unsigned long long m = __ballot(valid);
+    while (m) {
+      const int jj = __builtin_ctzll(m);
+      m &= m - 1;
+      const int b = rl_i(base, jj);
+      const float q0 = rl_f(w0, jj), q1 = rl_f(w1, jj);
+      const float* p = sino + (size_t)b * nzs + z;
+      acc = vfma(q1, vload<ZV>(p + step_j), vfma(q0, vload<ZV>(p), acc));
+    }
+  }
+  return acc;
+}
+
+// ZV for the wavefront-uniform path (0: not applicable)
+__host__ __device__ inline int wave_zv(int nz, int n, bool incl) {
+  if (incl || n < 2) return 0;
+  if (nz % 256 == 0) return 4;
+  if (nz % 128 == 0) return 2;
+  if (nz % 64 == 0) return 1;
+  return 0;
+}
+
 // host helpers implemented in parallel.hip
 int par_slice_minmax(vamx_plan* p, const float* vol, hipStream_t s);
 bool par_use_vec4(const vamx_plan* p, const void* a, const void* b);
+int par_wave_zv(const vamx_plan* p, const void* a, const void* b);
 
 }  // namespace vamx

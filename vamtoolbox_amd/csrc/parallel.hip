@@ -30,15 +30,25 @@ __global__ __launch_bounds__(256) void slice_minmax_part(const float* __restrict
 
 __global__ __launch_bounds__(256) void slice_minmax_final(const float* __restrict__ part, int nz, int parts,
                                                           float* __restrict__ out) {
-  const int z = blockIdx.x * blockDim.x + threadIdx.x;
-  if (z >= nz) return;
-  float lo = INFINITY, hi = -INFINITY;
-  for (int b = 0; b < parts; ++b) {
-    lo = fminf(lo, part[(size_t)b * 2 * nz + z]);
-    hi = fmaxf(hi, part[(size_t)b * 2 * nz + nz + z]);
+  // one block: out[0:nz] = min, out[nz:2nz] = max, and an int flag at out[2nz]
+  // telling whether any slice lacks 0 in its range (the clip is then active)
+  __shared__ int any_sh;
+  if (threadIdx.x == 0) any_sh = 0;
+  __syncthreads();
+  int any = 0;
+  for (int z = threadIdx.x; z < nz; z += blockDim.x) {
+    float lo = INFINITY, hi = -INFINITY;
+    for (int b = 0; b < parts; ++b) {
+      lo = fminf(lo, part[(size_t)b * 2 * nz + z]);
+      hi = fmaxf(hi, part[(size_t)b * 2 * nz + nz + z]);
+    }
+    out[z] = lo;
+    out[nz + z] = hi;
+    if (!(lo <= 0.0f && 0.0f <= hi)) any = 1;
   }
-  out[z] = lo;
-  out[nz + z] = hi;
+  if (any) any_sh = 1;  // benign same-value race in LDS
+  __syncthreads();
+  if (threadIdx.x == 0) reinterpret_cast<int*>(out)[2 * nz] = any_sh;
 }
 
 // ---------------------------------------------------------------------------
@@ -55,6 +65,44 @@ __global__ __launch_bounds__(256) void par_fp_kernel(ParDev g, const float* __re
   const long long ray = tid / nzv;
   const int k = (int)(ray % g.n), i = (int)(ray / g.n);
   vstore<ZV>(sino + ((size_t)k * g.na + i) * g.nz + z, fp_ray<ZV, CLIP>(g, vol, minmax, k, i, z));
+}
+
+// wavefront-uniform forward projection: one wavefront = one ray x 64*ZV z
+template <int ZV>
+__global__ __launch_bounds__(256) void par_fp_wave_kernel(ParDev g, const float* __restrict__ vol,
+                                                          float* __restrict__ sino, const float* __restrict__ minmax,
+                                                          int clip) {
+  const int lane = threadIdx.x & 63;
+  const long long wave = ((long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
+  const int nch = g.nz / (64 * ZV);
+  if (wave >= (long long)g.na * g.n * nch) return;
+  const int zc = (int)(wave % nch);
+  const long long ray = wave / nch;
+  const int k = (int)(ray % g.n), i = (int)(ray / g.n);
+  const int z = zc * 64 * ZV + lane * ZV;
+  const bool clip_on = clip && reinterpret_cast<const int*>(minmax)[2 * g.nz] != 0;
+  typename VecT<ZV>::T acc = clip_on ? fp_ray_wave<ZV, true>(g, vol, minmax, k, i, z, lane)
+                                     : fp_ray_wave<ZV, false>(g, vol, minmax, k, i, z, lane);
+  vstore<ZV>(sino + ((size_t)k * g.na + i) * g.nz + z, acc);
+}
+
+// wavefront-uniform back projection: one wavefront = one voxel column x 64*ZV z
+template <int ZV>
+__global__ __launch_bounds__(256) void par_bp_wave_kernel(ParDev g, const float* __restrict__ sino,
+                                                          float* __restrict__ vol, int clipc) {
+  const int lane = threadIdx.x & 63;
+  const long long wave = ((long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
+  const int nch = g.nz / (64 * ZV);
+  if (wave >= (long long)g.n * g.n * nch) return;
+  const int zc = (int)(wave % nch);
+  const int pix = (int)(wave / nch);
+  const int z = zc * 64 * ZV + lane * ZV;
+  float* outp = vol + (size_t)pix * g.nz + z;
+  if (clipc && g.outside[pix]) {
+    vstore<ZV>(outp, vzero<ZV>());
+    return;
+  }
+  vstore<ZV>(outp, bp_voxel_wave<ZV>(g, sino, pix, z, lane));
 }
 
 __global__ __launch_bounds__(256) void par_fp_incl_kernel(ParDev g, const float* __restrict__ vol,
@@ -148,6 +196,16 @@ __global__ __launch_bounds__(256) void par_bpT_incl_kernel(ParDev g, const float
 // ===========================================================================
 using namespace vamx;
 
+int vamx::par_wave_zv(const vamx_plan* p, const void* a, const void* b) {
+  if (p->flags & VAMX_PAR_PER_LANE_GEOMETRY) return 0;
+  int zv = wave_zv(p->par.nz, p->par.n, p->par.incl != 0);
+  const uintptr_t al = (uintptr_t)a | (uintptr_t)b;
+  while (zv > 1 && al % (4 * zv) != 0) zv >>= 1;
+  if (zv == 1 && p->par.nz % 64 != 0) return 0;
+  if (zv == 2 && p->par.nz % 128 != 0) return 0;
+  return zv;
+}
+
 bool vamx::par_use_vec4(const vamx_plan* p, const void* a, const void* b) {
   return (p->par.nz % 4 == 0) && !p->par.incl && ((uintptr_t)a % 16 == 0) && ((uintptr_t)b % 16 == 0);
 }
@@ -206,7 +264,7 @@ extern "C" int vamx_par_create(const vamx_par_geom* geom, vamx_plan** out_plan) 
   if (e == hipSuccess) e = hipMalloc((void**)&p->d_cs, cs.size() * sizeof(double));
   if (e == hipSuccess) e = hipMalloc((void**)&p->d_outside, (size_t)n * n);
   if (e == hipSuccess) e = hipMalloc((void**)&p->d_minmax_part, (size_t)p->minmax_parts * 2 * nz * sizeof(float));
-  if (e == hipSuccess) e = hipMalloc((void**)&p->d_minmax, (size_t)2 * nz * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc((void**)&p->d_minmax, ((size_t)2 * nz + 4) * sizeof(float));
   if (e == hipSuccess) e = hipMemcpy(p->d_mats, mats.data(), mats.size() * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(p->d_cs, cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(p->d_outside, geom->outside_mask, (size_t)n * n, hipMemcpyHostToDevice);
@@ -226,8 +284,8 @@ int par_slice_minmax(vamx_plan* p, const float* vol, hipStream_t s) {
   const ParDev& g = p->par;
   hipLaunchKernelGGL(slice_minmax_part, dim3(p->minmax_parts), dim3(256), 0, s, vol, g.n * g.n, g.nz,
                      p->minmax_parts, p->d_minmax_part);
-  hipLaunchKernelGGL(slice_minmax_final, dim3(blocks_for(g.nz, 256)), dim3(256), 0, s, p->d_minmax_part, g.nz,
-                     p->minmax_parts, p->d_minmax);
+  hipLaunchKernelGGL(slice_minmax_final, dim3(1), dim3(256), 0, s, p->d_minmax_part, g.nz, p->minmax_parts,
+                     p->d_minmax);
   VAMX_HIP(hipGetLastError());
   return VAMX_OK;
 }
@@ -243,6 +301,20 @@ int par_forward_launch(vamx_plan* p, const float* vol, float* sino, hipStream_t 
   if (warp_clip) {
     int rc = par_slice_minmax(p, vol, s);
     if (rc) return rc;
+  }
+  const int wz = par_wave_zv(p, vol, sino);
+  if (wz) {
+    const long long threads = (long long)g.na * g.n * (g.nz / (64 * wz)) * 64;
+    const dim3 grid(blocks_for(threads, 256)), block(256);
+    const int clip = warp_clip ? 1 : 0;
+    if (wz == 4)
+      hipLaunchKernelGGL(par_fp_wave_kernel<4>, grid, block, 0, s, g, vol, sino, p->d_minmax, clip);
+    else if (wz == 2)
+      hipLaunchKernelGGL(par_fp_wave_kernel<2>, grid, block, 0, s, g, vol, sino, p->d_minmax, clip);
+    else
+      hipLaunchKernelGGL(par_fp_wave_kernel<1>, grid, block, 0, s, g, vol, sino, p->d_minmax, clip);
+    VAMX_HIP(hipGetLastError());
+    return VAMX_OK;
   }
   const bool v4 = par_use_vec4(p, vol, sino);
   const int zv = v4 ? 4 : 1;
@@ -269,6 +341,20 @@ int par_backward_launch(vamx_plan* p, const float* sino, float* vol, hipStream_t
     const long long total = (long long)g.n * g.n * g.nz;
     hipLaunchKernelGGL(par_bp_incl_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, s, g, sino, vol,
                        clipc ? 1 : 0);
+    VAMX_HIP(hipGetLastError());
+    return VAMX_OK;
+  }
+  const int wz = par_wave_zv(p, sino, vol);
+  if (wz) {
+    const long long threads = (long long)g.n * g.n * (g.nz / (64 * wz)) * 64;
+    const dim3 grid(blocks_for(threads, 256)), block(256);
+    const int cc = clipc ? 1 : 0;
+    if (wz == 4)
+      hipLaunchKernelGGL(par_bp_wave_kernel<4>, grid, block, 0, s, g, sino, vol, cc);
+    else if (wz == 2)
+      hipLaunchKernelGGL(par_bp_wave_kernel<2>, grid, block, 0, s, g, sino, vol, cc);
+    else
+      hipLaunchKernelGGL(par_bp_wave_kernel<1>, grid, block, 0, s, g, sino, vol, cc);
     VAMX_HIP(hipGetLastError());
     return VAMX_OK;
   }

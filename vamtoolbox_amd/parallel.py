@@ -29,12 +29,14 @@ class ParallelPlan:
     Parameters mirror ``vamx_par_geom``: ``n`` (square slice size), ``nz``,
     ``angles_deg``, ``det_offset`` (voxels), ``incl_deg``, ``mu`` (absorption
     per voxel), ``radius`` (cylinder radius, default (n-1)/2), and the two
-    clip switches (both on = reference behaviour).
+    clip switches (both on = reference behaviour).  ``per_lane_geometry``
+    selects the kernels that evaluate the geometry in every lane instead of
+    the wavefront-uniform ones (same results to float32 rounding).
     """
 
     def __init__(self, n: int, nz: int, angles_deg, det_offset: float = 0.0, incl_deg: float = 0.0,
                  mu: float = 0.0, radius: float | None = None, warp_clip: bool = True,
-                 circle_clip: bool = True, device=None):
+                 circle_clip: bool = True, per_lane_geometry: bool = False, device=None):
         self.lib = _lib.load()
         self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
         self.n, self.nz = int(n), int(nz)
@@ -43,7 +45,8 @@ class ParallelPlan:
         self.det_offset, self.incl_deg, self.mu = float(det_offset), float(incl_deg), float(mu)
         self.radius = float(radius) if radius is not None else (self.n - 1) / 2.0
         mask = np.ascontiguousarray(outside_circle_mask(self.n).astype(np.uint8))
-        flags = (0 if warp_clip else _lib.PAR_NO_WARP_CLIP) | (0 if circle_clip else _lib.PAR_NO_CIRCLE_CLIP)
+        flags = (0 if warp_clip else _lib.PAR_NO_WARP_CLIP) | (0 if circle_clip else _lib.PAR_NO_CIRCLE_CLIP) | \
+            (_lib.PAR_PER_LANE_GEOMETRY if per_lane_geometry else 0)
         geom = _lib.ParGeom(
             n=self.n, nz=self.nz, n_angles=self.na,
             angles_deg=self.angles_deg.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
