@@ -194,6 +194,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-torch-baseline", action="store_true")
+    ap.add_argument("--lr", type=float, default=0.05, help="BCLP learning rate")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="run only the forward / back projection kernels of the workload (5 launches each) "
+                         "and a 1 GiB device copy, for hardware-counter profiling")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the light-propagation path runs on a single MI355X; "
@@ -220,7 +224,23 @@ def main():
     g0 = P.forward(inv.map_inv(T.double()).float())
     g0 = vam.util.data.filterSinogram(g0, "ram-lak") * (np.pi / (2 * NA))
     g0 = torch.clamp(0.8 * g0, min=0).float().contiguous()
-    lr = 2e-5
+    lr = args.lr
+
+    if args.kernels_only:
+        plan = P.plan
+        sino = torch.empty(plan.sino_shape, device=dev)
+        vol = torch.empty(plan.vol_shape, device=dev)
+        for _ in range(5):
+            plan.forward(T, out=sino)
+        for _ in range(5):
+            plan.backward(g0, out=vol)
+        a = torch.empty(1 << 28, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)  # 1 GiB read + 1 GiB write: counter calibration
+        torch.cuda.synchronize()
+        log("kernels-only run done")
+        return
 
     def options(**kw):
         return vam.optimize.Options(method="BCLP", n_iter=kw.pop("n_iter"), response_model=rm, eps=0.05, p=2, q=1,

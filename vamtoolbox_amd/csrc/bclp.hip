@@ -161,9 +161,9 @@ __global__ __launch_bounds__(256) void bclp_bp_epilogue_wave(ParDev g, int clipc
   const long long wave = ((long long)lb * blockDim.x + threadIdx.x) >> 6;
   const int nch = g.nz / (64 * ZV);
   double part = 0.0;
-  if (wave < (long long)g.n * g.n * nch) {
+  int pix = 0;
+  if (wave < bp_tiled_pix(g.n) * nch && bp_tile_pix(wave / nch, g.n, pix)) {
     const int zc = (int)(wave % nch);
-    const int pix = (int)(wave / nch);
     const int z = zc * 64 * ZV + lane * ZV;
     const size_t base = (size_t)pix * g.nz + z;
     typename VecT<ZV>::T d = vzero<ZV>();
@@ -193,10 +193,10 @@ __global__ __launch_bounds__(256) void bclp_fp_update_wave(ParDev g, const float
   const int lane = threadIdx.x & 63;
   const long long wave = ((long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
   const int nch = g.nz / (64 * ZV);
-  if (wave >= (long long)g.na * g.n * nch) return;
+  if (wave >= fp_tiled_rays(g.n, g.na) * nch) return;
   const int zc = (int)(wave % nch);
-  const long long ray = wave / nch;
-  const int k = (int)(ray % g.n), i = (int)(ray / g.n);
+  int k, i;
+  if (!fp_tile_ray(wave / nch, g.n, g.na, k, i)) return;
   const int z = zc * 64 * ZV + lane * ZV;
   const bool clip_on = clip && reinterpret_cast<const int*>(minmax)[2 * g.nz] != 0;
   const typename VecT<ZV>::T fp = clip_on ? fp_ray_wave<ZV, true>(g, operand, minmax, k, i, z, lane)
@@ -294,7 +294,8 @@ WsLayout ws_layout(const vamx_plan* p) {
   const ParDev& g = p->par;
   WsLayout w;
   const long long items = (long long)g.n * g.n * g.nz;  // worst case ZV = 1
-  w.nparts = (int)blocks_for(items, 256);
+  const long long tiled = bp_tiled_pix(g.n) * g.nz;      // wavefront path with padding
+  w.nparts = (int)blocks_for(items > tiled ? items : tiled, 256);
   w.partials_off = 0;
   w.scale_off = ((size_t)w.nparts * sizeof(double) + 255) / 256 * 256;
   w.operand_off = w.scale_off + 256;
@@ -335,7 +336,7 @@ int launch_epilogue(vamx_plan* p, const RespDev& r, const BclpDev& b, const floa
     if ((wz == 1 && gd.nz % 64) || (wz == 2 && gd.nz % 128)) wz = 0;
   }
   if (wz) {
-    const long long threads = (long long)gd.n * gd.n * (gd.nz / (64 * wz)) * 64;
+    const long long threads = bp_tiled_pix(gd.n) * (gd.nz / (64 * wz)) * 64;
     nparts = (int)blocks_for(threads, 256);
     const dim3 grid(nparts), block(256);
     if (wz == 4)
@@ -409,7 +410,7 @@ extern "C" int vamx_bclp_step(vamx_plan* plan, const vamx_response* resp, const 
     }
     const int wz = par_wave_zv(plan, g, operand);
     if (wz) {
-      const long long threads = (long long)gd.na * gd.n * (gd.nz / (64 * wz)) * 64;
+      const long long threads = fp_tiled_rays(gd.n, gd.na) * (gd.nz / (64 * wz)) * 64;
       const dim3 grid(blocks_for(threads, 256)), block(256);
       const int cl = clip ? 1 : 0;
       if (wz == 4)

@@ -594,6 +594,42 @@ __device__ __forceinline__ typename VecT<ZV>::T bp_voxel_wave(const ParDev& g, c
   return acc;
 }
 
+// L2-aware work order of the wavefront-uniform kernels.  About 1024 waves are
+// resident per XCD at once (32 CUs x 32 waves), and consecutive logical
+// blocks run on the same XCD (xcd_remap), so 1024 consecutive waves form one
+// compact tile: forward = 16 consecutive angles x 64 consecutive rays (their
+// samples at the same along-ray step lie within a few tens of pixels), back
+// projection = a 32 x 32 patch of voxel columns (whose detector footprint at
+// one angle is ~45 bins instead of a full row).  Padding waves exit.
+constexpr int FP_TILE_A = 16, FP_TILE_K = 64;
+constexpr int BP_TILE = 32;
+
+__host__ __device__ inline long long fp_tiled_rays(int n, int na) {
+  return (long long)((na + FP_TILE_A - 1) / FP_TILE_A) * ((n + FP_TILE_K - 1) / FP_TILE_K) * FP_TILE_A * FP_TILE_K;
+}
+__device__ __forceinline__ bool fp_tile_ray(long long q, int n, int na, int& k, int& i) {
+  const int ntk = (n + FP_TILE_K - 1) / FP_TILE_K;
+  const long long tile = q / (FP_TILE_A * FP_TILE_K);
+  const int within = (int)(q % (FP_TILE_A * FP_TILE_K));
+  const int ti = (int)(tile / ntk), tk = (int)(tile % ntk);
+  i = ti * FP_TILE_A + within / FP_TILE_K;
+  k = tk * FP_TILE_K + within % FP_TILE_K;
+  return i < na && k < n;
+}
+__host__ __device__ inline long long bp_tiled_pix(int n) {
+  const long long t = (n + BP_TILE - 1) / BP_TILE;
+  return t * t * BP_TILE * BP_TILE;
+}
+__device__ __forceinline__ bool bp_tile_pix(long long q, int n, int& pix) {
+  const int nt = (n + BP_TILE - 1) / BP_TILE;
+  const long long tile = q / (BP_TILE * BP_TILE);
+  const int within = (int)(q % (BP_TILE * BP_TILE));
+  const int row = (int)(tile / nt) * BP_TILE + within / BP_TILE;
+  const int col = (int)(tile % nt) * BP_TILE + within % BP_TILE;
+  pix = row * n + col;
+  return row < n && col < n;
+}
+
 // ZV for the wavefront-uniform path (0: not applicable)
 __host__ __device__ inline int wave_zv(int nz, int n, bool incl) {
   if (incl || n < 2) return 0;

@@ -75,10 +75,10 @@ __global__ __launch_bounds__(256) void par_fp_wave_kernel(ParDev g, const float*
   const int lane = threadIdx.x & 63;
   const long long wave = ((long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
   const int nch = g.nz / (64 * ZV);
-  if (wave >= (long long)g.na * g.n * nch) return;
+  if (wave >= fp_tiled_rays(g.n, g.na) * nch) return;
   const int zc = (int)(wave % nch);
-  const long long ray = wave / nch;
-  const int k = (int)(ray % g.n), i = (int)(ray / g.n);
+  int k, i;
+  if (!fp_tile_ray(wave / nch, g.n, g.na, k, i)) return;
   const int z = zc * 64 * ZV + lane * ZV;
   const bool clip_on = clip && reinterpret_cast<const int*>(minmax)[2 * g.nz] != 0;
   typename VecT<ZV>::T acc = clip_on ? fp_ray_wave<ZV, true>(g, vol, minmax, k, i, z, lane)
@@ -93,9 +93,10 @@ __global__ __launch_bounds__(256) void par_bp_wave_kernel(ParDev g, const float*
   const int lane = threadIdx.x & 63;
   const long long wave = ((long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
   const int nch = g.nz / (64 * ZV);
-  if (wave >= (long long)g.n * g.n * nch) return;
+  if (wave >= bp_tiled_pix(g.n) * nch) return;
   const int zc = (int)(wave % nch);
-  const int pix = (int)(wave / nch);
+  int pix;
+  if (!bp_tile_pix(wave / nch, g.n, pix)) return;
   const int z = zc * 64 * ZV + lane * ZV;
   float* outp = vol + (size_t)pix * g.nz + z;
   if (clipc && g.outside[pix]) {
@@ -304,7 +305,7 @@ int par_forward_launch(vamx_plan* p, const float* vol, float* sino, hipStream_t 
   }
   const int wz = par_wave_zv(p, vol, sino);
   if (wz) {
-    const long long threads = (long long)g.na * g.n * (g.nz / (64 * wz)) * 64;
+    const long long threads = fp_tiled_rays(g.n, g.na) * (g.nz / (64 * wz)) * 64;
     const dim3 grid(blocks_for(threads, 256)), block(256);
     const int clip = warp_clip ? 1 : 0;
     if (wz == 4)
@@ -346,7 +347,7 @@ int par_backward_launch(vamx_plan* p, const float* sino, float* vol, hipStream_t
   }
   const int wz = par_wave_zv(p, sino, vol);
   if (wz) {
-    const long long threads = (long long)g.n * g.n * (g.nz / (64 * wz)) * 64;
+    const long long threads = bp_tiled_pix(g.n) * (g.nz / (64 * wz)) * 64;
     const dim3 grid(blocks_for(threads, 256)), block(256);
     const int cc = clipc ? 1 : 0;
     if (wz == 4)
