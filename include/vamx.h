@@ -81,6 +81,20 @@ int vamx_par_forward_adjoint(vamx_plan* plan, const float* sino, float* vol, voi
  * the dose operator; BCLP.py:353-375 uses P.forward in its place). */
 int vamx_par_backward_adjoint(vamx_plan* plan, const float* vol, float* sino, void* stream);
 
+/* Work order of the parallel-beam kernels (chosen by an autotuner on the host,
+ * see vamtoolbox_amd/parallel.py ParallelPlan.autotune).  No reference
+ * counterpart: ASTRA picks its launch shapes internally.  Results are
+ * identical for every setting up to float32 rounding of the z-vector width. */
+typedef struct vamx_par_tuning {
+  int32_t fp_tile_angles;   /* forward: angles per tile of consecutive waves (default 16) */
+  int32_t fp_tile_rays;     /* forward: rays per tile (default 64)                       */
+  int32_t fp_zv;            /* forward: z values per lane, 1, 2 or 4 (default 4)         */
+  int32_t bp_tile;          /* back projection: tile edge in voxel columns (default 32)  */
+  int32_t bp_zv;            /* back projection: z values per lane (default 4)            */
+} vamx_par_tuning;
+int vamx_par_set_tuning(vamx_plan* plan, const vamx_par_tuning* tuning);
+int vamx_par_get_tuning(vamx_plan* plan, vamx_par_tuning* tuning);
+
 /* ---------------------------------------------------------------------------
  * Fused BCLP iteration (vamtoolbox/optimizer/BCLP.py:277-386, 477-534)
  * ------------------------------------------------------------------------- */

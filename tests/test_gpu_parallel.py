@@ -5,6 +5,8 @@ reference discretisation (float32 data arithmetic, float64 geometry); the
 exact-adjoint pairs pass the dot-product test to 1e-6.
 """
 
+import itertools
+
 import numpy as np
 import pytest
 
@@ -161,3 +163,27 @@ def test_wave_kernels_parity(gpu, case, positive):
     print("rel L2 forward %.2e backward %.2e; wave vs lane %.2e %.2e" % (e1, e2, rel_l2(fw, fl), rel_l2(bw, bl)))
     assert e1 <= TOL and e2 <= TOL
     assert rel_l2(fw, fl) <= 1e-6 and rel_l2(bw, bl) <= 1e-6
+
+
+def test_tuning_settings_agree(gpu):
+    """Every work order (tile shape, z values per lane) gives the oracle's result."""
+    import torch
+    import oracle.d1 as d1
+    from vamtoolbox_amd.parallel import ParallelPlan
+    n, nz = 40, 128
+    ang = np.arange(0.0, 360.0, 7.0)
+    rng = np.random.default_rng(9)
+    vol = rng.uniform(-1, 1, (n, n, nz))
+    sino = rng.uniform(0, 1, (n, len(ang), nz))
+    tv = torch.tensor(vol, dtype=torch.float32, device=gpu)
+    ts = torch.tensor(sino, dtype=torch.float32, device=gpu)
+    plan = ParallelPlan(n, nz, ang, device=gpu, mu=0.02)
+    ref_f, ref_b = d1.forward(vol, ang, mu=0.02), d1.backward(sino, ang, mu=0.02)
+    for (ta, tk, zv), (tile, bzv) in zip(ParallelPlan.FP_CANDIDATES, itertools.cycle(ParallelPlan.BP_CANDIDATES)):
+        plan.set_tuning(fp_tile_angles=ta, fp_tile_rays=tk, fp_zv=zv, bp_tile=tile, bp_zv=bzv)
+        e1 = rel_l2(plan.forward(tv).cpu().numpy(), ref_f)
+        e2 = rel_l2(plan.backward(ts).cpu().numpy(), ref_b)
+        assert e1 <= TOL and e2 <= TOL, (ta, tk, zv, tile, bzv, e1, e2)
+    rep = plan.autotune(tv, ts, reps=1)
+    assert set(rep["choice"]) == {"fp_tile_angles", "fp_tile_rays", "fp_zv", "bp_tile", "bp_zv"}
+    assert plan.get_tuning() == rep["choice"]

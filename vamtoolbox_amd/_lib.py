@@ -36,6 +36,16 @@ class ParGeom(ctypes.Structure):
     ]
 
 
+class ParTuning(ctypes.Structure):
+    _fields_ = [
+        ("fp_tile_angles", c_int32),
+        ("fp_tile_rays", c_int32),
+        ("fp_zv", c_int32),
+        ("bp_tile", c_int32),
+        ("bp_zv", c_int32),
+    ]
+
+
 class Response(ctypes.Structure):
     _fields_ = [
         ("kind", c_int32),
@@ -107,6 +117,8 @@ SIGNATURES = {
     "vamx_par_backward": (c_int32, [_PLAN_P, c_void_p, c_void_p, c_void_p]),
     "vamx_par_forward_adjoint": (c_int32, [_PLAN_P, c_void_p, c_void_p, c_void_p]),
     "vamx_par_backward_adjoint": (c_int32, [_PLAN_P, c_void_p, c_void_p, c_void_p]),
+    "vamx_par_set_tuning": (c_int32, [_PLAN_P, ctypes.POINTER(ParTuning)]),
+    "vamx_par_get_tuning": (c_int32, [_PLAN_P, ctypes.POINTER(ParTuning)]),
     "vamx_bclp_workspace_bytes": (c_int32, [_PLAN_P, ctypes.POINTER(ctypes.c_int64)]),
     "vamx_bclp_step": (c_int32, [_PLAN_P, ctypes.POINTER(Response), ctypes.POINTER(BclpParams),
                                  c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,

@@ -159,12 +159,9 @@ __global__ __launch_bounds__(256) void bclp_bp_epilogue_wave(ParDev g, int clipc
   const int lane = threadIdx.x & 63;
   const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
   const long long wave = ((long long)lb * blockDim.x + threadIdx.x) >> 6;
-  const int nch = g.nz / (64 * ZV);
   double part = 0.0;
-  int pix = 0;
-  if (wave < bp_tiled_pix(g.n) * nch && bp_tile_pix(wave / nch, g.n, pix)) {
-    const int zc = (int)(wave % nch);
-    const int z = zc * 64 * ZV + lane * ZV;
+  int pix = 0, z = 0;
+  if (bp_wave_item<ZV>(wave, g, lane, pix, z)) {
     const size_t base = (size_t)pix * g.nz + z;
     typename VecT<ZV>::T d = vzero<ZV>();
     if (!(clipc && g.outside[pix])) d = bp_voxel_wave<ZV>(g, gsino, pix, z, lane);
@@ -192,12 +189,8 @@ __global__ __launch_bounds__(256) void bclp_fp_update_wave(ParDev g, const float
                                                            float gub) {
   const int lane = threadIdx.x & 63;
   const long long wave = ((long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
-  const int nch = g.nz / (64 * ZV);
-  if (wave >= fp_tiled_rays(g.n, g.na) * nch) return;
-  const int zc = (int)(wave % nch);
-  int k, i;
-  if (!fp_tile_ray(wave / nch, g.n, g.na, k, i)) return;
-  const int z = zc * 64 * ZV + lane * ZV;
+  int k, i, z;
+  if (!fp_wave_item<ZV>(wave, g, lane, k, i, z)) return;
   const bool clip_on = clip && reinterpret_cast<const int*>(minmax)[2 * g.nz] != 0;
   const typename VecT<ZV>::T fp = clip_on ? fp_ray_wave<ZV, true>(g, operand, minmax, k, i, z, lane)
                                           : fp_ray_wave<ZV, false>(g, operand, minmax, k, i, z, lane);
@@ -294,7 +287,7 @@ WsLayout ws_layout(const vamx_plan* p) {
   const ParDev& g = p->par;
   WsLayout w;
   const long long items = (long long)g.n * g.n * g.nz;  // worst case ZV = 1
-  const long long tiled = bp_tiled_pix(g.n) * g.nz;      // wavefront path with padding
+  const long long tiled = bp_tiled_pix(g) * g.nz;        // wavefront path with padding
   w.nparts = (int)blocks_for(items > tiled ? items : tiled, 256);
   w.partials_off = 0;
   w.scale_off = ((size_t)w.nparts * sizeof(double) + 255) / 256 * 256;
@@ -329,14 +322,14 @@ int launch_epilogue(vamx_plan* p, const RespDev& r, const BclpDev& b, const floa
   const bool v4 = !gd.incl && gd.nz % 4 == 0 && ((uintptr_t)g | (uintptr_t)target | (uintptr_t)dose |
                                                  (uintptr_t)operand) % 16 == 0 &&
                   (!b.eps_arr || (uintptr_t)b.eps_arr % 16 == 0) && (!b.w_arr || (uintptr_t)b.w_arr % 16 == 0);
-  int wz = par_wave_zv(p, g, dose);
+  int wz = par_wave_zv(p, g, dose, gd.bp_zv);
   if (wz) {
     const uintptr_t al = (uintptr_t)target | (uintptr_t)operand | (uintptr_t)b.eps_arr | (uintptr_t)b.w_arr;
     while (wz > 1 && al % (4 * wz) != 0) wz >>= 1;
-    if ((wz == 1 && gd.nz % 64) || (wz == 2 && gd.nz % 128)) wz = 0;
+    if (gd.nz % (64 * wz) != 0) wz = 0;
   }
   if (wz) {
-    const long long threads = bp_tiled_pix(gd.n) * (gd.nz / (64 * wz)) * 64;
+    const long long threads = bp_tiled_pix(gd) * (gd.nz / (64 * wz)) * 64;
     nparts = (int)blocks_for(threads, 256);
     const dim3 grid(nparts), block(256);
     if (wz == 4)
@@ -408,9 +401,9 @@ extern "C" int vamx_bclp_step(vamx_plan* plan, const vamx_response* resp, const 
       rc = par_slice_minmax(plan, operand, s);
       if (rc) return rc;
     }
-    const int wz = par_wave_zv(plan, g, operand);
+    const int wz = par_wave_zv(plan, g, operand, gd.fp_zv);
     if (wz) {
-      const long long threads = fp_tiled_rays(gd.n, gd.na) * (gd.nz / (64 * wz)) * 64;
+      const long long threads = fp_tiled_rays(gd) * (gd.nz / (64 * wz)) * 64;
       const dim3 grid(blocks_for(threads, 256)), block(256);
       const int cl = clip ? 1 : 0;
       if (wz == 4)

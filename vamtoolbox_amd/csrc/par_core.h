@@ -595,53 +595,68 @@ __device__ __forceinline__ typename VecT<ZV>::T bp_voxel_wave(const ParDev& g, c
 }
 
 // L2-aware work order of the wavefront-uniform kernels.  About 1024 waves are
-// resident per XCD at once (32 CUs x 32 waves), and consecutive logical
-// blocks run on the same XCD (xcd_remap), so 1024 consecutive waves form one
-// compact tile: forward = 16 consecutive angles x 64 consecutive rays (their
-// samples at the same along-ray step lie within a few tens of pixels), back
-// projection = a 32 x 32 patch of voxel columns (whose detector footprint at
-// one angle is ~45 bins instead of a full row).  Padding waves exit.
-constexpr int FP_TILE_A = 16, FP_TILE_K = 64;
-constexpr int BP_TILE = 32;
+// resident per XCD at once (32 CUs x 32 waves) and consecutive logical blocks
+// run on the same XCD (xcd_remap), so consecutive waves should form a compact
+// tile whose data fits the XCD's 4 MB L2: forward = fp_ta consecutive angles x
+// fp_tk consecutive rays (their samples at one along-ray step lie within a few
+// tens of pixels), back projection = a bp_tile x bp_tile patch of voxel
+// columns (detector footprint ~1.4 bp_tile bins per angle).  When a ray (or
+// column) spans several z chunks, the chunk is the slowest index, so that all
+// resident waves share one z slab.  Padding waves exit.
+__host__ __device__ inline long long fp_tiled_rays(const ParDev& g) {
+  return (long long)((g.na + g.fp_ta - 1) / g.fp_ta) * ((g.n + g.fp_tk - 1) / g.fp_tk) * g.fp_ta * g.fp_tk;
+}
+__device__ __forceinline__ bool fp_tile_ray(long long q, const ParDev& g, int& k, int& i) {
+  const int ntk = (g.n + g.fp_tk - 1) / g.fp_tk;
+  const int tsz = g.fp_ta * g.fp_tk;
+  const long long tile = q / tsz;
+  const int within = (int)(q % tsz);
+  i = (int)(tile / ntk) * g.fp_ta + within / g.fp_tk;
+  k = (int)(tile % ntk) * g.fp_tk + within % g.fp_tk;
+  return i < g.na && k < g.n;
+}
+__host__ __device__ inline long long bp_tiled_pix(const ParDev& g) {
+  const long long t = (g.n + g.bp_tile - 1) / g.bp_tile;
+  return t * t * g.bp_tile * g.bp_tile;
+}
+__device__ __forceinline__ bool bp_tile_pix(long long q, const ParDev& g, int& pix) {
+  const int nt = (g.n + g.bp_tile - 1) / g.bp_tile;
+  const int tsz = g.bp_tile * g.bp_tile;
+  const long long tile = q / tsz;
+  const int within = (int)(q % tsz);
+  const int row = (int)(tile / nt) * g.bp_tile + within / g.bp_tile;
+  const int col = (int)(tile % nt) * g.bp_tile + within % g.bp_tile;
+  pix = row * g.n + col;
+  return row < g.n && col < g.n;
+}
 
-__host__ __device__ inline long long fp_tiled_rays(int n, int na) {
-  return (long long)((na + FP_TILE_A - 1) / FP_TILE_A) * ((n + FP_TILE_K - 1) / FP_TILE_K) * FP_TILE_A * FP_TILE_K;
-}
-__device__ __forceinline__ bool fp_tile_ray(long long q, int n, int na, int& k, int& i) {
-  const int ntk = (n + FP_TILE_K - 1) / FP_TILE_K;
-  const long long tile = q / (FP_TILE_A * FP_TILE_K);
-  const int within = (int)(q % (FP_TILE_A * FP_TILE_K));
-  const int ti = (int)(tile / ntk), tk = (int)(tile % ntk);
-  i = ti * FP_TILE_A + within / FP_TILE_K;
-  k = tk * FP_TILE_K + within % FP_TILE_K;
-  return i < na && k < n;
-}
-__host__ __device__ inline long long bp_tiled_pix(int n) {
-  const long long t = (n + BP_TILE - 1) / BP_TILE;
-  return t * t * BP_TILE * BP_TILE;
-}
-__device__ __forceinline__ bool bp_tile_pix(long long q, int n, int& pix) {
-  const int nt = (n + BP_TILE - 1) / BP_TILE;
-  const long long tile = q / (BP_TILE * BP_TILE);
-  const int within = (int)(q % (BP_TILE * BP_TILE));
-  const int row = (int)(tile / nt) * BP_TILE + within / BP_TILE;
-  const int col = (int)(tile % nt) * BP_TILE + within % BP_TILE;
-  pix = row * n + col;
-  return row < n && col < n;
+// wave -> (ray, z) of the forward kernels; false for padding waves
+template <int ZV>
+__device__ __forceinline__ bool fp_wave_item(long long wave, const ParDev& g, int lane, int& k, int& i, int& z) {
+  const long long rays = fp_tiled_rays(g);
+  const int nch = g.nz / (64 * ZV);
+  if (wave >= rays * nch) return false;
+  const int zc = (int)(wave / rays);
+  if (!fp_tile_ray(wave % rays, g, k, i)) return false;
+  z = zc * 64 * ZV + lane * ZV;
+  return true;
 }
 
-// ZV for the wavefront-uniform path (0: not applicable)
-__host__ __device__ inline int wave_zv(int nz, int n, bool incl) {
-  if (incl || n < 2) return 0;
-  if (nz % 256 == 0) return 4;
-  if (nz % 128 == 0) return 2;
-  if (nz % 64 == 0) return 1;
-  return 0;
+// wave -> (voxel column, z) of the back-projection kernels
+template <int ZV>
+__device__ __forceinline__ bool bp_wave_item(long long wave, const ParDev& g, int lane, int& pix, int& z) {
+  const long long cols = bp_tiled_pix(g);
+  const int nch = g.nz / (64 * ZV);
+  if (wave >= cols * nch) return false;
+  const int zc = (int)(wave / cols);
+  if (!bp_tile_pix(wave % cols, g, pix)) return false;
+  z = zc * 64 * ZV + lane * ZV;
+  return true;
 }
 
 // host helpers implemented in parallel.hip
 int par_slice_minmax(vamx_plan* p, const float* vol, hipStream_t s);
 bool par_use_vec4(const vamx_plan* p, const void* a, const void* b);
-int par_wave_zv(const vamx_plan* p, const void* a, const void* b);
+int par_wave_zv(const vamx_plan* p, const void* a, const void* b, int requested);
 
 }  // namespace vamx

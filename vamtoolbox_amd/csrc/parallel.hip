@@ -74,12 +74,8 @@ __global__ __launch_bounds__(256) void par_fp_wave_kernel(ParDev g, const float*
                                                           int clip) {
   const int lane = threadIdx.x & 63;
   const long long wave = ((long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
-  const int nch = g.nz / (64 * ZV);
-  if (wave >= fp_tiled_rays(g.n, g.na) * nch) return;
-  const int zc = (int)(wave % nch);
-  int k, i;
-  if (!fp_tile_ray(wave / nch, g.n, g.na, k, i)) return;
-  const int z = zc * 64 * ZV + lane * ZV;
+  int k, i, z;
+  if (!fp_wave_item<ZV>(wave, g, lane, k, i, z)) return;
   const bool clip_on = clip && reinterpret_cast<const int*>(minmax)[2 * g.nz] != 0;
   typename VecT<ZV>::T acc = clip_on ? fp_ray_wave<ZV, true>(g, vol, minmax, k, i, z, lane)
                                      : fp_ray_wave<ZV, false>(g, vol, minmax, k, i, z, lane);
@@ -92,12 +88,8 @@ __global__ __launch_bounds__(256) void par_bp_wave_kernel(ParDev g, const float*
                                                           float* __restrict__ vol, int clipc) {
   const int lane = threadIdx.x & 63;
   const long long wave = ((long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
-  const int nch = g.nz / (64 * ZV);
-  if (wave >= bp_tiled_pix(g.n) * nch) return;
-  const int zc = (int)(wave % nch);
-  int pix;
-  if (!bp_tile_pix(wave / nch, g.n, pix)) return;
-  const int z = zc * 64 * ZV + lane * ZV;
+  int pix, z;
+  if (!bp_wave_item<ZV>(wave, g, lane, pix, z)) return;
   float* outp = vol + (size_t)pix * g.nz + z;
   if (clipc && g.outside[pix]) {
     vstore<ZV>(outp, vzero<ZV>());
@@ -197,14 +189,13 @@ __global__ __launch_bounds__(256) void par_bpT_incl_kernel(ParDev g, const float
 // ===========================================================================
 using namespace vamx;
 
-int vamx::par_wave_zv(const vamx_plan* p, const void* a, const void* b) {
-  if (p->flags & VAMX_PAR_PER_LANE_GEOMETRY) return 0;
-  int zv = wave_zv(p->par.nz, p->par.n, p->par.incl != 0);
+int vamx::par_wave_zv(const vamx_plan* p, const void* a, const void* b, int requested) {
+  // largest z-vector width <= requested with nz a multiple of 64 * zv and aligned buffers
+  if ((p->flags & VAMX_PAR_PER_LANE_GEOMETRY) || p->par.incl || p->par.n < 2) return 0;
   const uintptr_t al = (uintptr_t)a | (uintptr_t)b;
-  while (zv > 1 && al % (4 * zv) != 0) zv >>= 1;
-  if (zv == 1 && p->par.nz % 64 != 0) return 0;
-  if (zv == 2 && p->par.nz % 128 != 0) return 0;
-  return zv;
+  for (int zv = requested; zv >= 1; zv >>= 1)
+    if (p->par.nz % (64 * zv) == 0 && al % (4 * zv) == 0) return zv;
+  return 0;
 }
 
 bool vamx::par_use_vec4(const vamx_plan* p, const void* a, const void* b) {
@@ -259,6 +250,11 @@ extern "C" int vamx_par_create(const vamx_par_geom* geom, vamx_plan** out_plan) 
   g.incl = geom->incl_deg != 0.0;
   g.mu = geom->mu;
   g.radius = geom->radius > 0.0 ? geom->radius : (n - 1) / 2.0;
+  g.fp_ta = 16;
+  g.fp_tk = 64;
+  g.fp_zv = 4;
+  g.bp_tile = 32;
+  g.bp_zv = 4;
   p->minmax_parts = 256;
   hipError_t e = hipSuccess;
   e = hipMalloc((void**)&p->d_mats, mats.size() * sizeof(double));
@@ -303,9 +299,9 @@ int par_forward_launch(vamx_plan* p, const float* vol, float* sino, hipStream_t 
     int rc = par_slice_minmax(p, vol, s);
     if (rc) return rc;
   }
-  const int wz = par_wave_zv(p, vol, sino);
+  const int wz = par_wave_zv(p, vol, sino, g.fp_zv);
   if (wz) {
-    const long long threads = fp_tiled_rays(g.n, g.na) * (g.nz / (64 * wz)) * 64;
+    const long long threads = fp_tiled_rays(g) * (g.nz / (64 * wz)) * 64;
     const dim3 grid(blocks_for(threads, 256)), block(256);
     const int clip = warp_clip ? 1 : 0;
     if (wz == 4)
@@ -345,9 +341,9 @@ int par_backward_launch(vamx_plan* p, const float* sino, float* vol, hipStream_t
     VAMX_HIP(hipGetLastError());
     return VAMX_OK;
   }
-  const int wz = par_wave_zv(p, sino, vol);
+  const int wz = par_wave_zv(p, sino, vol, g.bp_zv);
   if (wz) {
-    const long long threads = bp_tiled_pix(g.n) * (g.nz / (64 * wz)) * 64;
+    const long long threads = bp_tiled_pix(g) * (g.nz / (64 * wz)) * 64;
     const dim3 grid(blocks_for(threads, 256)), block(256);
     const int cc = clipc ? 1 : 0;
     if (wz == 4)
@@ -429,5 +425,31 @@ extern "C" int vamx_par_backward_adjoint(vamx_plan* plan, const float* vol, floa
     hipLaunchKernelGGL((par_bpT_kernel<1>), dim3(blocks_for(total, 256)), dim3(256), 0, s, g, vol, sino, clipc);
   }
   VAMX_HIP(hipGetLastError());
+  return VAMX_OK;
+}
+
+extern "C" int vamx_par_set_tuning(vamx_plan* plan, const vamx_par_tuning* t) {
+  if (!check_plan(plan, PLAN_PARALLEL)) return set_error(VAMX_ERR_STATE, "vamx_par_set_tuning: not a parallel plan");
+  if (!t) return set_error(VAMX_ERR_ARG, "vamx_par_set_tuning: null tuning");
+  auto pow2 = [](int v) { return v == 1 || v == 2 || v == 4; };
+  if (t->fp_tile_angles < 1 || t->fp_tile_rays < 1 || t->bp_tile < 1 || !pow2(t->fp_zv) || !pow2(t->bp_zv) ||
+      t->fp_tile_angles * t->fp_tile_rays > (1 << 20) || t->bp_tile > 1024)
+    return set_error(VAMX_ERR_ARG, "vamx_par_set_tuning: tiles must be >= 1 and z widths 1, 2 or 4");
+  plan->par.fp_ta = t->fp_tile_angles;
+  plan->par.fp_tk = t->fp_tile_rays;
+  plan->par.fp_zv = t->fp_zv;
+  plan->par.bp_tile = t->bp_tile;
+  plan->par.bp_zv = t->bp_zv;
+  return VAMX_OK;
+}
+
+extern "C" int vamx_par_get_tuning(vamx_plan* plan, vamx_par_tuning* t) {
+  if (!check_plan(plan, PLAN_PARALLEL)) return set_error(VAMX_ERR_STATE, "vamx_par_get_tuning: not a parallel plan");
+  if (!t) return set_error(VAMX_ERR_ARG, "vamx_par_get_tuning: null tuning");
+  t->fp_tile_angles = plan->par.fp_ta;
+  t->fp_tile_rays = plan->par.fp_tk;
+  t->fp_zv = plan->par.fp_zv;
+  t->bp_tile = plan->par.bp_tile;
+  t->bp_zv = plan->par.bp_zv;
   return VAMX_OK;
 }

@@ -25,6 +25,9 @@ struct ParDev {
   const double* mats;        // [na][6]: R00 R01 R02 R10 R11 R12 (Projector3DParallel.py:137-143)
   const double* cs;          // [na][2]: cos, sin of deg2rad(angle)
   const uint8_t* outside;    // [n*n] clipToCircle mask
+  // work order of the wavefront-uniform kernels (set by vamx_par_set_tuning)
+  int fp_ta, fp_tk, fp_zv;   // forward: tile of fp_ta angles x fp_tk rays, z values per lane
+  int bp_tile, bp_zv;        // back projection: bp_tile x bp_tile voxel columns, z values per lane
 };
 
 // Geometry of the curved-ray tracer, passed by value.

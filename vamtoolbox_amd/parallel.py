@@ -109,3 +109,65 @@ class ParallelPlan:
         vol = self._in(vol, self.vol_shape)
         out = torch.empty(self.sino_shape, device=self.device, dtype=torch.float32) if out is None else out
         return self._run("vamx_par_backward_adjoint", vol, out)
+
+    # -- work-order tuning -----------------------------------------------------
+    def get_tuning(self) -> dict:
+        t = _lib.ParTuning()
+        _lib.check(self.lib.vamx_par_get_tuning(self.handle, ctypes.byref(t)), "vamx_par_get_tuning")
+        return {f: int(getattr(t, f)) for f, _ in _lib.ParTuning._fields_}
+
+    def set_tuning(self, **kw) -> None:
+        cur = self.get_tuning()
+        cur.update({k: int(v) for k, v in kw.items()})
+        t = _lib.ParTuning(**cur)
+        _lib.check(self.lib.vamx_par_set_tuning(self.handle, ctypes.byref(t)), "vamx_par_set_tuning")
+
+    FP_CANDIDATES = [(16, 64, 4), (8, 128, 4), (32, 32, 4), (16, 64, 2), (8, 64, 2), (32, 32, 2), (16, 32, 2),
+                     (8, 32, 2), (64, 16, 2), (16, 64, 1), (8, 64, 1)]
+    BP_CANDIDATES = [(32, 4), (16, 4), (64, 4), (32, 2), (16, 2), (24, 4)]
+
+    def autotune(self, vol: torch.Tensor | None = None, sino: torch.Tensor | None = None, reps: int = 3) -> dict:
+        """Time the forward and back projection for each candidate work order
+        (tile shape x z values per lane) on this plan's shape and keep the
+        fastest of each; returns {setting: time_ms} plus the choice.  Results
+        are cached per (n, nz, n_angles, absorption on/off)."""
+        key = (self.n, self.nz, self.na, self.mu != 0.0, self.incl_deg != 0.0)
+        if key in _TUNING_CACHE:
+            self.set_tuning(**_TUNING_CACHE[key]["choice"])
+            return _TUNING_CACHE[key]
+        dev = self.device
+        if vol is None:
+            vol = torch.rand(self.vol_shape, device=dev)
+        if sino is None:
+            sino = torch.rand(self.sino_shape, device=dev)
+        out_s = torch.empty(self.sino_shape, device=dev)
+        out_v = torch.empty(self.vol_shape, device=dev)
+
+        def timed(fn):
+            fn()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(reps):
+                fn()
+            b.record()
+            torch.cuda.synchronize(dev)
+            return a.elapsed_time(b) / reps
+
+        report = {"forward": {}, "backward": {}}
+        for ta, tk, zv in self.FP_CANDIDATES:
+            self.set_tuning(fp_tile_angles=ta, fp_tile_rays=tk, fp_zv=zv)
+            report["forward"][(ta, tk, zv)] = timed(lambda: self.forward(vol, out=out_s))
+        for tile, zv in self.BP_CANDIDATES:
+            self.set_tuning(bp_tile=tile, bp_zv=zv)
+            report["backward"][(tile, zv)] = timed(lambda: self.backward(sino, out=out_v))
+        fbest = min(report["forward"], key=report["forward"].get)
+        bbest = min(report["backward"], key=report["backward"].get)
+        choice = dict(fp_tile_angles=fbest[0], fp_tile_rays=fbest[1], fp_zv=fbest[2], bp_tile=bbest[0],
+                      bp_zv=bbest[1])
+        self.set_tuning(**choice)
+        report["choice"] = choice
+        _TUNING_CACHE[key] = report
+        return report
+
+
+_TUNING_CACHE: dict = {}
