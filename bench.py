@@ -197,6 +197,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-torch-baseline", action="store_true")
     ap.add_argument("--lr", type=float, default=0.01, help="BCLP learning rate (the reference's default)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time the candidate tile shapes first (the library defaults are the autotuner's choice "
+                         "for this workload)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="run only the forward / back projection kernels of the workload (5 launches each) "
                          "and a 1 GiB device copy, for hardware-counter profiling")
@@ -232,11 +235,14 @@ def main():
     g0 = (g0 * (0.7 / float(d0[T > 0.5].mean()))).contiguous()
     lr = args.lr
 
-    log("autotuning the work order")
-    rep = P.plan.autotune(T, torch.rand(P.plan.sino_shape, device=dev), reps=2)
-    log("tuning choice %s (forward %.3f ms, backward %.3f ms)" % (
-        rep["choice"], min(rep["forward"].values()), min(rep["backward"].values())))
-    tuning = rep["choice"]
+    if args.autotune:
+        log("autotuning the work order")
+        rep = P.plan.autotune(T, torch.rand(P.plan.sino_shape, device=dev), reps=3)
+        for what in ("forward", "backward"):
+            log("  %s: %s" % (what, ", ".join("%s %.3f ms" % (k, v) for k, v in sorted(rep[what].items(),
+                                                                                   key=lambda kv: kv[1]))))
+        log("tuning choice %s" % rep["choice"])
+    tuning = P.plan.get_tuning()
     if args.kernels_only:
         plan = P.plan
         sino = torch.empty(plan.sino_shape, device=dev)

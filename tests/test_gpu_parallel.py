@@ -184,6 +184,11 @@ def test_tuning_settings_agree(gpu):
         e1 = rel_l2(plan.forward(tv).cpu().numpy(), ref_f)
         e2 = rel_l2(plan.backward(ts).cpu().numpy(), ref_b)
         assert e1 <= TOL and e2 <= TOL, (ta, tk, zv, tile, bzv, e1, e2)
+    # the tile shape changes the order waves run in, not any value
+    plan.set_tuning(fp_tile_angles=32, fp_tile_rays=32, fp_zv=2, bp_tile=16, bp_zv=2)
+    a, b = plan.forward(tv), plan.backward(ts)
+    plan.set_tuning(fp_tile_angles=8, fp_tile_rays=64, bp_tile=7)
+    assert torch.equal(a, plan.forward(tv)) and torch.equal(b, plan.backward(ts))
     rep = plan.autotune(tv, ts, reps=1)
     assert set(rep["choice"]) == {"fp_tile_angles", "fp_tile_rays", "fp_zv", "bp_tile", "bp_zv"}
     assert plan.get_tuning() == rep["choice"]

@@ -191,7 +191,7 @@ __global__ __launch_bounds__(256) void bclp_fp_update_wave(ParDev g, const float
   const long long wave = ((long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
   int k, i, z;
   if (!fp_wave_item<ZV>(wave, g, lane, k, i, z)) return;
-  const bool clip_on = clip && reinterpret_cast<const int*>(minmax)[2 * g.nz] != 0;
+  const bool clip_on = clip && wave_clip_active<ZV>(minmax, g.nz, z);
   const typename VecT<ZV>::T fp = clip_on ? fp_ray_wave<ZV, true>(g, operand, minmax, k, i, z, lane)
                                           : fp_ray_wave<ZV, false>(g, operand, minmax, k, i, z, lane);
   const float scale = (float)(*scale_p);

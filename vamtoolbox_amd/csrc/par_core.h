@@ -468,6 +468,18 @@ __device__ __forceinline__ float rl_f(float v, int lane) {
 
 // slot weights of one axis: the taps floor / floor+1 (skimage's floor / ceil
 // with weights 1-d / d) mapped onto the pair (b, b+1) with b in [0, n-2]
+// The warp clip of a slice is a no-op when 0 lies in [min, max] of the slice:
+// bilinear samples are convex combinations of the slice's values and the zero
+// padding.  Wave-uniform test over the wave's 64*ZV slices.
+template <int ZV>
+__device__ __forceinline__ bool wave_clip_active(const float* __restrict__ minmax, int nz, int z) {
+  const typename VecT<ZV>::T lo = vload<ZV>(minmax + z), hi = vload<ZV>(minmax + nz + z);
+  bool need = false;
+#pragma unroll
+  for (int q = 0; q < ZV; ++q) need |= !(vget(lo, q) <= 0.0f && 0.0f <= vget(hi, q));
+  return __ballot(need) != 0ull;
+}
+
 __device__ __forceinline__ void axis_slots(double coord, int n, int& b, double& s0, double& s1) {
   const double f = floor(coord);
   const double d = coord - f;

@@ -11,44 +11,72 @@ namespace vamx {
 // ---------------------------------------------------------------------------
 // per-slice min / max of a volume (skimage warp(..., clip=True) output clip)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void slice_minmax_part(const float* __restrict__ vol, int npix, int nz,
-                                                         int parts, float* __restrict__ part) {
-  const int pb = blockIdx.x;
+// Per-slice min / max of a volume (the warp clip range), pass 1: block b
+// reduces pixels [b*chunk, (b+1)*chunk) for every z.  Threads form groups of
+// min(nz, 1024) consecutive z values; the groups split the block's pixels and
+// are combined through LDS.
+__global__ __launch_bounds__(1024) void slice_minmax_part(const float* __restrict__ vol, int npix, int nz,
+                                                          int parts, float* __restrict__ part) {
+  __shared__ float slo[1024], shi[1024];
+  const int t = threadIdx.x;
+  const int zs = nz < 1024 ? nz : 1024;
+  const int groups = 1024 / zs;
+  const int grp = t / zs, zt = t % zs;
   const int chunk = (npix + parts - 1) / parts;
-  const int p0 = pb * chunk, p1 = min(npix, p0 + chunk);
-  for (int z = threadIdx.x; z < nz; z += blockDim.x) {
+  const int p0 = blockIdx.x * chunk, p1 = min(npix, p0 + chunk);
+  for (int z0 = 0; z0 < nz; z0 += zs) {
+    const int z = z0 + zt;
     float lo = INFINITY, hi = -INFINITY;
-    for (int p = p0; p < p1; ++p) {
-      const float v = vol[(size_t)p * nz + z];
-      lo = fminf(lo, v);
-      hi = fmaxf(hi, v);
+    if (grp < groups && z < nz) {
+#pragma unroll 8
+      for (int p = p0 + grp; p < p1; p += groups) {
+        const float v = vol[(size_t)p * nz + z];
+        lo = fminf(lo, v);
+        hi = fmaxf(hi, v);
+      }
     }
-    part[(size_t)pb * 2 * nz + z] = lo;
-    part[(size_t)pb * 2 * nz + nz + z] = hi;
+    slo[t] = lo;
+    shi[t] = hi;
+    __syncthreads();
+    if (grp == 0 && z < nz) {
+      for (int q = 1; q < groups; ++q) {
+        lo = fminf(lo, slo[q * zs + zt]);
+        hi = fmaxf(hi, shi[q * zs + zt]);
+      }
+      part[(size_t)blockIdx.x * 2 * nz + z] = lo;
+      part[(size_t)blockIdx.x * 2 * nz + nz + z] = hi;
+    }
+    __syncthreads();
   }
 }
 
-__global__ __launch_bounds__(256) void slice_minmax_final(const float* __restrict__ part, int nz, int parts,
-                                                          float* __restrict__ out) {
-  // one block: out[0:nz] = min, out[nz:2nz] = max, and an int flag at out[2nz]
-  // telling whether any slice lacks 0 in its range (the clip is then active)
-  __shared__ int any_sh;
-  if (threadIdx.x == 0) any_sh = 0;
-  __syncthreads();
-  int any = 0;
-  for (int z = threadIdx.x; z < nz; z += blockDim.x) {
-    float lo = INFINITY, hi = -INFINITY;
-    for (int b = 0; b < parts; ++b) {
+// pass 2: block b owns z in [64b, 64b+64); 16 groups of 64 threads split the
+// parts.  out[0:nz] = min, out[nz:2nz] = max.
+__global__ __launch_bounds__(1024) void slice_minmax_final(const float* __restrict__ part, int nz, int parts,
+                                                           float* __restrict__ out) {
+  __shared__ float slo[1024], shi[1024];
+  const int t = threadIdx.x;
+  const int zt = t & 63, grp = t >> 6;
+  const int z = blockIdx.x * 64 + zt;
+  float lo = INFINITY, hi = -INFINITY;
+  if (z < nz) {
+#pragma unroll 4
+    for (int b = grp; b < parts; b += 16) {
       lo = fminf(lo, part[(size_t)b * 2 * nz + z]);
       hi = fmaxf(hi, part[(size_t)b * 2 * nz + nz + z]);
     }
+  }
+  slo[t] = lo;
+  shi[t] = hi;
+  __syncthreads();
+  if (grp == 0 && z < nz) {
+    for (int q = 1; q < 16; ++q) {
+      lo = fminf(lo, slo[q * 64 + zt]);
+      hi = fmaxf(hi, shi[q * 64 + zt]);
+    }
     out[z] = lo;
     out[nz + z] = hi;
-    if (!(lo <= 0.0f && 0.0f <= hi)) any = 1;
   }
-  if (any) any_sh = 1;  // benign same-value race in LDS
-  __syncthreads();
-  if (threadIdx.x == 0) reinterpret_cast<int*>(out)[2 * nz] = any_sh;
 }
 
 // ---------------------------------------------------------------------------
@@ -76,7 +104,7 @@ __global__ __launch_bounds__(256) void par_fp_wave_kernel(ParDev g, const float*
   const long long wave = ((long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
   int k, i, z;
   if (!fp_wave_item<ZV>(wave, g, lane, k, i, z)) return;
-  const bool clip_on = clip && reinterpret_cast<const int*>(minmax)[2 * g.nz] != 0;
+  const bool clip_on = clip && wave_clip_active<ZV>(minmax, g.nz, z);
   typename VecT<ZV>::T acc = clip_on ? fp_ray_wave<ZV, true>(g, vol, minmax, k, i, z, lane)
                                      : fp_ray_wave<ZV, false>(g, vol, minmax, k, i, z, lane);
   vstore<ZV>(sino + ((size_t)k * g.na + i) * g.nz + z, acc);
@@ -250,10 +278,11 @@ extern "C" int vamx_par_create(const vamx_par_geom* geom, vamx_plan** out_plan) 
   g.incl = geom->incl_deg != 0.0;
   g.mu = geom->mu;
   g.radius = geom->radius > 0.0 ? geom->radius : (n - 1) / 2.0;
-  g.fp_ta = 16;
-  g.fp_tk = 64;
+  // defaults: ParallelPlan.autotune's choice at 256^3 x 360 angles on MI355X
+  g.fp_ta = 32;
+  g.fp_tk = 32;
   g.fp_zv = 4;
-  g.bp_tile = 32;
+  g.bp_tile = 16;
   g.bp_zv = 4;
   p->minmax_parts = 256;
   hipError_t e = hipSuccess;
@@ -279,10 +308,10 @@ extern "C" int vamx_par_create(const vamx_par_geom* geom, vamx_plan** out_plan) 
 namespace vamx {
 int par_slice_minmax(vamx_plan* p, const float* vol, hipStream_t s) {
   const ParDev& g = p->par;
-  hipLaunchKernelGGL(slice_minmax_part, dim3(p->minmax_parts), dim3(256), 0, s, vol, g.n * g.n, g.nz,
+  hipLaunchKernelGGL(slice_minmax_part, dim3(p->minmax_parts), dim3(1024), 0, s, vol, g.n * g.n, g.nz,
                      p->minmax_parts, p->d_minmax_part);
-  hipLaunchKernelGGL(slice_minmax_final, dim3(1), dim3(256), 0, s, p->d_minmax_part, g.nz, p->minmax_parts,
-                     p->d_minmax);
+  hipLaunchKernelGGL(slice_minmax_final, dim3((g.nz + 63) / 64), dim3(1024), 0, s, p->d_minmax_part, g.nz,
+                     p->minmax_parts, p->d_minmax);
   VAMX_HIP(hipGetLastError());
   return VAMX_OK;
 }

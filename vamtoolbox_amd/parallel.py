@@ -122,8 +122,9 @@ class ParallelPlan:
         t = _lib.ParTuning(**cur)
         _lib.check(self.lib.vamx_par_set_tuning(self.handle, ctypes.byref(t)), "vamx_par_set_tuning")
 
-    FP_CANDIDATES = [(16, 64, 4), (8, 128, 4), (32, 32, 4), (16, 64, 2), (8, 64, 2), (32, 32, 2), (16, 32, 2),
-                     (8, 32, 2), (64, 16, 2), (16, 64, 1), (8, 64, 1)]
+    # (angles per tile, rays per tile, z values per lane)
+    FP_CANDIDATES = [(32, 32, 4), (16, 64, 4), (8, 128, 4), (64, 16, 4), (16, 32, 4), (32, 32, 2), (16, 64, 2),
+                     (64, 16, 2), (32, 32, 1), (16, 64, 1)]
     BP_CANDIDATES = [(32, 4), (16, 4), (64, 4), (32, 2), (16, 2), (24, 4)]
 
     def autotune(self, vol: torch.Tensor | None = None, sino: torch.Tensor | None = None, reps: int = 3) -> dict:
