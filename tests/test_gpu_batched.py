@@ -5,6 +5,8 @@ import ctypes
 import numpy as np
 import pytest
 
+from tests.conftest import rel_l2
+
 pytestmark = pytest.mark.gpu
 
 
@@ -39,3 +41,55 @@ def test_batched_equals_individual(gpu):
     for i in range(k):
         assert torch.equal(sinos[i], ref_s[i]), i
         assert torch.equal(bps[i][1], ref_b[i]), i
+
+
+def _osmo_problems():
+    import vamtoolbox_amd as vam
+    from tests.test_gpu_optim import _ellipsoids
+    specs = [(24, 3, 30, 0.0), (40, 5, 45, 0.01), (32, 8, 60, 0.0), (16, 2, 20, 0.02)]
+    tgs, pgs = [], []
+    for s, (n, nz, na, mu) in enumerate(specs):
+        tgs.append(vam.geometry.TargetGeometry(target=_ellipsoids(n, nz, seed=20 + s)))
+        pgs.append(vam.geometry.ProjectionGeometry(np.linspace(0, 360, na, endpoint=False), "parallel", CUDA=True,
+                                                   absorption_per_voxel=mu))
+    return tgs, pgs
+
+
+def test_batched_osmo_matches_single_problems(gpu):
+    """Batched OSMO (graph-replayed) = OSMO on each problem alone."""
+    import vamtoolbox_amd as vam
+    from vamtoolbox_amd.batch import minimizeOSMOBatched
+    tgs, pgs = _osmo_problems()
+    opts = vam.optimize.Options(method="OSMO", n_iter=6, d_h=0.85, d_l=0.6, filter="ram-lak")
+    res = minimizeOSMOBatched(tgs, pgs, opts, hip_graph=True)
+    for i, (tg, pg) in enumerate(zip(tgs, pgs)):
+        sino, recon, err = vam.optimize.optimize(tg, pg, vam.optimize.Options(method="OSMO", n_iter=6, d_h=0.85,
+                                                                             d_l=0.6, filter="ram-lak"))
+        b, x, e = res[i]
+        eb, ex = rel_l2(b.array, sino.array), rel_l2(x.array, recon.array)
+        print("problem %d: rel L2 sinogram %.2e recon %.2e" % (i, eb, ex))
+        assert eb <= 1e-6 and ex <= 1e-6
+        np.testing.assert_allclose(e, err, atol=1e-6)
+
+
+def test_batched_osmo_graph_replays_equal_eager(gpu):
+    """Every replay of the captured iteration equals the eager iteration, with
+    eager work and host synchronisation between the replays."""
+    import torch
+    import vamtoolbox_amd as vam
+    from vamtoolbox_amd.batch import OSMOBatch
+    tgs, pgs = _osmo_problems()
+    opts = vam.optimize.Options(method="OSMO", n_iter=6, d_h=0.85, d_l=0.6, filter="ram-lak")
+    eager = OSMOBatch(tgs, pgs, opts, device=gpu)
+    graph = OSMOBatch(tgs, pgs, opts, device=gpu)
+    eager.step()
+    graph.step()
+    graph.capture()
+    for it in range(4):
+        eager.step()
+        torch.cuda.synchronize()
+        graph.graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(eager.x, graph.x), it
+        assert torch.equal(eager.b, graph.b), it
+        assert torch.equal(eager.x_model, graph.x_model), it

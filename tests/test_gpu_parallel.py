@@ -179,16 +179,17 @@ def test_tuning_settings_agree(gpu):
     ts = torch.tensor(sino, dtype=torch.float32, device=gpu)
     plan = ParallelPlan(n, nz, ang, device=gpu, mu=0.02)
     ref_f, ref_b = d1.forward(vol, ang, mu=0.02), d1.backward(sino, ang, mu=0.02)
-    for (ta, tk, zv), (tile, bzv) in zip(ParallelPlan.FP_CANDIDATES, itertools.cycle(ParallelPlan.BP_CANDIDATES)):
-        plan.set_tuning(fp_tile_angles=ta, fp_tile_rays=tk, fp_zv=zv, bp_tile=tile, bp_zv=bzv)
+    for (ta, tk, zv, ru), (tile, bzv) in zip(ParallelPlan.FP_CANDIDATES, itertools.cycle(ParallelPlan.BP_CANDIDATES)):
+        plan.set_tuning(fp_tile_angles=ta, fp_tile_rays=tk, fp_zv=zv, fp_reuse=ru, bp_tile=tile, bp_zv=bzv)
         e1 = rel_l2(plan.forward(tv).cpu().numpy(), ref_f)
         e2 = rel_l2(plan.backward(ts).cpu().numpy(), ref_b)
-        assert e1 <= TOL and e2 <= TOL, (ta, tk, zv, tile, bzv, e1, e2)
-    # the tile shape changes the order waves run in, not any value
-    plan.set_tuning(fp_tile_angles=32, fp_tile_rays=32, fp_zv=2, bp_tile=16, bp_zv=2)
+        assert e1 <= TOL and e2 <= TOL, (ta, tk, zv, ru, tile, bzv, e1, e2)
+    # the tile shape changes the order waves run in and tap reuse which
+    # instruction loads a tap, not any value
+    plan.set_tuning(fp_tile_angles=32, fp_tile_rays=32, fp_zv=2, fp_reuse=0, bp_tile=16, bp_zv=2)
     a, b = plan.forward(tv), plan.backward(ts)
-    plan.set_tuning(fp_tile_angles=8, fp_tile_rays=64, bp_tile=7)
+    plan.set_tuning(fp_tile_angles=8, fp_tile_rays=64, fp_reuse=1, bp_tile=7)
     assert torch.equal(a, plan.forward(tv)) and torch.equal(b, plan.backward(ts))
     rep = plan.autotune(tv, ts, reps=1)
-    assert set(rep["choice"]) == {"fp_tile_angles", "fp_tile_rays", "fp_zv", "bp_tile", "bp_zv"}
+    assert set(rep["choice"]) == {"fp_tile_angles", "fp_tile_rays", "fp_zv", "fp_reuse", "bp_tile", "bp_zv"}
     assert plan.get_tuning() == rep["choice"]

@@ -41,6 +41,7 @@ class ParTuning(ctypes.Structure):
         ("fp_tile_angles", c_int32),
         ("fp_tile_rays", c_int32),
         ("fp_zv", c_int32),
+        ("fp_reuse", c_int32),
         ("bp_tile", c_int32),
         ("bp_zv", c_int32),
     ]

@@ -1,0 +1,249 @@
+"""Many independent parallel-beam problems of uneven size, projected in one
+persistent launch, and the OSMO iteration over all of them at once.
+
+VAMToolbox solves one target at a time (vamtoolbox/optimize.py:174-217).  A
+batch holds k problems -- each its own ``ParallelPlan`` with its own size,
+angle set and absorption -- with all volumes in one flat float32 buffer and
+all sinograms in another.  ``ParallelBatch.forward`` / ``backward`` project
+every problem with ``vamx_par_forward_batched`` / ``vamx_par_backward_batched``:
+one launch whose blocks take work tiles from a device-side queue, largest
+problem first, so small problems fill the gaps the large ones leave.
+
+``minimizeOSMOBatched`` runs the OSMO iteration of
+vamtoolbox/optimizer/OSMO.py:9-178 for every problem (the same steps as
+``vamtoolbox_amd.optimizer.OSMO.minimizeOSMO`` on each problem alone) with
+the element-wise steps on the flat buffers and the projections batched; the
+iteration is captured once in a HIP graph and replayed.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import time
+
+import numpy as np
+import torch
+
+from . import _lib, geometry, metrics
+from .parallel import ParallelPlan
+from .util import data
+
+
+class ParallelBatch:
+    """k parallel-beam plans on one GPU sharing flat volume / sinogram buffers.
+
+    Problem i's volume is ``vol_flat[vol_off[i] : vol_off[i] + vol_size[i]]``
+    in its plan's (n, n, nz) layout, and likewise for the sinogram.
+    """
+
+    def __init__(self, plans):
+        self.plans = list(plans)
+        if not self.plans:
+            raise ValueError("ParallelBatch needs at least one plan")
+        self.device = self.plans[0].device
+        for p in self.plans:
+            if p.device != self.device:
+                raise ValueError("all plans of a batch must live on one device")
+        self.k = len(self.plans)
+        self.lib = _lib.load()
+        self.vol_size = [int(np.prod(p.vol_shape)) for p in self.plans]
+        self.sino_size = [int(np.prod(p.sino_shape)) for p in self.plans]
+        self.vol_off = np.concatenate([[0], np.cumsum(self.vol_size)[:-1]]).astype(np.int64)
+        self.sino_off = np.concatenate([[0], np.cumsum(self.sino_size)[:-1]]).astype(np.int64)
+        self.n_vol = int(sum(self.vol_size))
+        self.n_sino = int(sum(self.sino_size))
+        self._handles = (ctypes.c_void_p * self.k)(*[p.handle.value for p in self.plans])
+        self._ptrs = {}
+
+    # -- buffers -----------------------------------------------------------
+    def empty_vol(self) -> torch.Tensor:
+        return torch.empty(self.n_vol, device=self.device, dtype=torch.float32)
+
+    def empty_sino(self) -> torch.Tensor:
+        return torch.empty(self.n_sino, device=self.device, dtype=torch.float32)
+
+    def vol_views(self, flat: torch.Tensor):
+        return [flat[o:o + s].view(p.vol_shape) for o, s, p in zip(self.vol_off, self.vol_size, self.plans)]
+
+    def sino_views(self, flat: torch.Tensor):
+        return [flat[o:o + s].view(p.sino_shape) for o, s, p in zip(self.sino_off, self.sino_size, self.plans)]
+
+    def pack_vols(self, vols) -> torch.Tensor:
+        flat = self.empty_vol()
+        for v, dst in zip(vols, self.vol_views(flat)):
+            dst.copy_(torch.as_tensor(np.asarray(v) if not isinstance(v, torch.Tensor) else v,
+                                      dtype=torch.float32).reshape(dst.shape))
+        return flat
+
+    def _pointer_table(self, t: torch.Tensor, offs):
+        key = (t.data_ptr(), id(offs))
+        tab = self._ptrs.get(key)
+        if tab is None:
+            base = t.data_ptr()
+            tab = (ctypes.c_void_p * self.k)(*[base + 4 * int(o) for o in offs])
+            if len(self._ptrs) > 64:
+                self._ptrs.clear()
+            self._ptrs[key] = tab
+        return tab
+
+    def _check(self, t: torch.Tensor, n: int, what: str):
+        if not isinstance(t, torch.Tensor) or t.device != self.device or t.dtype != torch.float32 or \
+                not t.is_contiguous() or t.numel() != n:
+            raise ValueError("%s must be a contiguous float32 tensor of %d elements on %s" % (what, n, self.device))
+
+    # -- operators -----------------------------------------------------------
+    def forward(self, vol_flat: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """Sinograms of every problem (one persistent launch)."""
+        self._check(vol_flat, self.n_vol, "vol_flat")
+        out = self.empty_sino() if out is None else out
+        self._check(out, self.n_sino, "out")
+        _lib.check(self.lib.vamx_par_forward_batched(self._handles, self.k, self._pointer_table(vol_flat, self.vol_off),
+                                                     self._pointer_table(out, self.sino_off),
+                                                     _lib.stream_ptr(self.device)), "vamx_par_forward_batched")
+        return out
+
+    def backward(self, sino_flat: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """Back projections of every problem (one persistent launch)."""
+        self._check(sino_flat, self.n_sino, "sino_flat")
+        out = self.empty_vol() if out is None else out
+        self._check(out, self.n_vol, "out")
+        _lib.check(self.lib.vamx_par_backward_batched(self._handles, self.k,
+                                                      self._pointer_table(sino_flat, self.sino_off),
+                                                      self._pointer_table(out, self.vol_off),
+                                                      _lib.stream_ptr(self.device)), "vamx_par_backward_batched")
+        return out
+
+
+def _plan_for(target_geo, proj_geo, device):
+    arr = np.asarray(target_geo.array)
+    n = arr.shape[0]
+    if arr.shape[1] != n:
+        raise ValueError("the parallel-beam projector needs square slices (nY == nX)")
+    nz = arr.shape[2] if arr.ndim == 3 else 1
+    if getattr(proj_geo, "zero_dose_sino", None) is not None or \
+            (proj_geo.absorption_coeff is not None and getattr(proj_geo, "absorption_mask", None) is not None):
+        raise NotImplementedError("batched OSMO: zero-dose constraints and absorption masks are per-problem "
+                                  "options of minimizeOSMO; use absorption_per_voxel for absorption")
+    return ParallelPlan(n, nz, np.asarray(proj_geo.angles, dtype=np.float64), det_offset=proj_geo.det_offset,
+                        incl_deg=float(proj_geo.inclination_angle or 0.0), mu=proj_geo.absorption_per_voxel,
+                        radius=proj_geo.absorption_radius, device=device)
+
+
+class OSMOBatch:
+    """State of the batched OSMO iteration (OSMO.py:9-178 for every problem).
+
+    ``step()`` enqueues one iteration (void step then gel step, each one
+    batched forward and one batched back projection with the per-problem max
+    normalisations) without host synchronisation; ``run(n)`` replays it,
+    from a captured HIP graph when ``hip_graph`` is set.
+    """
+
+    def __init__(self, target_geos, proj_geos, options, device=None):
+        from ._device import require_gpu
+        self.device = require_gpu(device)
+        self.target_geos = list(target_geos)
+        self.proj_geos = list(proj_geos)
+        if len(self.target_geos) != len(self.proj_geos):
+            raise ValueError("one projection geometry per target")
+        self.options = options
+        self.batch = ParallelBatch([_plan_for(t, p, self.device) for t, p in zip(self.target_geos, self.proj_geos)])
+        B = self.batch
+        dev = self.device
+        self.d_h, self.d_l = float(options.d_h), float(options.d_l)
+        self.min_proj_val = float(options.inhibition)
+        if options.bit_depth is not None:
+            raise NotImplementedError("batched OSMO: bit_depth discretisation is a per-problem option of minimizeOSMO")
+        gel = torch.zeros(B.n_vol, device=dev, dtype=torch.bool)
+        void = torch.zeros(B.n_vol, device=dev, dtype=torch.bool)
+        for t, g_, v_ in zip(self.target_geos, B.vol_views(gel), B.vol_views(void)):
+            g_.copy_(torch.as_tensor(np.asarray(t.gel_inds).reshape(g_.shape)))
+            v_.copy_(torch.as_tensor(np.asarray(t.void_inds).reshape(v_.shape)))
+        self.gel, self.void = gel, void
+        # initialisation (OSMO.py:128-140), per problem
+        self.x_model = B.empty_vol()
+        for t, xm in zip(self.target_geos, B.vol_views(self.x_model)):
+            T = torch.as_tensor(np.asarray(t.array, dtype=np.float64), device=dev)
+            f = data.filterTargetOSMO(T, options.filter)
+            xm.copy_((f.real if torch.is_complex(f) else f).reshape(xm.shape).to(torch.float32))
+        self.b = B.forward(self.x_model)
+        self.b.clamp_(min=0.0)
+        self.x = B.backward(self.b)
+        self._normalise(self.x, B.vol_views(self.x), clamp_min=None)
+        self.x_views = B.vol_views(self.x)
+        self.b_views = B.sino_views(self.b)
+        self.graph = None
+
+    @staticmethod
+    def _normalise(flat, views, clamp_min):
+        for v in views:
+            v.div_(torch.amax(v))
+        if clamp_min is not None:
+            flat.clamp_(min=clamp_min)
+
+    def _project(self):
+        B = self.batch
+        B.forward(self.x_model, out=self.b)
+        for v in self.b_views:
+            v.div_(torch.amax(v))
+        self.b.clamp_(min=self.min_proj_val)      # b < inhibition -> inhibition (OSMO.py:63-66)
+        B.backward(self.b, out=self.x)
+        for v in self.x_views:
+            v.div_(torch.amax(v))
+        self.x.clamp_(min=0.0)
+
+    def step(self):
+        """One OSMO iteration for every problem (OSMO.py:36-109)."""
+        void_diff = torch.clamp(self.x - self.d_l, min=0.0)
+        self.x_model.sub_(torch.where(self.void, void_diff, torch.zeros((), device=self.device)))
+        self._project()
+        gel_diff = torch.clamp(self.d_h - self.x, min=0.0)
+        self.x_model.add_(torch.where(self.gel, gel_diff, torch.zeros((), device=self.device)))
+        self._project()
+
+    def capture(self):
+        """Capture one iteration in a HIP graph (after one eager iteration)."""
+        s = torch.cuda.Stream(device=self.device)
+        s.wait_stream(torch.cuda.current_stream(self.device))
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                self.step()
+        torch.cuda.current_stream(self.device).wait_stream(s)
+        self.graph = g
+        return g
+
+    def ver(self):
+        return np.array([metrics.calcVER(t, x) for t, x in zip(self.target_geos, self.x_views)])
+
+    def results(self):
+        opts = self.options
+        return [(geometry.Sinogram(b.double().cpu().numpy(), p, opts),
+                 geometry.Reconstruction(x.double().cpu().numpy(), p, opts))
+                for b, x, p in zip(self.b_views, self.x_views, self.proj_geos)]
+
+
+def minimizeOSMOBatched(target_geos, proj_geos, options, hip_graph: bool = True, device=None):
+    """OSMO on k independent problems at once.
+
+    Returns a list of (Sinogram, Reconstruction, error) per problem, with the
+    same meaning as ``minimizeOSMO``'s return value; ``error`` is the VER per
+    iteration (iteration 0 = the initialisation).  With ``hip_graph`` the
+    first iteration runs eagerly and the rest replay a captured graph.
+    """
+    st = OSMOBatch(target_geos, proj_geos, options, device=device)
+    n_iter = options.n_iter
+    err = np.full((st.batch.k, n_iter), np.nan)
+    t0 = time.perf_counter()
+    err[:, 0] = st.ver()
+    iter_times = np.zeros(n_iter)
+    for it in range(1, n_iter):
+        if hip_graph and it >= 2:
+            if st.graph is None:
+                st.capture()
+            st.graph.replay()
+        else:
+            st.step()
+        err[:, it] = st.ver()
+        iter_times[it] = time.perf_counter() - t0
+    options.iter_times = iter_times
+    return [(s, r, e) for (s, r), e in zip(st.results(), err)]

@@ -81,13 +81,23 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchItem* __restrict_
 
 namespace {
 
-struct BatchScratch {
+// Device copies of recently used item tables.  A call whose table matches a
+// cached one enqueues only its two kernels (no copy, no synchronisation), so
+// a fixed sequence of batched calls -- e.g. forward then backward of an
+// iteration -- can be captured in a HIP graph once each table was uploaded
+// by an earlier call.
+struct BatchTable {
+  std::vector<char> bytes;
   BatchItem* d_items = nullptr;
-  size_t cap = 0;
-  unsigned long long* d_counter = nullptr;
-  int device = -1;
-  std::vector<char> last;
 };
+
+struct BatchScratch {
+  int device = -1;
+  unsigned long long* d_counter = nullptr;
+  std::vector<BatchTable> tables;   // most recently used last
+};
+
+constexpr size_t kMaxTables = 16;
 
 thread_local BatchScratch g_scratch;
 
@@ -132,25 +142,44 @@ int run_batched(vamx_plan* const* plans, int n, const float* const* ins, float* 
     tiles += (sorted[(size_t)r].items + 255) / 256;
   }
   BatchScratch& sc = g_scratch;
-  if (sc.device != dev || sc.cap < (size_t)n) {
-    if (sc.d_items) hipFree(sc.d_items);
+  if (sc.device != dev) {
+    for (auto& t : sc.tables) hipFree(t.d_items);
+    sc.tables.clear();
     if (sc.d_counter) hipFree(sc.d_counter);
-    sc.d_items = nullptr;
     sc.d_counter = nullptr;
-    sc.cap = std::max<size_t>((size_t)n, 64);
-    VAMX_HIP(hipMalloc((void**)&sc.d_items, sc.cap * sizeof(BatchItem)));
     VAMX_HIP(hipMalloc((void**)&sc.d_counter, sizeof(unsigned long long)));
     sc.device = dev;
-    sc.last.clear();
   }
-  // the item table is uploaded only when it differs from the previous call
-  // (a blocking copy); repeated calls with the same problems therefore enqueue
-  // only the two kernels below and can be captured in a HIP graph
   const size_t bytes = (size_t)n * sizeof(BatchItem);
-  if (sc.last.size() != bytes || std::memcmp(sc.last.data(), sorted.data(), bytes) != 0) {
+  const char* raw = reinterpret_cast<const char*>(sorted.data());
+  BatchItem* d_items = nullptr;
+  for (size_t t = 0; t < sc.tables.size(); ++t) {
+    if (sc.tables[t].bytes.size() == bytes && std::memcmp(sc.tables[t].bytes.data(), raw, bytes) == 0) {
+      BatchTable hit = std::move(sc.tables[t]);
+      sc.tables.erase(sc.tables.begin() + (long)t);
+      sc.tables.push_back(std::move(hit));
+      d_items = sc.tables.back().d_items;
+      break;
+    }
+  }
+  if (!d_items) {
+    // a new table: blocking upload (not allowed under graph capture, where
+    // hipStreamSynchronize fails and the error is returned)
     VAMX_HIP(hipStreamSynchronize(s));
-    VAMX_HIP(hipMemcpy(sc.d_items, sorted.data(), bytes, hipMemcpyHostToDevice));
-    sc.last.assign((const char*)sorted.data(), (const char*)sorted.data() + bytes);
+    if (sc.tables.size() >= kMaxTables) {
+      hipFree(sc.tables.front().d_items);
+      sc.tables.erase(sc.tables.begin());
+    }
+    BatchTable nt;
+    VAMX_HIP(hipMalloc((void**)&nt.d_items, bytes));
+    hipError_t e = hipMemcpy(nt.d_items, raw, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      hipFree(nt.d_items);
+      return hip_check(e, "hipMemcpy(batch table)");
+    }
+    nt.bytes.assign(raw, raw + bytes);
+    sc.tables.push_back(std::move(nt));
+    d_items = sc.tables.back().d_items;
   }
   hipLaunchKernelGGL(batch_reset, dim3(1), dim3(1), 0, s, sc.d_counter);
   int cus = 0;
@@ -158,10 +187,10 @@ int run_batched(vamx_plan* const* plans, int n, const float* const* ins, float* 
   long long grid = (long long)std::max(cus, 1) * 8;
   if (grid > tiles) grid = tiles;
   if (forward)
-    hipLaunchKernelGGL(batch_kernel<true>, dim3((unsigned)grid), dim3(256), 0, s, sc.d_items, n, tiles,
+    hipLaunchKernelGGL(batch_kernel<true>, dim3((unsigned)grid), dim3(256), 0, s, d_items, n, tiles,
                        sc.d_counter);
   else
-    hipLaunchKernelGGL(batch_kernel<false>, dim3((unsigned)grid), dim3(256), 0, s, sc.d_items, n, tiles,
+    hipLaunchKernelGGL(batch_kernel<false>, dim3((unsigned)grid), dim3(256), 0, s, d_items, n, tiles,
                        sc.d_counter);
   VAMX_HIP(hipGetLastError());
   return VAMX_OK;

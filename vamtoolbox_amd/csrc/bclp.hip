@@ -181,7 +181,7 @@ __global__ __launch_bounds__(256) void bclp_bp_epilogue_wave(ParDev g, int clipc
 }
 
 // C (wavefront-uniform variant)
-template <int ZV>
+template <int ZV, bool REUSE>
 __global__ __launch_bounds__(256) void bclp_fp_update_wave(ParDev g, const float* __restrict__ operand,
                                                            float* __restrict__ gsino,
                                                            const float* __restrict__ minmax, int clip,
@@ -192,8 +192,8 @@ __global__ __launch_bounds__(256) void bclp_fp_update_wave(ParDev g, const float
   int k, i, z;
   if (!fp_wave_item<ZV>(wave, g, lane, k, i, z)) return;
   const bool clip_on = clip && wave_clip_active<ZV>(minmax, g.nz, z);
-  const typename VecT<ZV>::T fp = clip_on ? fp_ray_wave<ZV, true>(g, operand, minmax, k, i, z, lane)
-                                          : fp_ray_wave<ZV, false>(g, operand, minmax, k, i, z, lane);
+  const typename VecT<ZV>::T fp = clip_on ? fp_ray_wave<ZV, true, REUSE>(g, operand, minmax, k, i, z, lane)
+                                          : fp_ray_wave<ZV, false, REUSE>(g, operand, minmax, k, i, z, lane);
   const float scale = (float)(*scale_p);
   float* gp = gsino + ((size_t)k * g.na + i) * g.nz + z;
   typename VecT<ZV>::T gv = vload<ZV>(gp);
@@ -406,15 +406,19 @@ extern "C" int vamx_bclp_step(vamx_plan* plan, const vamx_response* resp, const 
       const long long threads = fp_tiled_rays(gd) * (gd.nz / (64 * wz)) * 64;
       const dim3 grid(blocks_for(threads, 256)), block(256);
       const int cl = clip ? 1 : 0;
-      if (wz == 4)
-        hipLaunchKernelGGL(bclp_fp_update_wave<4>, grid, block, 0, s, gd, operand, g, plan->d_minmax, cl, scale, lr,
-                           glb, gub);
-      else if (wz == 2)
-        hipLaunchKernelGGL(bclp_fp_update_wave<2>, grid, block, 0, s, gd, operand, g, plan->d_minmax, cl, scale, lr,
-                           glb, gub);
-      else
-        hipLaunchKernelGGL(bclp_fp_update_wave<1>, grid, block, 0, s, gd, operand, g, plan->d_minmax, cl, scale, lr,
-                           glb, gub);
+#define VAMX_FPU(Z, R)                                                                                 \
+  hipLaunchKernelGGL((bclp_fp_update_wave<Z, R>), grid, block, 0, s, gd, operand, g, plan->d_minmax, cl, scale, \
+                     lr, glb, gub)
+      if (gd.fp_reuse) {
+        if (wz == 4) VAMX_FPU(4, true);
+        else if (wz == 2) VAMX_FPU(2, true);
+        else VAMX_FPU(1, true);
+      } else {
+        if (wz == 4) VAMX_FPU(4, false);
+        else if (wz == 2) VAMX_FPU(2, false);
+        else VAMX_FPU(1, false);
+      }
+#undef VAMX_FPU
       VAMX_HIP(hipGetLastError());
       return VAMX_OK;
     }

@@ -493,8 +493,12 @@ __device__ __forceinline__ void axis_slots(double coord, int n, int& b, double& 
   }
 }
 
-// forward projection of ray (k, i), lanes = z; requires n >= 2
-template <int ZV, bool CLIP>
+// forward projection of ray (k, i), lanes = z; requires n >= 2.  REUSE keeps
+// the previous sample's four taps in registers: between consecutive samples
+// the floor corner b moves by at most one pixel per axis, and when it moves
+// along one axis only, two of the four taps are the previous sample's.  b is
+// wave-uniform, so the case test is a scalar branch.
+template <int ZV, bool CLIP, bool REUSE>
 __device__ __forceinline__ typename VecT<ZV>::T fp_ray_wave(const ParDev& g, const float* __restrict__ vol,
                                                              const float* __restrict__ minmax, int k, int i, int z,
                                                              int lane) {
@@ -513,6 +517,8 @@ __device__ __forceinline__ typename VecT<ZV>::T fp_ray_wave(const ParDev& g, con
   const size_t nzs = (size_t)g.nz;
   const size_t step_c = nzs, step_r = (size_t)n * nzs;
   V acc = vzero<ZV>();
+  V t00 = vzero<ZV>(), t01 = vzero<ZV>(), t10 = vzero<ZV>(), t11 = vzero<ZV>();
+  int prev = -2 * n - 2;   // no taps held
   for (int r0 = 0; r0 < n; r0 += 64) {
     const int r = r0 + lane;
     int base = 0;
@@ -545,8 +551,41 @@ __device__ __forceinline__ typename VecT<ZV>::T fp_ray_wave(const ParDev& g, con
       const int b = rl_i(base, j);
       const float q00 = rl_f(w00, j), q01 = rl_f(w01, j), q10 = rl_f(w10, j), q11 = rl_f(w11, j);
       const float* p = vol + (size_t)b * nzs + z;
-      const V t00 = vload<ZV>(p), t01 = vload<ZV>(p + step_c);
-      const V t10 = vload<ZV>(p + step_r), t11 = vload<ZV>(p + step_r + step_c);
+      if constexpr (REUSE) {
+        const int d = b - prev;
+        prev = b;
+        if (d == 1) {
+          t00 = t01;
+          t10 = t11;
+          t01 = vload<ZV>(p + step_c);
+          t11 = vload<ZV>(p + step_r + step_c);
+        } else if (d == n) {
+          t00 = t10;
+          t01 = t11;
+          t10 = vload<ZV>(p + step_r);
+          t11 = vload<ZV>(p + step_r + step_c);
+        } else if (d == -1) {
+          t01 = t00;
+          t11 = t10;
+          t00 = vload<ZV>(p);
+          t10 = vload<ZV>(p + step_r);
+        } else if (d == -n) {
+          t10 = t00;
+          t11 = t01;
+          t00 = vload<ZV>(p);
+          t01 = vload<ZV>(p + step_c);
+        } else if (d != 0) {
+          t00 = vload<ZV>(p);
+          t01 = vload<ZV>(p + step_c);
+          t10 = vload<ZV>(p + step_r);
+          t11 = vload<ZV>(p + step_r + step_c);
+        }
+      } else {
+        t00 = vload<ZV>(p);
+        t01 = vload<ZV>(p + step_c);
+        t10 = vload<ZV>(p + step_r);
+        t11 = vload<ZV>(p + step_r + step_c);
+      }
       if constexpr (CLIP) {
         V v = vfma(q11, t11, vfma(q10, t10, vfma(q01, t01, vmul(q00, t00))));
         v = vclamp(v, lo, hi);  // every enumerated sample has an in-range tap of positive weight

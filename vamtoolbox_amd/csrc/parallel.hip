@@ -96,7 +96,7 @@ __global__ __launch_bounds__(256) void par_fp_kernel(ParDev g, const float* __re
 }
 
 // wavefront-uniform forward projection: one wavefront = one ray x 64*ZV z
-template <int ZV>
+template <int ZV, bool REUSE>
 __global__ __launch_bounds__(256) void par_fp_wave_kernel(ParDev g, const float* __restrict__ vol,
                                                           float* __restrict__ sino, const float* __restrict__ minmax,
                                                           int clip) {
@@ -105,8 +105,8 @@ __global__ __launch_bounds__(256) void par_fp_wave_kernel(ParDev g, const float*
   int k, i, z;
   if (!fp_wave_item<ZV>(wave, g, lane, k, i, z)) return;
   const bool clip_on = clip && wave_clip_active<ZV>(minmax, g.nz, z);
-  typename VecT<ZV>::T acc = clip_on ? fp_ray_wave<ZV, true>(g, vol, minmax, k, i, z, lane)
-                                     : fp_ray_wave<ZV, false>(g, vol, minmax, k, i, z, lane);
+  typename VecT<ZV>::T acc = clip_on ? fp_ray_wave<ZV, true, REUSE>(g, vol, minmax, k, i, z, lane)
+                                     : fp_ray_wave<ZV, false, REUSE>(g, vol, minmax, k, i, z, lane);
   vstore<ZV>(sino + ((size_t)k * g.na + i) * g.nz + z, acc);
 }
 
@@ -279,9 +279,10 @@ extern "C" int vamx_par_create(const vamx_par_geom* geom, vamx_plan** out_plan) 
   g.mu = geom->mu;
   g.radius = geom->radius > 0.0 ? geom->radius : (n - 1) / 2.0;
   // defaults: ParallelPlan.autotune's choice at 256^3 x 360 angles on MI355X
-  g.fp_ta = 32;
-  g.fp_tk = 32;
+  g.fp_ta = 64;
+  g.fp_tk = 16;
   g.fp_zv = 4;
+  g.fp_reuse = 1;
   g.bp_tile = 16;
   g.bp_zv = 4;
   p->minmax_parts = 256;
@@ -333,12 +334,17 @@ int par_forward_launch(vamx_plan* p, const float* vol, float* sino, hipStream_t 
     const long long threads = fp_tiled_rays(g) * (g.nz / (64 * wz)) * 64;
     const dim3 grid(blocks_for(threads, 256)), block(256);
     const int clip = warp_clip ? 1 : 0;
-    if (wz == 4)
-      hipLaunchKernelGGL(par_fp_wave_kernel<4>, grid, block, 0, s, g, vol, sino, p->d_minmax, clip);
-    else if (wz == 2)
-      hipLaunchKernelGGL(par_fp_wave_kernel<2>, grid, block, 0, s, g, vol, sino, p->d_minmax, clip);
-    else
-      hipLaunchKernelGGL(par_fp_wave_kernel<1>, grid, block, 0, s, g, vol, sino, p->d_minmax, clip);
+#define VAMX_FPW(Z, R) hipLaunchKernelGGL((par_fp_wave_kernel<Z, R>), grid, block, 0, s, g, vol, sino, p->d_minmax, clip)
+    if (g.fp_reuse) {
+      if (wz == 4) VAMX_FPW(4, true);
+      else if (wz == 2) VAMX_FPW(2, true);
+      else VAMX_FPW(1, true);
+    } else {
+      if (wz == 4) VAMX_FPW(4, false);
+      else if (wz == 2) VAMX_FPW(2, false);
+      else VAMX_FPW(1, false);
+    }
+#undef VAMX_FPW
     VAMX_HIP(hipGetLastError());
     return VAMX_OK;
   }
@@ -462,11 +468,13 @@ extern "C" int vamx_par_set_tuning(vamx_plan* plan, const vamx_par_tuning* t) {
   if (!t) return set_error(VAMX_ERR_ARG, "vamx_par_set_tuning: null tuning");
   auto pow2 = [](int v) { return v == 1 || v == 2 || v == 4; };
   if (t->fp_tile_angles < 1 || t->fp_tile_rays < 1 || t->bp_tile < 1 || !pow2(t->fp_zv) || !pow2(t->bp_zv) ||
-      t->fp_tile_angles * t->fp_tile_rays > (1 << 20) || t->bp_tile > 1024)
-    return set_error(VAMX_ERR_ARG, "vamx_par_set_tuning: tiles must be >= 1 and z widths 1, 2 or 4");
+      t->fp_tile_angles * t->fp_tile_rays > (1 << 20) || t->bp_tile > 1024 || t->fp_reuse < 0 || t->fp_reuse > 1)
+    return set_error(VAMX_ERR_ARG,
+                     "vamx_par_set_tuning: tiles must be >= 1, z widths 1, 2 or 4 and fp_reuse 0 or 1");
   plan->par.fp_ta = t->fp_tile_angles;
   plan->par.fp_tk = t->fp_tile_rays;
   plan->par.fp_zv = t->fp_zv;
+  plan->par.fp_reuse = t->fp_reuse;
   plan->par.bp_tile = t->bp_tile;
   plan->par.bp_zv = t->bp_zv;
   return VAMX_OK;
@@ -478,6 +486,7 @@ extern "C" int vamx_par_get_tuning(vamx_plan* plan, vamx_par_tuning* t) {
   t->fp_tile_angles = plan->par.fp_ta;
   t->fp_tile_rays = plan->par.fp_tk;
   t->fp_zv = plan->par.fp_zv;
+  t->fp_reuse = plan->par.fp_reuse;
   t->bp_tile = plan->par.bp_tile;
   t->bp_zv = plan->par.bp_zv;
   return VAMX_OK;

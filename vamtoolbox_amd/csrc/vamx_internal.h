@@ -27,6 +27,7 @@ struct ParDev {
   const uint8_t* outside;    // [n*n] clipToCircle mask
   // work order of the wavefront-uniform kernels (set by vamx_par_set_tuning)
   int fp_ta, fp_tk, fp_zv;   // forward: tile of fp_ta angles x fp_tk rays, z values per lane
+  int fp_reuse;              // forward: keep the previous sample's taps in registers
   int bp_tile, bp_zv;        // back projection: bp_tile x bp_tile voxel columns, z values per lane
 };
 

@@ -122,9 +122,10 @@ class ParallelPlan:
         t = _lib.ParTuning(**cur)
         _lib.check(self.lib.vamx_par_set_tuning(self.handle, ctypes.byref(t)), "vamx_par_set_tuning")
 
-    # (angles per tile, rays per tile, z values per lane)
-    FP_CANDIDATES = [(32, 32, 4), (16, 64, 4), (8, 128, 4), (64, 16, 4), (16, 32, 4), (32, 32, 2), (16, 64, 2),
-                     (64, 16, 2), (32, 32, 1), (16, 64, 1)]
+    # (angles per tile, rays per tile, z values per lane, register tap reuse)
+    FP_CANDIDATES = [(64, 16, 4, 1), (128, 8, 4, 1), (64, 8, 4, 1), (128, 16, 4, 1), (64, 32, 4, 1), (32, 32, 4, 1),
+                     (16, 64, 4, 1), (64, 16, 2, 1), (32, 32, 2, 1), (64, 16, 4, 0), (32, 32, 4, 0), (16, 32, 4, 0),
+                     (32, 32, 1, 0)]
     BP_CANDIDATES = [(32, 4), (16, 4), (64, 4), (32, 2), (16, 2), (24, 4)]
 
     def autotune(self, vol: torch.Tensor | None = None, sino: torch.Tensor | None = None, reps: int = 3) -> dict:
@@ -155,16 +156,16 @@ class ParallelPlan:
             return a.elapsed_time(b) / reps
 
         report = {"forward": {}, "backward": {}}
-        for ta, tk, zv in self.FP_CANDIDATES:
-            self.set_tuning(fp_tile_angles=ta, fp_tile_rays=tk, fp_zv=zv)
-            report["forward"][(ta, tk, zv)] = timed(lambda: self.forward(vol, out=out_s))
+        for ta, tk, zv, ru in self.FP_CANDIDATES:
+            self.set_tuning(fp_tile_angles=ta, fp_tile_rays=tk, fp_zv=zv, fp_reuse=ru)
+            report["forward"][(ta, tk, zv, ru)] = timed(lambda: self.forward(vol, out=out_s))
         for tile, zv in self.BP_CANDIDATES:
             self.set_tuning(bp_tile=tile, bp_zv=zv)
             report["backward"][(tile, zv)] = timed(lambda: self.backward(sino, out=out_v))
         fbest = min(report["forward"], key=report["forward"].get)
         bbest = min(report["backward"], key=report["backward"].get)
-        choice = dict(fp_tile_angles=fbest[0], fp_tile_rays=fbest[1], fp_zv=fbest[2], bp_tile=bbest[0],
-                      bp_zv=bbest[1])
+        choice = dict(fp_tile_angles=fbest[0], fp_tile_rays=fbest[1], fp_zv=fbest[2], fp_reuse=fbest[3],
+                      bp_tile=bbest[0], bp_zv=bbest[1])
         self.set_tuning(**choice)
         report["choice"] = choice
         _TUNING_CACHE[key] = report
