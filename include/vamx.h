@@ -87,7 +87,7 @@ int vamx_par_backward_adjoint(vamx_plan* plan, const float* vol, float* sino, vo
  * identical for every setting up to float32 rounding of the z-vector width. */
 typedef struct vamx_par_tuning {
   int32_t fp_tile_angles;   /* forward: angles per tile of consecutive waves (default 64) */
-  int32_t fp_tile_rays;     /* forward: rays per tile (default 16)                       */
+  int32_t fp_tile_rays;     /* forward: rays per tile (default 8)                        */
   int32_t fp_zv;            /* forward: z values per lane, 1, 2 or 4 (default 4)         */
   int32_t fp_reuse;         /* forward: 1 = keep the previous sample's taps in registers (default 1) */
   int32_t bp_tile;          /* back projection: tile edge in voxel columns (default 16)  */

@@ -158,7 +158,7 @@ __global__ __launch_bounds__(256) void bclp_bp_epilogue_wave(ParDev g, int clipc
   __shared__ double sh[256];
   const int lane = threadIdx.x & 63;
   const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
-  const long long wave = ((long long)lb * blockDim.x + threadIdx.x) >> 6;
+  const long long wave = wave_index(lb);
   double part = 0.0;
   int pix = 0, z = 0;
   if (bp_wave_item<ZV>(wave, g, lane, pix, z)) {
@@ -188,7 +188,7 @@ __global__ __launch_bounds__(256) void bclp_fp_update_wave(ParDev g, const float
                                                            const double* __restrict__ scale_p, float lr, float glb,
                                                            float gub) {
   const int lane = threadIdx.x & 63;
-  const long long wave = ((long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
+  const long long wave = wave_index(xcd_remap(blockIdx.x, gridDim.x));
   int k, i, z;
   if (!fp_wave_item<ZV>(wave, g, lane, k, i, z)) return;
   const bool clip_on = clip && wave_clip_active<ZV>(minmax, g.nz, z);

@@ -681,6 +681,15 @@ __device__ __forceinline__ bool bp_tile_pix(long long q, const ParDev& g, int& p
   return row < g.n && col < g.n;
 }
 
+// Global index of the calling wavefront for logical block lb.  The wave's
+// slot in its block is made wave-uniform explicitly (readfirstlane), so that
+// everything derived from it -- ray, angle, voxel column, geometry constants
+// -- lives in scalar registers.
+__device__ __forceinline__ long long wave_index(unsigned lb) {
+  const int slot = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  return (long long)lb * (blockDim.x >> 6) + slot;
+}
+
 // wave -> (ray, z) of the forward kernels; false for padding waves
 template <int ZV>
 __device__ __forceinline__ bool fp_wave_item(long long wave, const ParDev& g, int lane, int& k, int& i, int& z) {

@@ -101,7 +101,7 @@ __global__ __launch_bounds__(256) void par_fp_wave_kernel(ParDev g, const float*
                                                           float* __restrict__ sino, const float* __restrict__ minmax,
                                                           int clip) {
   const int lane = threadIdx.x & 63;
-  const long long wave = ((long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
+  const long long wave = wave_index(xcd_remap(blockIdx.x, gridDim.x));
   int k, i, z;
   if (!fp_wave_item<ZV>(wave, g, lane, k, i, z)) return;
   const bool clip_on = clip && wave_clip_active<ZV>(minmax, g.nz, z);
@@ -115,7 +115,7 @@ template <int ZV>
 __global__ __launch_bounds__(256) void par_bp_wave_kernel(ParDev g, const float* __restrict__ sino,
                                                           float* __restrict__ vol, int clipc) {
   const int lane = threadIdx.x & 63;
-  const long long wave = ((long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
+  const long long wave = wave_index(xcd_remap(blockIdx.x, gridDim.x));
   int pix, z;
   if (!bp_wave_item<ZV>(wave, g, lane, pix, z)) return;
   float* outp = vol + (size_t)pix * g.nz + z;
@@ -280,7 +280,7 @@ extern "C" int vamx_par_create(const vamx_par_geom* geom, vamx_plan** out_plan) 
   g.radius = geom->radius > 0.0 ? geom->radius : (n - 1) / 2.0;
   // defaults: ParallelPlan.autotune's choice at 256^3 x 360 angles on MI355X
   g.fp_ta = 64;
-  g.fp_tk = 16;
+  g.fp_tk = 8;
   g.fp_zv = 4;
   g.fp_reuse = 1;
   g.bp_tile = 16;
