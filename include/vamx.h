@@ -90,7 +90,7 @@ typedef struct vamx_par_tuning {
   int32_t fp_tile_rays;     /* forward: rays per tile (default 8)                        */
   int32_t fp_zv;            /* forward: z values per lane, 1, 2 or 4 (default 4)         */
   int32_t fp_reuse;         /* forward: 1 = keep the previous sample's taps in registers (default 1) */
-  int32_t bp_tile;          /* back projection: tile edge in voxel columns (default 16)  */
+  int32_t bp_tile;          /* back projection: tile edge in 2x2 quads of voxel columns (default 8) */
   int32_t bp_zv;            /* back projection: z values per lane (default 4)            */
 } vamx_par_tuning;
 int vamx_par_set_tuning(vamx_plan* plan, const vamx_par_tuning* tuning);

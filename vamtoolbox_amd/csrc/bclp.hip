@@ -148,7 +148,8 @@ __global__ __launch_bounds__(256) void bclp_bp_epilogue(ParDev g, int clipc, Res
   if (threadIdx.x == 0) partials[lb] = s;
 }
 
-// A (wavefront-uniform variant): one wavefront = one voxel column x 64*ZV z
+// A (wavefront-uniform variant): one wavefront = one 2x2 quad of voxel
+// columns x 64*ZV z
 template <int ZV>
 __global__ __launch_bounds__(256) void bclp_bp_epilogue_wave(ParDev g, int clipc, RespDev r, BclpDev b,
                                                              const float* __restrict__ gsino,
@@ -160,21 +161,28 @@ __global__ __launch_bounds__(256) void bclp_bp_epilogue_wave(ParDev g, int clipc
   const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
   const long long wave = wave_index(lb);
   double part = 0.0;
-  int pix = 0, z = 0;
-  if (bp_wave_item<ZV>(wave, g, lane, pix, z)) {
-    const size_t base = (size_t)pix * g.nz + z;
-    typename VecT<ZV>::T d = vzero<ZV>();
-    if (!(clipc && g.outside[pix])) d = bp_voxel_wave<ZV>(g, gsino, pix, z, lane);
-    vstore<ZV>(dose + base, d);
-    typename VecT<ZV>::T op = vzero<ZV>();
-    const typename VecT<ZV>::T tg = vload<ZV>(target + base);
+  int r0 = 0, c0 = 0, z = 0;
+  if (bp_wave_quad<ZV>(wave, g, lane, r0, c0, z)) {
+    unsigned inrange, active;
+    bp_quad_masks(g, r0, c0, clipc, inrange, active);
+    typename VecT<ZV>::T acc[4];
+    bp_quad_wave<ZV>(g, gsino, r0, c0, active, z, lane, acc);
 #pragma unroll
-    for (int q = 0; q < ZV; ++q) {
-      float o;
-      part += bclp_voxel(r, b, vget(d, q), vget(tg, q), base + q, o);
-      vset(op, q, o);
+    for (int q = 0; q < 4; ++q) {
+      if (!((inrange >> q) & 1u)) continue;
+      const size_t base = ((size_t)(r0 + (q >> 1)) * g.n + c0 + (q & 1)) * g.nz + z;
+      const typename VecT<ZV>::T d = ((active >> q) & 1u) ? acc[q] : vzero<ZV>();
+      vstore<ZV>(dose + base, d);
+      typename VecT<ZV>::T op = vzero<ZV>();
+      const typename VecT<ZV>::T tg = vload<ZV>(target + base);
+#pragma unroll
+      for (int e = 0; e < ZV; ++e) {
+        float o;
+        part += bclp_voxel(r, b, vget(d, e), vget(tg, e), base + e, o);
+        vset(op, e, o);
+      }
+      vstore<ZV>(operand + base, op);
     }
-    vstore<ZV>(operand + base, op);
   }
   const double s = block_sum_256(part, sh);
   if (threadIdx.x == 0) partials[lb] = s;
@@ -287,7 +295,8 @@ WsLayout ws_layout(const vamx_plan* p) {
   const ParDev& g = p->par;
   WsLayout w;
   const long long items = (long long)g.n * g.n * g.nz;  // worst case ZV = 1
-  const long long tiled = bp_tiled_pix(g) * g.nz;        // wavefront path with padding
+  // wavefront path: one wave per quad and 64 z (ZV = 1), any tile setting
+  const long long tiled = bp_tiled_quads_max(g) * g.nz;
   w.nparts = (int)blocks_for(items > tiled ? items : tiled, 256);
   w.partials_off = 0;
   w.scale_off = ((size_t)w.nparts * sizeof(double) + 255) / 256 * 256;
@@ -329,7 +338,7 @@ int launch_epilogue(vamx_plan* p, const RespDev& r, const BclpDev& b, const floa
     if (gd.nz % (64 * wz) != 0) wz = 0;
   }
   if (wz) {
-    const long long threads = bp_tiled_pix(gd) * (gd.nz / (64 * wz)) * 64;
+    const long long threads = bp_tiled_quads(gd) * (gd.nz / (64 * wz)) * 64;
     nparts = (int)blocks_for(threads, 256);
     const dim3 grid(nparts), block(256);
     if (wz == 4)

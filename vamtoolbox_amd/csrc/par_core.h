@@ -598,51 +598,91 @@ __device__ __forceinline__ typename VecT<ZV>::T fp_ray_wave(const ParDev& g, con
   return acc;
 }
 
-// back projection of voxel column pix, lanes = z (no circle clip here); n >= 2
+// Back projection of a 2x2 quad of voxel columns, lanes = z.  Pixel p
+// (0..3) is at row r0 + (p >> 1), column c0 + (p & 1); bit p of `active`
+// says whether it is computed (in range and not circle-clipped).  Within one
+// angle the four pixels' detector coordinates t differ by at most
+// |cos| + |sin| <= sqrt(2), so their lower np.interp nodes span at most two
+// detector rows and the quad needs at most four distinct rows (usually
+// three) where four separate columns would load eight.  Lanes first evaluate
+// the float64 geometry of 16 angles x 4 pixels; the per-angle loop then
+// loads the rows [bmin, bmax + 1] once and every pixel accumulates
+// fma(w1, b[j+1], fma(w0, b[j], acc)) in angle order -- the same operations,
+// in the same order, as one column at a time.  Requires n >= 2.
 template <int ZV>
-__device__ __forceinline__ typename VecT<ZV>::T bp_voxel_wave(const ParDev& g, const float* __restrict__ sino,
-                                                               int pix, int z, int lane) {
+__device__ __forceinline__ void bp_quad_wave(const ParDev& g, const float* __restrict__ sino, int r0, int c0,
+                                             unsigned active, int z, int lane, typename VecT<ZV>::T acc[4]) {
   using V = typename VecT<ZV>::T;
-  const int row = pix / g.n, col = pix % g.n;
-  const double x = (double)(col - g.c), y = (double)(row - g.c);
+  const int p = lane & 3, a = lane >> 2;
+  const bool pin = (active >> p) & 1u;
+  const double x = (double)(c0 + (p & 1) - g.c), y = (double)(r0 + (p >> 1) - g.c);
   const size_t nzs = (size_t)g.nz;
   const size_t step_j = (size_t)g.na * nzs;
-  V acc = vzero<ZV>();
-  for (int i0 = 0; i0 < g.na; i0 += 64) {
-    const int i = i0 + lane;
-    int base = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) acc[q] = vzero<ZV>();
+  for (int i0 = 0; i0 < g.na; i0 += 16) {
+    const int i = i0 + a;
+    int jr = 0;
     float w0 = 0.f, w1 = 0.f;
     bool valid = false;
-    if (i < g.na) {
+    if (pin && i < g.na) {
       const double co = g.cs[2 * i], sn = g.cs[2 * i + 1];
       const double t = x * co - y * sn;
       int j, j1;
       float w;
       if (interp_node(g, t, j, j1, w)) {
-        const float a = (g.mu != 0.0) ? abs_weight(g, t, x * sn + y * co) : 1.0f;
-        valid = a != 0.0f;
+        const float aw = (g.mu != 0.0) ? abs_weight(g, t, x * sn + y * co) : 1.0f;
+        valid = aw != 0.0f;
         if (j == g.n - 1) {        // exactly at the last node: fp[n-1]
-          base = (j - 1) * g.na + i;
-          w0 = 0.0f;
-          w1 = a;
+          jr = j - 1;
+          w1 = aw;
         } else {
-          base = j * g.na + i;
-          w0 = a * (1.0f - w);
-          w1 = a * w;
+          jr = j;
+          w0 = aw * (1.0f - w);
+          w1 = aw * w;
         }
       }
     }
-    unsigned long long m = __ballot(valid);
-    while (m) {
-      const int jj = __builtin_ctzll(m);
-      m &= m - 1;
-      const int b = rl_i(base, jj);
-      const float q0 = rl_f(w0, jj), q1 = rl_f(w1, jj);
-      const float* p = sino + (size_t)b * nzs + z;
-      acc = vfma(q1, vload<ZV>(p + step_j), vfma(q0, vload<ZV>(p), acc));
+    const unsigned long long m = __ballot(valid);
+    unsigned long long am = (m | (m >> 1) | (m >> 2) | (m >> 3)) & 0x1111111111111111ull;
+    while (am) {
+      const int l0 = __builtin_ctzll(am);   // lane of pixel 0 of this angle
+      am &= am - 1;
+      const unsigned pm = (unsigned)(m >> l0) & 0xFu;
+      const int ia = i0 + (l0 >> 2);
+      int jq[4];
+      float u0[4], u1[4];
+      int bmin = g.n, bmax = -1;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        jq[q] = rl_i(jr, l0 + q);
+        u0[q] = rl_f(w0, l0 + q);
+        u1[q] = rl_f(w1, l0 + q);
+        if ((pm >> q) & 1u) {
+          bmin = jq[q] < bmin ? jq[q] : bmin;
+          bmax = jq[q] > bmax ? jq[q] : bmax;
+        }
+      }
+      // rows bmin .. bmax + 1 <= n - 1 (jr <= n - 2); span <= 2 by the bound above
+      const float* pb = sino + ((size_t)bmin * g.na + ia) * nzs + z;
+      const int span = bmax - bmin;
+      const V T0 = vload<ZV>(pb), T1 = vload<ZV>(pb + step_j);
+      V T2 = vzero<ZV>(), T3 = vzero<ZV>();
+      if (span >= 1) T2 = vload<ZV>(pb + 2 * step_j);
+      if (span >= 2) T3 = vload<ZV>(pb + 3 * step_j);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (!((pm >> q) & 1u)) continue;
+        const int off = jq[q] - bmin;
+        if (off == 0)
+          acc[q] = vfma(u1[q], T1, vfma(u0[q], T0, acc[q]));
+        else if (off == 1)
+          acc[q] = vfma(u1[q], T2, vfma(u0[q], T1, acc[q]));
+        else
+          acc[q] = vfma(u1[q], T3, vfma(u0[q], T2, acc[q]));
+      }
     }
   }
-  return acc;
 }
 
 // L2-aware work order of the wavefront-uniform kernels.  About 1024 waves are
@@ -650,35 +690,54 @@ __device__ __forceinline__ typename VecT<ZV>::T bp_voxel_wave(const ParDev& g, c
 // run on the same XCD (xcd_remap), so consecutive waves should form a compact
 // tile whose data fits the XCD's 4 MB L2: forward = fp_ta consecutive angles x
 // fp_tk consecutive rays (their samples at one along-ray step lie within a few
-// tens of pixels), back projection = a bp_tile x bp_tile patch of voxel
-// columns (detector footprint ~1.4 bp_tile bins per angle).  When a ray (or
+// tens of pixels), back projection = a bp_tile x bp_tile patch of 2x2 voxel
+// quads (detector footprint ~2.8 bp_tile bins per angle).  When a ray (or
 // column) spans several z chunks, the chunk is the slowest index, so that all
-// resident waves share one z slab.  Padding waves exit.
+// resident waves share one z slab.  Padding waves exit.  Tiles never exceed
+// the problem: a tile edge beyond the angle / ray / quad count would only add
+// padding waves.
+__host__ __device__ inline int fp_ta_eff(const ParDev& g) { return g.fp_ta < g.na ? g.fp_ta : g.na; }
+__host__ __device__ inline int fp_tk_eff(const ParDev& g) { return g.fp_tk < g.n ? g.fp_tk : g.n; }
 __host__ __device__ inline long long fp_tiled_rays(const ParDev& g) {
-  return (long long)((g.na + g.fp_ta - 1) / g.fp_ta) * ((g.n + g.fp_tk - 1) / g.fp_tk) * g.fp_ta * g.fp_tk;
+  const int ta = fp_ta_eff(g), tk = fp_tk_eff(g);
+  return (long long)((g.na + ta - 1) / ta) * ((g.n + tk - 1) / tk) * ta * tk;
 }
 __device__ __forceinline__ bool fp_tile_ray(long long q, const ParDev& g, int& k, int& i) {
-  const int ntk = (g.n + g.fp_tk - 1) / g.fp_tk;
-  const int tsz = g.fp_ta * g.fp_tk;
+  const int ta = fp_ta_eff(g), tk = fp_tk_eff(g);
+  const int ntk = (g.n + tk - 1) / tk;
+  const int tsz = ta * tk;
   const long long tile = q / tsz;
   const int within = (int)(q % tsz);
-  i = (int)(tile / ntk) * g.fp_ta + within / g.fp_tk;
-  k = (int)(tile % ntk) * g.fp_tk + within % g.fp_tk;
+  i = (int)(tile / ntk) * ta + within / tk;
+  k = (int)(tile % ntk) * tk + within % tk;
   return i < g.na && k < g.n;
 }
-__host__ __device__ inline long long bp_tiled_pix(const ParDev& g) {
-  const long long t = (g.n + g.bp_tile - 1) / g.bp_tile;
-  return t * t * g.bp_tile * g.bp_tile;
+// back projection: quads of 2x2 voxel columns, tiles of bp_tile x bp_tile quads
+__host__ __device__ inline int bp_nquad(const ParDev& g) { return (g.n + 1) / 2; }
+__host__ __device__ inline int bp_tile_eff(const ParDev& g) {
+  return g.bp_tile < bp_nquad(g) ? g.bp_tile : bp_nquad(g);
 }
-__device__ __forceinline__ bool bp_tile_pix(long long q, const ParDev& g, int& pix) {
-  const int nt = (g.n + g.bp_tile - 1) / g.bp_tile;
-  const int tsz = g.bp_tile * g.bp_tile;
+__host__ __device__ inline long long bp_tiled_quads(const ParDev& g) {
+  const int bt = bp_tile_eff(g);
+  const long long t = (bp_nquad(g) + bt - 1) / bt;
+  return t * t * bt * bt;
+}
+// upper bound of bp_tiled_quads over every tile setting (tile <= nquad)
+__host__ __device__ inline long long bp_tiled_quads_max(const ParDev& g) {
+  const long long e = 2ll * bp_nquad(g);
+  return e * e;
+}
+__device__ __forceinline__ bool bp_tile_quad(long long q, const ParDev& g, int& r0, int& c0) {
+  const int bt = bp_tile_eff(g);
+  const int nt = (bp_nquad(g) + bt - 1) / bt;
+  const int tsz = bt * bt;
   const long long tile = q / tsz;
   const int within = (int)(q % tsz);
-  const int row = (int)(tile / nt) * g.bp_tile + within / g.bp_tile;
-  const int col = (int)(tile % nt) * g.bp_tile + within % g.bp_tile;
-  pix = row * g.n + col;
-  return row < g.n && col < g.n;
+  const int qr = (int)(tile / nt) * bt + within / bt;
+  const int qc = (int)(tile % nt) * bt + within % bt;
+  r0 = 2 * qr;
+  c0 = 2 * qc;
+  return qr < bp_nquad(g) && qc < bp_nquad(g);
 }
 
 // Global index of the calling wavefront for logical block lb.  The wave's
@@ -702,16 +761,32 @@ __device__ __forceinline__ bool fp_wave_item(long long wave, const ParDev& g, in
   return true;
 }
 
-// wave -> (voxel column, z) of the back-projection kernels
+// wave -> (voxel quad, z) of the back-projection kernels; false for padding
 template <int ZV>
-__device__ __forceinline__ bool bp_wave_item(long long wave, const ParDev& g, int lane, int& pix, int& z) {
-  const long long cols = bp_tiled_pix(g);
+__device__ __forceinline__ bool bp_wave_quad(long long wave, const ParDev& g, int lane, int& r0, int& c0, int& z) {
+  const long long quads = bp_tiled_quads(g);
   const int nch = g.nz / (64 * ZV);
-  if (wave >= cols * nch) return false;
-  const int zc = (int)(wave / cols);
-  if (!bp_tile_pix(wave % cols, g, pix)) return false;
+  if (wave >= quads * nch) return false;
+  const int zc = (int)(wave / quads);
+  if (!bp_tile_quad(wave % quads, g, r0, c0)) return false;
   z = zc * 64 * ZV + lane * ZV;
   return true;
+}
+
+// pixels of quad (r0, c0) inside the grid (bit p = pixel p) and, of those,
+// the ones computed (not zeroed by the circle clip)
+__device__ __forceinline__ void bp_quad_masks(const ParDev& g, int r0, int c0, int clipc, unsigned& inrange,
+                                              unsigned& active) {
+  inrange = 0u;
+  active = 0u;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int row = r0 + (q >> 1), col = c0 + (q & 1);
+    if (row < g.n && col < g.n) {
+      inrange |= 1u << q;
+      if (!(clipc && g.outside[row * g.n + col])) active |= 1u << q;
+    }
+  }
 }
 
 // host helpers implemented in parallel.hip

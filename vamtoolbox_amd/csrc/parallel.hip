@@ -110,20 +110,25 @@ __global__ __launch_bounds__(256) void par_fp_wave_kernel(ParDev g, const float*
   vstore<ZV>(sino + ((size_t)k * g.na + i) * g.nz + z, acc);
 }
 
-// wavefront-uniform back projection: one wavefront = one voxel column x 64*ZV z
+// wavefront-uniform back projection: one wavefront = one 2x2 quad of voxel
+// columns x 64*ZV z
 template <int ZV>
 __global__ __launch_bounds__(256) void par_bp_wave_kernel(ParDev g, const float* __restrict__ sino,
                                                           float* __restrict__ vol, int clipc) {
   const int lane = threadIdx.x & 63;
   const long long wave = wave_index(xcd_remap(blockIdx.x, gridDim.x));
-  int pix, z;
-  if (!bp_wave_item<ZV>(wave, g, lane, pix, z)) return;
-  float* outp = vol + (size_t)pix * g.nz + z;
-  if (clipc && g.outside[pix]) {
-    vstore<ZV>(outp, vzero<ZV>());
-    return;
+  int r0, c0, z;
+  if (!bp_wave_quad<ZV>(wave, g, lane, r0, c0, z)) return;
+  unsigned inrange, active;
+  bp_quad_masks(g, r0, c0, clipc, inrange, active);
+  typename VecT<ZV>::T acc[4];
+  bp_quad_wave<ZV>(g, sino, r0, c0, active, z, lane, acc);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (!((inrange >> q) & 1u)) continue;
+    float* outp = vol + ((size_t)(r0 + (q >> 1)) * g.n + c0 + (q & 1)) * g.nz + z;
+    vstore<ZV>(outp, ((active >> q) & 1u) ? acc[q] : vzero<ZV>());
   }
-  vstore<ZV>(outp, bp_voxel_wave<ZV>(g, sino, pix, z, lane));
 }
 
 __global__ __launch_bounds__(256) void par_fp_incl_kernel(ParDev g, const float* __restrict__ vol,
@@ -283,7 +288,7 @@ extern "C" int vamx_par_create(const vamx_par_geom* geom, vamx_plan** out_plan) 
   g.fp_tk = 8;
   g.fp_zv = 4;
   g.fp_reuse = 1;
-  g.bp_tile = 16;
+  g.bp_tile = 8;
   g.bp_zv = 4;
   p->minmax_parts = 256;
   hipError_t e = hipSuccess;
@@ -378,7 +383,7 @@ int par_backward_launch(vamx_plan* p, const float* sino, float* vol, hipStream_t
   }
   const int wz = par_wave_zv(p, sino, vol, g.bp_zv);
   if (wz) {
-    const long long threads = bp_tiled_pix(g) * (g.nz / (64 * wz)) * 64;
+    const long long threads = bp_tiled_quads(g) * (g.nz / (64 * wz)) * 64;
     const dim3 grid(blocks_for(threads, 256)), block(256);
     const int cc = clipc ? 1 : 0;
     if (wz == 4)

@@ -126,7 +126,8 @@ class ParallelPlan:
     FP_CANDIDATES = [(64, 16, 4, 1), (128, 8, 4, 1), (64, 8, 4, 1), (128, 16, 4, 1), (64, 32, 4, 1), (32, 32, 4, 1),
                      (16, 64, 4, 1), (64, 16, 2, 1), (32, 32, 2, 1), (64, 16, 4, 0), (32, 32, 4, 0), (16, 32, 4, 0),
                      (32, 32, 1, 0)]
-    BP_CANDIDATES = [(32, 4), (16, 4), (64, 4), (32, 2), (16, 2), (24, 4)]
+    # (tile edge in 2x2 voxel quads, z values per lane)
+    BP_CANDIDATES = [(8, 4), (16, 4), (4, 4), (32, 4), (8, 2), (16, 2)]
 
     def autotune(self, vol: torch.Tensor | None = None, sino: torch.Tensor | None = None, reps: int = 3) -> dict:
         """Time the forward and back projection for each candidate work order
