@@ -203,13 +203,20 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="run only the forward / back projection kernels of the workload (5 launches each) "
                          "and a 1 GiB device copy, for hardware-counter profiling")
+    ap.add_argument("--workload", choices=("bclp", "osmo", "batched", "raytrace"), default="bclp",
+                    help="bclp (default): fused BCLP at 256^3 x 360; osmo: plain forward/back projection and "
+                         "OSMO at 256^3 x 360; batched: OSMO on 64 uneven problems in one persistent launch; "
+                         "raytrace: curved-ray tracer at 192^3 x 360 angles (RK4) and its BCLP iteration")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the light-propagation path runs on a single MI355X; "
               "multi-GPU runs are not part of this build (see DESIGN.md for how it would shard)",
               file=sys.stderr)
         sys.exit(2)
+    {"bclp": run_bclp, "osmo": run_osmo, "batched": run_batched, "raytrace": run_raytrace}[args.workload](args)
 
+
+def run_bclp(args):
     import torch
     import vamtoolbox_amd as vam
     from vamtoolbox_amd import _lib
@@ -441,6 +448,277 @@ def main():
         "tuning": tuning,
     }
     print(json.dumps(out), flush=True)
+
+
+def _line(metric, value, unit, ms, args, config, roofline, cpu, **extra):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic", "config": config, "roofline": roofline, "cpu_baseline": cpu}
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+def _time_events(fn, reps):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def _ellipsoid_union(shape, count, semi, centre_radius, rng, device):
+    """Binary union of `count` random ellipsoids (semi-axes U[semi], centres in a
+    ball of `centre_radius` voxels, random rotations), built on the GPU."""
+    import torch
+    n, _, nz = shape
+    ax = torch.arange(n, device=device, dtype=torch.float32) - (n - 1) / 2
+    zz = torch.arange(nz, device=device, dtype=torch.float32) - (nz - 1) / 2
+    Y, X, Z = torch.meshgrid(ax, ax, zz, indexing="ij")
+    vol = torch.zeros(shape, device=device)
+    for _ in range(count):
+        a, b, c = rng.uniform(semi[0], semi[1], 3)
+        c = min(c, nz / 2) if nz < 2 * semi[1] else c
+        ctr = rng.uniform(-centre_radius, centre_radius, 3)
+        ctr[2] = rng.uniform(-nz / 4, nz / 4)
+        Rt = torch.tensor(_rotation(rng), device=device, dtype=torch.float32)
+        u = torch.stack([X - ctr[0], Y - ctr[1], Z - ctr[2]], dim=-1) @ Rt.T
+        inside = (u[..., 0] / a) ** 2 + (u[..., 1] / b) ** 2 + (u[..., 2] / c) ** 2 <= 1
+        vol = torch.maximum(vol, inside.float())
+    return vol
+
+
+# ---------------------------------------------------------------------------
+# OSMO at 256^3 x 360 (plain forward / back projection + 50 OSMO iterations)
+# ---------------------------------------------------------------------------
+def run_osmo(args):
+    import torch
+    import vamtoolbox_amd as vam
+    from vamtoolbox_amd.util.data import clipToCircle
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    rng = np.random.default_rng(2)
+    T = clipToCircle(_ellipsoid_union((N, N, NZ), 8, (15, 70), 50, rng, dev))
+    angles = np.arange(NA, dtype=np.float64)
+    tg = vam.geometry.TargetGeometry(target=T.cpu().numpy())
+    pg = vam.geometry.ProjectionGeometry(angles, "parallel", CUDA=True, absorption_per_voxel=MU, det_offset=OFFSET)
+    P = vam.projectorconstructor.projectorconstructor(tg, pg)
+    plan = P.plan
+    sino = torch.empty(plan.sino_shape, device=dev)
+    vol = torch.empty(plan.vol_shape, device=dev)
+    fp_ms = _time_events(lambda: plan.forward(T, out=sino), 10)
+    bp_ms = _time_events(lambda: plan.backward(sino, out=vol), 10)
+    V, S = N * N * NZ, N * NA * NZ
+    fp_bytes, bp_bytes = 4.0 * (NA * V + S), 4.0 * (NA * V + V)
+    log("FP %.3f ms (%.0f GB/s)  BP %.3f ms (%.0f GB/s)" % (fp_ms, fp_bytes / fp_ms / 1e6, bp_ms,
+                                                         bp_bytes / bp_ms / 1e6))
+    n_iter = 50
+    opts = vam.optimize.Options(method="OSMO", n_iter=n_iter, d_h=0.8, d_l=0.7, filter="ram-lak")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sino_o, recon, err = vam.optimize.optimize(tg, pg, opts)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    it_s = (n_iter - 1) / max(float(opts.iter_times[-1] - opts.iter_times[0]), 1e-9)
+    log("OSMO %d iterations: %.2f s total (with host VER each iteration), %.1f iters/s; VER %.4f -> %.4f" % (
+        n_iter, dt, it_s, err[0], err[-1]))
+    achieved = fp_bytes / (fp_ms * 1e-3) / 1e9
+    _line("OSMO iters/s at 256^3 x 360 angles", it_s, "iters/s", 1e3 / it_s, args,
+          {"workload": "OSMO 50 iterations, 256^3 binary target of 8 ellipsoids, 360 angles, absorption "
+                       "0.01/voxel, detector offset 10 voxels (2 forward + 2 back projections per iteration)",
+           "n": N, "nz": NZ, "n_angles": NA},
+          {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "par_fp_wave_kernel", "kernel_ms": fp_ms,
+           "algorithmic_bytes": fp_bytes},
+          None, forward_ms=fp_ms, backward_ms=bp_ms, forward_gbs=fp_bytes / fp_ms / 1e6,
+          backward_gbs=bp_bytes / bp_ms / 1e6, ver_first=float(err[0]), ver_last=float(err[-1]),
+          total_s=dt)
+
+
+# ---------------------------------------------------------------------------
+# 64 uneven OSMO problems in one persistent launch per projection
+# ---------------------------------------------------------------------------
+def run_batched(args):
+    import torch
+    import vamtoolbox_amd as vam
+    from vamtoolbox_amd.batch import OSMOBatch
+    from vamtoolbox_amd.util.data import clipToCircle
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    rng = np.random.default_rng(4)
+    tgs, pgs, specs = [], [], []
+    log("building 64 problems")
+    for _ in range(64):
+        n = int(rng.choice([128, 192, 256, 384]))
+        nz = int(rng.integers(32, 129))
+        na = int(rng.choice([90, 180, 360, 720]))
+        mu = float(rng.uniform(0.0, 0.02))
+        T = clipToCircle(_ellipsoid_union((n, n, nz), 5, (0.08 * n, 0.27 * n), 0.2 * n, rng, dev))
+        tgs.append(vam.geometry.TargetGeometry(target=T.cpu().numpy()))
+        pgs.append(vam.geometry.ProjectionGeometry(np.linspace(0, 360, na, endpoint=False), "parallel", CUDA=True,
+                                                   absorption_per_voxel=mu))
+        specs.append((n, nz, na, mu))
+    opts = vam.optimize.Options(method="OSMO", n_iter=args.steps + args.warmup + 1, d_h=0.8, d_l=0.7,
+                                filter="ram-lak")
+    log("setting up the batch")
+    st = OSMOBatch(tgs, pgs, opts, device=dev)
+    B = st.batch
+    samples = sum(na * n * n * nz for n, nz, na, _ in specs)
+    sino_el = sum(n * na * nz for n, nz, na, _ in specs)
+    vol_el = sum(n * n * nz for n, nz, _, _ in specs)
+    fp_bytes = 4.0 * (samples + sino_el)
+    bp_bytes = 4.0 * (samples + vol_el)
+    fp_ms = _time_events(lambda: B.forward(st.x_model, out=st.b), 5)
+    bp_ms = _time_events(lambda: B.backward(st.b, out=st.x), 5)
+    log("batched FP %.3f ms (%.0f GB/s algorithmic)  BP %.3f ms (%.0f GB/s)" % (
+        fp_ms, fp_bytes / fp_ms / 1e6, bp_ms, bp_bytes / bp_ms / 1e6))
+    # restart from the initial state for the timed iterations
+    st = OSMOBatch(tgs, pgs, opts, device=dev)
+    st.step()
+    st.capture()
+    for _ in range(max(args.warmup - 1, 0)):
+        st.graph.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st.graph.replay()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms = dt / args.steps * 1e3
+    ver = st.ver()
+    log("batched OSMO: %.2f ms per iteration of all 64 problems (%.1f iters/s); mean VER %.4f" % (
+        ms, 1e3 / ms, float(np.mean(ver))))
+    achieved = fp_bytes / (fp_ms * 1e-3) / 1e9
+    _line("batched OSMO iters/s, 64 uneven problems in one persistent launch", 1e3 / ms, "iters/s", ms, args,
+          {"workload": "64 OSMO problems, n in {128,192,256,384}, nz in U{32..128}, angles in {90,180,360,720}, "
+                       "absorption U[0,0.02]/voxel, ellipsoid-union targets; one batched forward and one "
+                       "batched back projection launch per half step, HIP-graph-replayed iteration",
+           "problems": 64, "voxels": vol_el, "sinogram_elements": sino_el, "samples_per_projection": samples},
+          {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "batch_kernel<true>", "kernel_ms": fp_ms,
+           "algorithmic_bytes": fp_bytes},
+          None, forward_ms=fp_ms, backward_ms=bp_ms, forward_gbs=fp_bytes / fp_ms / 1e6,
+          backward_gbs=bp_bytes / bp_ms / 1e6, mean_ver=float(np.mean(ver)))
+
+
+# ---------------------------------------------------------------------------
+# curved-ray tracer at 192^3, 360 angles x 192^2 rays, RK4, and its BCLP iteration
+# ---------------------------------------------------------------------------
+def run_raytrace(args):
+    import torch
+    import vamtoolbox_amd as vam
+    from vamtoolbox_amd import _lib
+    from vamtoolbox_amd.optimizer.BCLP import BCLPNorm
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    n, na = 192, 360
+    rng = np.random.default_rng(5)
+    T = _ellipsoid_union((n, n, n), 8, (12, 50), 40, rng, dev)
+    r = torch.arange(-8, 9, device=dev, dtype=torch.float32)
+    k = torch.exp(-r ** 2 / 8.0)
+    k = k / k.sum()
+    v = T[None, None]
+    for dim in range(3):
+        shape = [1, 1, 1, 1, 1]
+        shape[2 + dim] = 17
+        pad = [0, 0, 0, 0, 0, 0]
+        pad[2 * (2 - dim)] = 8
+        pad[2 * (2 - dim) + 1] = 8
+        v = torch.nn.functional.conv3d(torch.nn.functional.pad(v, pad), k.reshape(shape))
+    T = v[0, 0].clamp(0, 1)
+    tg = vam.geometry.TargetGeometry(target=T.cpu().numpy())
+    cv = tg.constructCoordVec()
+    vox = float(cv[0][1] - cv[0][0])
+    X, Y, Z = np.meshgrid(*cv, indexing="ij")
+    nf = np.full(X.shape, 1.5)
+    for _ in range(6):
+        c = rng.uniform(-0.5, 0.5, 3) * np.array([cv[0].max(), cv[1].max(), cv[2].max()])
+        s = rng.uniform(10, 30) * vox
+        nf += rng.choice([-0.02, 0.02]) * np.exp(-((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2) / (2 * s * s))
+    im = vam.medium.IndexModel(cv, type="interpolation", form="freeform", n_x=nf)
+    am = vam.medium.AttenuationModel(cv, type="analytical", form="homogeneous_cylinder", alpha_internal=0.005 / vox)
+    ab = vam.medium.AbsorptionModel(cv, type="analytical", form="homogeneous_cylinder", alpha_internal=0.005 / vox)
+    angles = np.arange(na, dtype=np.float64)
+    pg = vam.geometry.ProjectionGeometry(angles, "ray_trace", CUDA=True, index_model=im, attenuation_model=am,
+                                         absorption_model=ab, ray_trace_method="eikonal",
+                                         eikonal_parametrization="canonical", ray_trace_ode_solver="rk4",
+                                         ray_trace_ray_config="parallel")
+    P = vam.projectorconstructor.projectorconstructor(tg, pg)
+    P.output_torch_tensor = True
+    f = T.reshape(-1).contiguous()
+    g = P.forward(f)
+    fwd_ms = _time_events(lambda: P.forward(f), 3)
+    steps_f = P.stats()
+    bwd_ms = _time_events(lambda: P.backward(g), 3)
+    steps_b = P.stats()
+    in_box = float(steps_f[0])
+    log("ray trace: integrate %.1f ms, deposit %.1f ms, %.3e ray-steps per trace (%d unfinished)" % (
+        fwd_ms, bwd_ms, in_box, steps_f[1]))
+    # BCLP on the ray tracer (unfused loop: trace-deposit, response, trace-integrate)
+    rm = vam.response.ResponseModel(type="analytical", B=20, M=0.6)
+    g0 = torch.clamp(P.forward(f), min=0)
+    d0 = P.backward(g0).reshape(T.shape)
+    g0 = (g0 * (0.7 / float(d0[T > 0.5].mean()))).reshape(-1).contiguous()
+    opts = vam.optimize.Options(method="BCLP", n_iter=20, response_model=rm, eps=0.05, p=2, q=1,
+                                learning_rate=0.01, g0=g0, log_metrics=False)
+    bc = BCLPNorm(tg, pg, opts)
+    gi = bc.g0.reshape(-1).clone()
+    # step size: the first step moves g by 5 % of its mean magnitude
+    _, _, _, _, _, loss0, op0 = bc._eval_state(gi)
+    G0 = bc._grad(op0, float(loss0.item()))
+    lr = 0.05 * float(gi.abs().mean()) / max(float(G0.abs().mean()), 1e-30)
+    times, losses = [], []
+    for it in range(20):
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        d_, m_, eT, eb, vmask, loss, operand = bc._eval_state(gi)
+        L = float(loss.item())
+        G = bc._grad(operand, L)
+        gi = bc.imposeSinogramConstraints(gi - lr * G.reshape(-1)).reshape(-1)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - ta)
+        losses.append(L)
+    it_ms = 1e3 * float(np.mean(times[2:]))
+    log("ray-trace BCLP: %.1f ms per iteration; loss %.4g -> %.4g" % (it_ms, losses[0], losses[-1]))
+    bytes_per_step = 72.0
+    achieved = in_box * bytes_per_step / (fwd_ms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        cpu = _raytrace_cpu_baseline(tg, pg, cv, nf, am, ab, na)
+    _line("ray-trace BCLP iters/s at 192^3 x 360 angles (RK4)", 1e3 / it_ms, "iters/s", it_ms, args,
+          {"workload": "curved-ray tracer, 192^3 medium n = 1.5 + 6 Gaussian bumps (+-0.02, sigma 10-30 voxels), "
+                       "attenuation and absorption 0.005/voxel in the cylinder, 360 angles x 192 x 192 rays, RK4, "
+                       "step 0.5 voxel; BCLP iteration = deposit + response + integrate",
+           "n": n, "n_angles": na, "rays": int(192 * 192 * na)},
+          {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "rt_trace_kernel<false>",
+           "kernel_ms": fwd_ms, "algorithmic_bytes": in_box * bytes_per_step},
+          cpu, integrate_ms=fwd_ms, deposit_ms=bwd_ms, ray_steps=in_box, ray_steps_per_s=in_box / (fwd_ms * 1e-3),
+          unfinished_rays=int(steps_f[1]), loss_first=losses[0], loss_last=losses[-1], learning_rate=lr)
+
+
+def _raytrace_cpu_baseline(tg, pg, cv, nf, am, ab, na, k_angles=1):
+    """float64 NumPy port of the reference tracer (RK4 here) on k_angles of the
+    360 angles, scaled to one integrate pass."""
+    import oracle.raytrace as ort
+    med = ort.GridMedium(cv, nf)
+    tr = ort.Tracer(med, param="canonical", solver="rk4",
+                    att=("cylinder", am.params["alpha_internal"], am.params["R_physical"], None),
+                    abs_grid=ab.alpha_at_grid())
+    x0, v0, shape, e0 = ort.setup_rays(cv, pg.angles[:k_angles])
+    f = np.asarray(tg.array, dtype=np.float64)
+    t0 = time.perf_counter()
+    tr.run(x0, v0, e0, f=f)
+    dt = time.perf_counter() - t0
+    per = dt * (na / k_angles)
+    return {"value": 1.0 / per, "unit": "integrate passes/s", "cores": 1, "kind": "port",
+            "sample": "float64 NumPy port of the reference ray tracer (RK4, canonical), %d of %d angles "
+                      "(%d rays), %.1f s measured, scaled x%d" % (k_angles, na, x0.shape[0], dt, na // k_angles),
+            "host": "%d host cores visible (%s)" % (os.cpu_count() or 0, _cpu_model())}
 
 
 def _cpu_model():
