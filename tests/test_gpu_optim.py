@@ -195,11 +195,13 @@ def test_autograd_dot_products(gpu):
     assert abs(lhs - rhs) / abs(lhs) <= 1e-6
 
 
-def test_bclp_fused_wave_path_matches_oracle(gpu):
-    """nz = 64: the fused BCLP kernels take the wavefront-uniform path."""
+@pytest.mark.parametrize("nz", [64, 40])
+def test_bclp_fused_wave_path_matches_oracle(gpu, nz):
+    """nz = 64 and 40 (a partial z chunk): the fused BCLP kernels take the
+    wavefront-uniform path."""
     import oracle.optim as oo
     import vamtoolbox_amd as vam
-    tg, pg, ang = _problem(n=24, nz=64, na=30, seed=4, gray=True, absorption_per_voxel=0.01, det_offset=1.0)
+    tg, pg, ang = _problem(n=24, nz=nz, na=30, seed=4, gray=True, absorption_per_voxel=0.01, det_offset=1.0)
     fwd, bwd = _oracle_ops(ang, mu=0.01, det_offset=1.0)
     g0 = 0.8 * oo.bclp_init(tg.array, fwd, oo.Response("interpolation", B=20, M=0.6), len(ang))
     rm = vam.response.ResponseModel(type="analytical", B=20, M=0.6)

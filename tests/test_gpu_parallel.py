@@ -131,11 +131,15 @@ def test_determinism(gpu):
     assert torch.equal(a1, a2) and torch.equal(b1, b2)
 
 
-# shapes where a wavefront covers one ray / voxel column (nz a multiple of 64)
+# shapes taken by the wavefront-uniform kernels: a wavefront covers one ray /
+# voxel quad and 64 x ZV consecutive z; the last z chunk may be partial
 WAVE_CASES = [
     (32, 64, np.arange(0, 360, 9.0), {}),
     (24, 128, np.arange(0, 180, 6.0), dict(mu=0.03, det_offset=2.0)),
     (20, 256, np.arange(0, 360, 20.0), dict(det_offset=-1.5)),
+    (27, 40, np.arange(0, 360, 11.0), dict(mu=0.02)),                 # one partial chunk, ZV = 1
+    (22, 100, np.arange(0, 180, 7.5), dict(det_offset=0.5)),          # partial chunk, ZV = 2
+    (17, 198, np.arange(3, 360, 13.0), {}),                           # odd n, 2 chunks of 128, ZV = 2
 ]
 
 

@@ -2,12 +2,15 @@
 // size in one persistent launch (gfx950).
 //
 // Each problem is a parallel plan (vamx_par_create) with its own volume and
-// sinogram.  The flattened work of every problem is cut into tiles of 256
-// (ray or voxel column, z-vector) items; tiles are ordered largest problem
-// first and handed out by a device-side counter, so the resident blocks of
-// the single launch drain one queue until it is empty (every block exits
-// when the counter passes the tile count).  The counter is reset by a
-// one-thread kernel on the same stream, so the sequence is graph-capturable.
+// sinogram.  A problem's work is cut into tiles of one 256-thread block:
+// four wavefronts of the wavefront-uniform kernels (one ray or one 2x2 voxel
+// quad and 64 x ZV z values per wavefront, in the plan's L2-aware order) or,
+// for shapes those do not take (inclination, very thin z), 256 items of the
+// per-lane kernels.  Tiles are ordered largest problem first and handed out
+// by a device-side counter, so the resident blocks of the single launch drain
+// one queue until it is empty (every block exits when the counter passes the
+// tile count).  The counter is reset by a one-thread kernel on the same
+// stream, so the sequence is graph-capturable.
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -22,22 +25,92 @@ struct BatchItem {
   float* out;
   const float* minmax;
   long long tile_begin;  // first tile of this problem
-  long long items;       // work items of this problem
-  int zv;                // 4 or 1
+  long long items;       // wavefronts (wave mode) or threads (per-lane mode) of this problem
+  int zv;                // z values per lane: 1, 2, 4 (wave mode) or 1, 4 (per-lane mode)
   int clip;              // forward: warp clip; backward: circle clip
+  int wave;              // 1: wavefront-uniform kernels
 };
 
 __global__ void batch_reset(unsigned long long* counter) { *counter = 0ull; }
+
+template <int ZV>
+__device__ __forceinline__ void batch_fp_wave(const BatchItem& it, long long w, int lane) {
+  const ParDev& g = it.g;
+  int k, i, z;
+  if (!fp_wave_item<ZV>(w, g, lane, k, i, z)) return;
+  const int zl = z < g.nz - ZV ? z : g.nz - ZV;
+  const bool clip_on = it.clip && wave_clip_active<ZV>(it.minmax, g.nz, zl);
+  const typename VecT<ZV>::T acc = clip_on ? fp_ray_wave<ZV, true, true>(g, it.in, it.minmax, k, i, zl, lane)
+                                           : fp_ray_wave<ZV, false, true>(g, it.in, it.minmax, k, i, zl, lane);
+  if (z < g.nz) vstore<ZV>(it.out + ((size_t)k * g.na + i) * g.nz + z, acc);
+}
+
+template <int ZV>
+__device__ __forceinline__ void batch_bp_wave(const BatchItem& it, long long w, int lane) {
+  const ParDev& g = it.g;
+  int r0, c0, z;
+  if (!bp_wave_quad<ZV>(w, g, lane, r0, c0, z)) return;
+  const int zl = z < g.nz - ZV ? z : g.nz - ZV;
+  unsigned inrange, active;
+  bp_quad_masks(g, r0, c0, it.clip, inrange, active);
+  typename VecT<ZV>::T acc[4];
+  bp_quad_wave<ZV>(g, it.in, r0, c0, active, zl, lane, acc);
+  if (z >= g.nz) return;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (!((inrange >> q) & 1u)) continue;
+    float* outp = it.out + ((size_t)(r0 + (q >> 1)) * g.n + c0 + (q & 1)) * g.nz + z;
+    vstore<ZV>(outp, ((active >> q) & 1u) ? acc[q] : vzero<ZV>());
+  }
+}
+
+template <bool FORWARD>
+__device__ __forceinline__ void batch_lane_item(const BatchItem& it, long long w) {
+  const ParDev& g = it.g;
+  const int nzv = g.nz / it.zv;
+  const int z = (int)(w % nzv) * it.zv;
+  const long long outer = w / nzv;
+  if (FORWARD) {
+    const int k = (int)(outer % g.n), i = (int)(outer / g.n);
+    float* dst = it.out + ((size_t)k * g.na + i) * g.nz + z;
+    if (g.incl) {
+      *dst = fp_ray_incl(g, it.in, k, i, z);
+    } else if (it.zv == 4) {
+      vstore<4>(dst, it.clip ? fp_ray<4, true>(g, it.in, it.minmax, k, i, z)
+                             : fp_ray<4, false>(g, it.in, it.minmax, k, i, z));
+    } else {
+      *dst = it.clip ? fp_ray<1, true>(g, it.in, it.minmax, k, i, z)
+                     : fp_ray<1, false>(g, it.in, it.minmax, k, i, z);
+    }
+  } else {
+    const int pix = (int)outer;
+    float* dst = it.out + (size_t)pix * g.nz + z;
+    const bool zero = it.clip && g.outside[pix];
+    if (g.incl) {
+      *dst = zero ? 0.0f : bp_voxel_incl(g, it.in, pix, z);
+    } else if (it.zv == 4) {
+      vstore<4>(dst, zero ? vzero<4>() : bp_voxel<4>(g, it.in, pix, z));
+    } else {
+      *dst = zero ? 0.0f : bp_voxel<1>(g, it.in, pix, z);
+    }
+  }
+}
 
 template <bool FORWARD>
 __global__ __launch_bounds__(256) void batch_kernel(const BatchItem* __restrict__ items, int nitems,
                                                     long long ntiles, unsigned long long* counter) {
   __shared__ long long tile_sh;
+  const int lane = threadIdx.x & 63;
+  const int slot = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   for (;;) {
     if (threadIdx.x == 0) tile_sh = (long long)atomicAdd(counter, 1ull);
     __syncthreads();
-    const long long tile = tile_sh;
+    const long long tv = tile_sh;
     __syncthreads();
+    // the tile is block-uniform: keep it (and the item search) scalar
+    const long long tile = (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(tv >> 32))
+                                        << 32) |
+                                       (unsigned)__builtin_amdgcn_readfirstlane((int)(tv & 0xffffffffll)));
     if (tile >= ntiles) return;
     // problem owning this tile (items are ordered by tile_begin)
     int lo = 0, hi = nitems - 1;
@@ -47,34 +120,25 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchItem* __restrict_
     }
     const BatchItem& it = items[lo];
     const ParDev& g = it.g;
-    const long long w = (tile - it.tile_begin) * 256 + threadIdx.x;
-    if (w >= it.items) continue;
-    const int nzv = g.nz / it.zv;
-    const int z = (int)(w % nzv) * it.zv;
-    const long long outer = w / nzv;
-    if (FORWARD) {
-      const int k = (int)(outer % g.n), i = (int)(outer / g.n);
-      float* dst = it.out + ((size_t)k * g.na + i) * g.nz + z;
-      if (g.incl) {
-        *dst = fp_ray_incl(g, it.in, k, i, z);
-      } else if (it.zv == 4) {
-        vstore<4>(dst, it.clip ? fp_ray<4, true>(g, it.in, it.minmax, k, i, z)
-                               : fp_ray<4, false>(g, it.in, it.minmax, k, i, z));
-      } else {
-        *dst = it.clip ? fp_ray<1, true>(g, it.in, it.minmax, k, i, z)
-                       : fp_ray<1, false>(g, it.in, it.minmax, k, i, z);
+    // No `continue` below: a per-thread continue makes the loop header (and
+    // its barriers) run for a subset of lanes, so waves of the block would
+    // disagree on the barrier count.  Every thread reaches the loop end.
+    if (it.wave) {
+      const long long w = (tile - it.tile_begin) * 4 + slot;
+      if (w < it.items) {
+        if (FORWARD) {
+          if (it.zv == 4) batch_fp_wave<4>(it, w, lane);
+          else if (it.zv == 2) batch_fp_wave<2>(it, w, lane);
+          else batch_fp_wave<1>(it, w, lane);
+        } else {
+          if (it.zv == 4) batch_bp_wave<4>(it, w, lane);
+          else if (it.zv == 2) batch_bp_wave<2>(it, w, lane);
+          else batch_bp_wave<1>(it, w, lane);
+        }
       }
     } else {
-      const int pix = (int)outer;
-      float* dst = it.out + (size_t)pix * g.nz + z;
-      const bool zero = it.clip && g.outside[pix];
-      if (g.incl) {
-        *dst = zero ? 0.0f : bp_voxel_incl(g, it.in, pix, z);
-      } else if (it.zv == 4) {
-        vstore<4>(dst, zero ? vzero<4>() : bp_voxel<4>(g, it.in, pix, z));
-      } else {
-        *dst = zero ? 0.0f : bp_voxel<1>(g, it.in, pix, z);
-      }
+      const long long w = (tile - it.tile_begin) * 256 + threadIdx.x;
+      if (w < it.items) batch_lane_item<FORWARD>(it, w);
     }
   }
 }
@@ -118,10 +182,18 @@ int run_batched(vamx_plan* const* plans, int n, const float* const* ins, float* 
     it.in = ins[i];
     it.out = outs[i];
     it.minmax = p->d_minmax;
-    it.zv = par_use_vec4(p, ins[i], outs[i]) ? 4 : 1;
     it.clip = forward ? !(p->flags & VAMX_PAR_NO_WARP_CLIP) : !(p->flags & VAMX_PAR_NO_CIRCLE_CLIP);
-    const long long outer = forward ? (long long)it.g.na * it.g.n : (long long)it.g.n * it.g.n;
-    it.items = outer * (it.g.nz / it.zv);
+    const int wz = par_wave_zv(p, ins[i], outs[i], forward ? it.g.fp_zv : it.g.bp_zv);
+    if (wz) {
+      it.wave = 1;
+      it.zv = wz;
+      it.items = (forward ? fp_tiled_rays(it.g) : bp_tiled_quads(it.g)) * wave_zchunks(it.g.nz, wz);
+    } else {
+      it.wave = 0;
+      it.zv = par_use_vec4(p, ins[i], outs[i]) ? 4 : 1;
+      const long long outer = forward ? (long long)it.g.na * it.g.n : (long long)it.g.n * it.g.n;
+      it.items = outer * (it.g.nz / it.zv);
+    }
     if (forward && it.clip && !it.g.incl) {
       int rc = par_slice_minmax(p, ins[i], s);
       if (rc) return rc;
@@ -129,9 +201,9 @@ int run_batched(vamx_plan* const* plans, int n, const float* const* ins, float* 
   }
   // largest work first (cost ~ items * inner loop length)
   std::iota(order.begin(), order.end(), 0);
-  auto cost = [&](int i) {
+  auto cost = [&](int i) {   // interpolated samples of the problem
     const BatchItem& it = items[(size_t)i];
-    return (double)it.items * it.zv * (forward ? it.g.n : it.g.na);
+    return (double)it.g.n * it.g.n * it.g.na * it.g.nz;
   };
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
   std::vector<BatchItem> sorted((size_t)n);
@@ -139,7 +211,7 @@ int run_batched(vamx_plan* const* plans, int n, const float* const* ins, float* 
   for (int r = 0; r < n; ++r) {
     sorted[(size_t)r] = items[(size_t)order[(size_t)r]];
     sorted[(size_t)r].tile_begin = tiles;
-    tiles += (sorted[(size_t)r].items + 255) / 256;
+    tiles += (sorted[(size_t)r].items + (sorted[(size_t)r].wave ? 3 : 255)) / (sorted[(size_t)r].wave ? 4 : 256);
   }
   BatchScratch& sc = g_scratch;
   if (sc.device != dev) {

@@ -163,10 +163,12 @@ __global__ __launch_bounds__(256) void bclp_bp_epilogue_wave(ParDev g, int clipc
   double part = 0.0;
   int r0 = 0, c0 = 0, z = 0;
   if (bp_wave_quad<ZV>(wave, g, lane, r0, c0, z)) {
+    const int zl = z < g.nz - ZV ? z : g.nz - ZV;
     unsigned inrange, active;
     bp_quad_masks(g, r0, c0, clipc, inrange, active);
     typename VecT<ZV>::T acc[4];
-    bp_quad_wave<ZV>(g, gsino, r0, c0, active, z, lane, acc);
+    bp_quad_wave<ZV>(g, gsino, r0, c0, active, zl, lane, acc);
+    if (z >= g.nz) inrange = 0u;   // lanes past nz: no store, no loss term
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       if (!((inrange >> q) & 1u)) continue;
@@ -199,9 +201,11 @@ __global__ __launch_bounds__(256) void bclp_fp_update_wave(ParDev g, const float
   const long long wave = wave_index(xcd_remap(blockIdx.x, gridDim.x));
   int k, i, z;
   if (!fp_wave_item<ZV>(wave, g, lane, k, i, z)) return;
-  const bool clip_on = clip && wave_clip_active<ZV>(minmax, g.nz, z);
-  const typename VecT<ZV>::T fp = clip_on ? fp_ray_wave<ZV, true, REUSE>(g, operand, minmax, k, i, z, lane)
-                                          : fp_ray_wave<ZV, false, REUSE>(g, operand, minmax, k, i, z, lane);
+  const int zl = z < g.nz - ZV ? z : g.nz - ZV;
+  const bool clip_on = clip && wave_clip_active<ZV>(minmax, g.nz, zl);
+  const typename VecT<ZV>::T fp = clip_on ? fp_ray_wave<ZV, true, REUSE>(g, operand, minmax, k, i, zl, lane)
+                                          : fp_ray_wave<ZV, false, REUSE>(g, operand, minmax, k, i, zl, lane);
+  if (z >= g.nz) return;
   const float scale = (float)(*scale_p);
   float* gp = gsino + ((size_t)k * g.na + i) * g.nz + z;
   typename VecT<ZV>::T gv = vload<ZV>(gp);
@@ -246,6 +250,7 @@ __global__ __launch_bounds__(256) void bclp_fp_update(ParDev g, const float* __r
   typename VecT<ZV>::T fp;
   if constexpr (INCL) fp = fp_ray_incl(g, operand, k, i, z);
   else fp = fp_ray<ZV, CLIP>(g, operand, minmax, k, i, z);
+  if (z >= g.nz) return;
   const float scale = (float)(*scale_p);
   float* gp = gsino + ((size_t)k * g.na + i) * g.nz + z;
   typename VecT<ZV>::T gv = vload<ZV>(gp);
@@ -272,6 +277,7 @@ __global__ __launch_bounds__(256) void bclp_bpT_update(ParDev g, int clipc, cons
   typename VecT<ZV>::T gr;
   if constexpr (INCL) gr = bpT_bin_incl(g, operand, k, i, z, clipc);
   else gr = bpT_bin<ZV>(g, operand, k, i, z, clipc);
+  if (z >= g.nz) return;
   const float scale = (float)(*scale_p);
   float* gp = gsino + ((size_t)k * g.na + i) * g.nz + z;
   typename VecT<ZV>::T gv = vload<ZV>(gp);
@@ -295,8 +301,8 @@ WsLayout ws_layout(const vamx_plan* p) {
   const ParDev& g = p->par;
   WsLayout w;
   const long long items = (long long)g.n * g.n * g.nz;  // worst case ZV = 1
-  // wavefront path: one wave per quad and 64 z (ZV = 1), any tile setting
-  const long long tiled = bp_tiled_quads_max(g) * g.nz;
+  // wavefront path: one wave per quad and z chunk (widest grid: ZV = 1), any tile setting
+  const long long tiled = bp_tiled_quads_max(g) * wave_zchunks(g.nz, 1) * 64;
   w.nparts = (int)blocks_for(items > tiled ? items : tiled, 256);
   w.partials_off = 0;
   w.scale_off = ((size_t)w.nparts * sizeof(double) + 255) / 256 * 256;
@@ -331,14 +337,14 @@ int launch_epilogue(vamx_plan* p, const RespDev& r, const BclpDev& b, const floa
   const bool v4 = !gd.incl && gd.nz % 4 == 0 && ((uintptr_t)g | (uintptr_t)target | (uintptr_t)dose |
                                                  (uintptr_t)operand) % 16 == 0 &&
                   (!b.eps_arr || (uintptr_t)b.eps_arr % 16 == 0) && (!b.w_arr || (uintptr_t)b.w_arr % 16 == 0);
-  int wz = par_wave_zv(p, g, dose, gd.bp_zv);
-  if (wz) {
-    const uintptr_t al = (uintptr_t)target | (uintptr_t)operand | (uintptr_t)b.eps_arr | (uintptr_t)b.w_arr;
-    while (wz > 1 && al % (4 * wz) != 0) wz >>= 1;
-    if (gd.nz % (64 * wz) != 0) wz = 0;
+  int wz = 0;
+  if (!(p->flags & VAMX_PAR_PER_LANE_GEOMETRY) && !gd.incl && gd.n >= 2) {
+    const uintptr_t al = (uintptr_t)g | (uintptr_t)dose | (uintptr_t)target | (uintptr_t)operand |
+                         (uintptr_t)b.eps_arr | (uintptr_t)b.w_arr;
+    wz = wave_zv_choice(gd.nz, al, gd.bp_zv);
   }
   if (wz) {
-    const long long threads = bp_tiled_quads(gd) * (gd.nz / (64 * wz)) * 64;
+    const long long threads = bp_tiled_quads(gd) * wave_zchunks(gd.nz, wz) * 64;
     nparts = (int)blocks_for(threads, 256);
     const dim3 grid(nparts), block(256);
     if (wz == 4)
@@ -412,7 +418,7 @@ extern "C" int vamx_bclp_step(vamx_plan* plan, const vamx_response* resp, const 
     }
     const int wz = par_wave_zv(plan, g, operand, gd.fp_zv);
     if (wz) {
-      const long long threads = fp_tiled_rays(gd) * (gd.nz / (64 * wz)) * 64;
+      const long long threads = fp_tiled_rays(gd) * wave_zchunks(gd.nz, wz) * 64;
       const dim3 grid(blocks_for(threads, 256)), block(256);
       const int cl = clip ? 1 : 0;
 #define VAMX_FPU(Z, R)                                                                                 \

@@ -750,10 +750,14 @@ __device__ __forceinline__ long long wave_index(unsigned lb) {
 }
 
 // wave -> (ray, z) of the forward kernels; false for padding waves
+// z chunks of a wavefront: 64 lanes x ZV values; the last chunk may be
+// partial (lanes past nz load from a clamped z and store nothing)
+__host__ __device__ inline int wave_zchunks(int nz, int zv) { return (nz + 64 * zv - 1) / (64 * zv); }
+
 template <int ZV>
 __device__ __forceinline__ bool fp_wave_item(long long wave, const ParDev& g, int lane, int& k, int& i, int& z) {
   const long long rays = fp_tiled_rays(g);
-  const int nch = g.nz / (64 * ZV);
+  const int nch = wave_zchunks(g.nz, ZV);
   if (wave >= rays * nch) return false;
   const int zc = (int)(wave / rays);
   if (!fp_tile_ray(wave % rays, g, k, i)) return false;
@@ -765,7 +769,7 @@ __device__ __forceinline__ bool fp_wave_item(long long wave, const ParDev& g, in
 template <int ZV>
 __device__ __forceinline__ bool bp_wave_quad(long long wave, const ParDev& g, int lane, int& r0, int& c0, int& z) {
   const long long quads = bp_tiled_quads(g);
-  const int nch = g.nz / (64 * ZV);
+  const int nch = wave_zchunks(g.nz, ZV);
   if (wave >= quads * nch) return false;
   const int zc = (int)(wave / quads);
   if (!bp_tile_quad(wave % quads, g, r0, c0)) return false;
@@ -793,5 +797,6 @@ __device__ __forceinline__ void bp_quad_masks(const ParDev& g, int r0, int c0, i
 int par_slice_minmax(vamx_plan* p, const float* vol, hipStream_t s);
 bool par_use_vec4(const vamx_plan* p, const void* a, const void* b);
 int par_wave_zv(const vamx_plan* p, const void* a, const void* b, int requested);
+int wave_zv_choice(int nz, uintptr_t align_bits, int requested);
 
 }  // namespace vamx

@@ -104,10 +104,11 @@ __global__ __launch_bounds__(256) void par_fp_wave_kernel(ParDev g, const float*
   const long long wave = wave_index(xcd_remap(blockIdx.x, gridDim.x));
   int k, i, z;
   if (!fp_wave_item<ZV>(wave, g, lane, k, i, z)) return;
-  const bool clip_on = clip && wave_clip_active<ZV>(minmax, g.nz, z);
-  typename VecT<ZV>::T acc = clip_on ? fp_ray_wave<ZV, true, REUSE>(g, vol, minmax, k, i, z, lane)
-                                     : fp_ray_wave<ZV, false, REUSE>(g, vol, minmax, k, i, z, lane);
-  vstore<ZV>(sino + ((size_t)k * g.na + i) * g.nz + z, acc);
+  const int zl = z < g.nz - ZV ? z : g.nz - ZV;   // lanes past nz compute a copy and store nothing
+  const bool clip_on = clip && wave_clip_active<ZV>(minmax, g.nz, zl);
+  typename VecT<ZV>::T acc = clip_on ? fp_ray_wave<ZV, true, REUSE>(g, vol, minmax, k, i, zl, lane)
+                                     : fp_ray_wave<ZV, false, REUSE>(g, vol, minmax, k, i, zl, lane);
+  if (z < g.nz) vstore<ZV>(sino + ((size_t)k * g.na + i) * g.nz + z, acc);
 }
 
 // wavefront-uniform back projection: one wavefront = one 2x2 quad of voxel
@@ -119,10 +120,12 @@ __global__ __launch_bounds__(256) void par_bp_wave_kernel(ParDev g, const float*
   const long long wave = wave_index(xcd_remap(blockIdx.x, gridDim.x));
   int r0, c0, z;
   if (!bp_wave_quad<ZV>(wave, g, lane, r0, c0, z)) return;
+  const int zl = z < g.nz - ZV ? z : g.nz - ZV;
   unsigned inrange, active;
   bp_quad_masks(g, r0, c0, clipc, inrange, active);
   typename VecT<ZV>::T acc[4];
-  bp_quad_wave<ZV>(g, sino, r0, c0, active, z, lane, acc);
+  bp_quad_wave<ZV>(g, sino, r0, c0, active, zl, lane, acc);
+  if (z >= g.nz) return;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     if (!((inrange >> q) & 1u)) continue;
@@ -223,12 +226,26 @@ __global__ __launch_bounds__(256) void par_bpT_incl_kernel(ParDev g, const float
 using namespace vamx;
 
 int vamx::par_wave_zv(const vamx_plan* p, const void* a, const void* b, int requested) {
-  // largest z-vector width <= requested with nz a multiple of 64 * zv and aligned buffers
+  // z-vector width <= requested for the wavefront kernels: nz a multiple of
+  // zv, buffers aligned to it, and the best lane utilisation
+  // nz / (chunks * 64 * zv) (ties -> the wider vector, fewer waves); 0 (use
+  // the per-lane kernels) when even the best leaves half the lanes idle
   if ((p->flags & VAMX_PAR_PER_LANE_GEOMETRY) || p->par.incl || p->par.n < 2) return 0;
-  const uintptr_t al = (uintptr_t)a | (uintptr_t)b;
-  for (int zv = requested; zv >= 1; zv >>= 1)
-    if (p->par.nz % (64 * zv) == 0 && al % (4 * zv) == 0) return zv;
-  return 0;
+  return wave_zv_choice(p->par.nz, (uintptr_t)a | (uintptr_t)b, requested);
+}
+
+int vamx::wave_zv_choice(int nz, uintptr_t align_bits, int requested) {
+  int best = 0;
+  double best_util = 0.0;
+  for (int zv = 1; zv <= requested && zv <= 4; zv <<= 1) {
+    if (nz % zv != 0 || align_bits % (4 * zv) != 0) continue;
+    const double util = (double)nz / ((double)wave_zchunks(nz, zv) * 64.0 * zv);
+    if (util >= best_util) {
+      best_util = util;
+      best = zv;
+    }
+  }
+  return best_util >= 0.5 ? best : 0;
 }
 
 bool vamx::par_use_vec4(const vamx_plan* p, const void* a, const void* b) {
@@ -336,7 +353,7 @@ int par_forward_launch(vamx_plan* p, const float* vol, float* sino, hipStream_t 
   }
   const int wz = par_wave_zv(p, vol, sino, g.fp_zv);
   if (wz) {
-    const long long threads = fp_tiled_rays(g) * (g.nz / (64 * wz)) * 64;
+    const long long threads = fp_tiled_rays(g) * wave_zchunks(g.nz, wz) * 64;
     const dim3 grid(blocks_for(threads, 256)), block(256);
     const int clip = warp_clip ? 1 : 0;
 #define VAMX_FPW(Z, R) hipLaunchKernelGGL((par_fp_wave_kernel<Z, R>), grid, block, 0, s, g, vol, sino, p->d_minmax, clip)
@@ -383,7 +400,7 @@ int par_backward_launch(vamx_plan* p, const float* sino, float* vol, hipStream_t
   }
   const int wz = par_wave_zv(p, sino, vol, g.bp_zv);
   if (wz) {
-    const long long threads = bp_tiled_quads(g) * (g.nz / (64 * wz)) * 64;
+    const long long threads = bp_tiled_quads(g) * wave_zchunks(g.nz, wz) * 64;
     const dim3 grid(blocks_for(threads, 256)), block(256);
     const int cc = clipc ? 1 : 0;
     if (wz == 4)
