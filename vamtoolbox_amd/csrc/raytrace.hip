@@ -43,8 +43,8 @@ __device__ __forceinline__ d3 scl(double s, d3 a) { return {s * a.x, s * a.y, s 
 // trilinear sample of the float4 field, grid_sample(align_corners=True,
 // padding_mode="border"): index coordinates clamped to [0, size-1]
 __device__ __forceinline__ void field_sample(const RtDev& g, d3 x, double& n, d3& gn) {
-  double ix = (x.x - g.xmin) / g.dx, iy = (x.y - g.ymin) / g.dy;
-  double iz = g.nz > 1 ? (x.z - g.zmin) / g.dz : 0.0;
+  double ix = (x.x - g.xmin) * g.inv_dx, iy = (x.y - g.ymin) * g.inv_dy;
+  double iz = g.nz > 1 ? (x.z - g.zmin) * g.inv_dz : 0.0;
   ix = fmin(fmax(ix, 0.0), (double)(g.nx - 1));
   iy = fmin(fmax(iy, 0.0), (double)(g.ny - 1));
   iz = fmin(fmax(iz, 0.0), (double)(g.nz - 1));
@@ -73,8 +73,8 @@ __device__ __forceinline__ double att_alpha(const RtDev& g, d3 x) {
   if (g.att_kind == VAMX_RT_ATT_BALL) return sqrt(x.x * x.x + x.y * x.y + x.z * x.z) < g.att_radius ? g.att_alpha : 0.0;
   if (g.att_kind == VAMX_RT_ATT_GRID) {
     // grid_sample(align_corners=True, padding_mode="zeros")
-    const double ix = (x.x - g.xmin) / g.dx, iy = (x.y - g.ymin) / g.dy;
-    const double iz = g.nz > 1 ? (x.z - g.zmin) / g.dz : 0.0;
+    const double ix = (x.x - g.xmin) * g.inv_dx, iy = (x.y - g.ymin) * g.inv_dy;
+    const double iz = g.nz > 1 ? (x.z - g.zmin) * g.inv_dz : 0.0;
     const double fx = floor(ix), fy = floor(iy), fz = floor(iz);
     const double wx = ix - fx, wy = iy - fy, wz = iz - fz;
     const int x0 = (int)fx, y0 = (int)fy, z0 = (int)fz;
@@ -90,16 +90,17 @@ __device__ __forceinline__ double att_alpha(const RtDev& g, d3 x) {
   return 0.0;
 }
 
-// right-hand side of the ray equation in the chosen parametrisation
+// right-hand side of the ray equation in parametrisation PARAM
 // (pyTorchRayTrace.py:695-735)
+template <int PARAM>
 __device__ __forceinline__ void rhs(const RtDev& g, d3 x, d3 v, d3& dx, d3& dv) {
   double n;
   d3 gn;
   field_sample(g, x, n, gn);
-  if (g.param == VAMX_RT_CANONICAL) {
+  if (PARAM == VAMX_RT_CANONICAL) {
     dx = v;
     dv = scl(n, gn);
-  } else if (g.param == VAMX_RT_PHYSICAL) {
+  } else if (PARAM == VAMX_RT_PHYSICAL) {
     dx = scl(1.0 / n, v);
     dv = gn;
   } else {
@@ -108,37 +109,50 @@ __device__ __forceinline__ void rhs(const RtDev& g, d3 x, d3 v, d3& dx, d3& dv) 
   }
 }
 
-// one step of the solver: (x, v) -> (x', v')
+template <int PARAM>
+__device__ __forceinline__ d3 accel(double n, d3 gn) {
+  if (PARAM == VAMX_RT_CANONICAL) return scl(n, gn);
+  if (PARAM == VAMX_RT_PHYSICAL) return gn;
+  return scl(1.0 / n, gn);
+}
+
+// one step of solver SOLVER: (x, v) -> (x', v')
+template <int SOLVER, int PARAM>
 __device__ __forceinline__ void advance(const RtDev& g, d3& x, d3& v) {
   const double h = g.step;
-  if (g.solver == VAMX_RT_RK4) {
-    d3 k1x, k1v, k2x, k2v, k3x, k3v, k4x, k4v;
-    rhs(g, x, v, k1x, k1v);
-    rhs(g, add(x, scl(0.5 * h, k1x)), add(v, scl(0.5 * h, k1v)), k2x, k2v);
-    rhs(g, add(x, scl(0.5 * h, k2x)), add(v, scl(0.5 * h, k2v)), k3x, k3v);
-    rhs(g, add(x, scl(h, k3x)), add(v, scl(h, k3v)), k4x, k4v);
-    x = add(x, scl(h / 6.0, add(add(k1x, scl(2.0, k2x)), add(scl(2.0, k3x), k4x))));
-    v = add(v, scl(h / 6.0, add(add(k1v, scl(2.0, k2v)), add(scl(2.0, k3v), k4v))));
+  if (SOLVER == VAMX_RT_RK4) {
+    // x' = x + h/6 ((k1 + 2 k2) + (2 k3 + k4)), likewise v; the two bracketed
+    // sums are formed as soon as their terms exist, so only they stay live
+    d3 kx, kv;
+    rhs<PARAM>(g, x, v, kx, kv);
+    d3 sx = kx, sv = kv;                                   // k1
+    rhs<PARAM>(g, add(x, scl(0.5 * h, kx)), add(v, scl(0.5 * h, kv)), kx, kv);
+    sx = add(sx, scl(2.0, kx));                            // k1 + 2 k2
+    sv = add(sv, scl(2.0, kv));
+    rhs<PARAM>(g, add(x, scl(0.5 * h, kx)), add(v, scl(0.5 * h, kv)), kx, kv);
+    d3 tx = scl(2.0, kx), tv = scl(2.0, kv);               // 2 k3
+    rhs<PARAM>(g, add(x, scl(h, kx)), add(v, scl(h, kv)), kx, kv);
+    tx = add(tx, kx);                                      // 2 k3 + k4
+    tv = add(tv, kv);
+    x = add(x, scl(h / 6.0, add(sx, tx)));
+    v = add(v, scl(h / 6.0, add(sv, tv)));
     return;
   }
   double n;
   d3 gn;
   field_sample(g, x, n, gn);
-  d3 dv;
-  if (g.param == VAMX_RT_CANONICAL) dv = scl(n, gn);
-  else if (g.param == VAMX_RT_PHYSICAL) dv = gn;
-  else dv = scl(1.0 / n, gn);
-  const d3 vnew = add(v, scl(h, dv));
-  const d3 vx = (g.solver == VAMX_RT_FORWARD_EULER) ? v : vnew;  // symplectic uses v_{i+1}
+  const d3 vnew = add(v, scl(h, accel<PARAM>(n, gn)));
+  const d3 vx = (SOLVER == VAMX_RT_FORWARD_EULER) ? v : vnew;  // symplectic uses v_{i+1}
   d3 dx;
-  if (g.param == VAMX_RT_CANONICAL) dx = vx;
-  else if (g.param == VAMX_RT_PHYSICAL) dx = scl(1.0 / n, vx);
+  if (PARAM == VAMX_RT_CANONICAL) dx = vx;
+  else if (PARAM == VAMX_RT_PHYSICAL) dx = scl(1.0 / n, vx);
   else dx = scl(1.0 / (n * n), vx);
   x = add(x, scl(h, dx));
   v = vnew;
 }
 
 // initial state of ray (t, a, zr) (setupRaysParallel, pyTorchRayTrace.py:1638-1666)
+template <int SOLVER, int PARAM>
 __device__ __forceinline__ void ray_init(const RtDev& g, int t, int a, int zr, d3& x, d3& v) {
   const double ct = g.ray_cs[2 * a], st = g.ray_cs[2 * a + 1];
   const double ci = cos(g.incl), si = sin(g.incl);
@@ -147,16 +161,12 @@ __device__ __forceinline__ void ray_init(const RtDev& g, int t, int a, int zr, d
   x.y = D * st * ci + (G0 * ct - G2 * st * si);
   x.z = D * si + G2 * ci;
   v = mk(-ct * ci, -st * ci, -si);
-  if (g.solver == VAMX_RT_LEAPFROG) {
+  if (SOLVER == VAMX_RT_LEAPFROG) {
     // _leapfrog_init: v -= dv(x, v) h / 2 (pyTorchRayTrace.py:817-828)
     double n;
     d3 gn;
     field_sample(g, x, n, gn);
-    d3 dv;
-    if (g.param == VAMX_RT_CANONICAL) dv = scl(n, gn);
-    else if (g.param == VAMX_RT_PHYSICAL) dv = gn;
-    else dv = scl(1.0 / n, gn);
-    v = sub(v, scl(g.step / 2.0, dv));
+    v = sub(v, scl(g.step / 2.0, accel<PARAM>(n, gn)));
   }
 }
 
@@ -178,14 +188,14 @@ struct Taps {
 };
 __device__ __forceinline__ Taps taps_at(const RtDev& g, d3 x) {
   Taps t;
-  const double ix = (x.x - g.xmin) / g.dx, iy = (x.y - g.ymin) / g.dy;
+  const double ix = (x.x - g.xmin) * g.inv_dx, iy = (x.y - g.ymin) * g.inv_dy;
   const double fx = floor(ix), fy = floor(iy);
   t.x0 = (int)fmax(fmin(fx, 2.0e9), -2.0e9);
   t.y0 = (int)fmax(fmin(fy, 2.0e9), -2.0e9);
   t.wx = ix - fx;
   t.wy = iy - fy;
   if (g.nz > 1) {
-    const double iz = (x.z - g.zmin) / g.dz;
+    const double iz = (x.z - g.zmin) * g.inv_dz;
     const double fz = floor(iz);
     t.z0 = (int)fmax(fmin(fz, 2.0e9), -2.0e9);
     t.wz = iz - fz;
@@ -198,7 +208,7 @@ __device__ __forceinline__ Taps taps_at(const RtDev& g, d3 x) {
   return t;
 }
 
-template <bool DEPOSIT>
+template <bool DEPOSIT, int SOLVER, int PARAM>
 __global__ __launch_bounds__(256) void rt_trace_kernel(RtDev g, const float* __restrict__ feff, float* __restrict__ gout,
                                                        const float* __restrict__ gin,
                                                        unsigned long long* __restrict__ acc,
@@ -206,59 +216,74 @@ __global__ __launch_bounds__(256) void rt_trace_kernel(RtDev g, const float* __r
                                                        unsigned long long* __restrict__ stats) {
   const long long nrays = (long long)g.nt * g.na * g.nzr;
   const long long rid = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (rid >= nrays) return;
-  const int zr = (int)(rid % g.nzr);
-  const long long ta = rid / g.nzr;
+  // every lane runs to the end (no early return): the step statistics are
+  // summed across the wavefront before one atomic per wavefront
+  bool live = rid < nrays;
+  const long long r = live ? rid : 0;
+  const int zr = (int)(r % g.nzr);
+  const long long ta = r / g.nzr;
   const int a = (int)(ta % g.na), t = (int)(ta / g.na);
   double E0 = (double)g.energy0;
-  if (DEPOSIT) {
-    E0 = (double)gin[rid];
-    if (E0 == 0.0) return;  // contributes nothing
+  if (DEPOSIT && live) {
+    E0 = (double)gin[r];
+    live = E0 != 0.0;  // a ray of zero energy contributes nothing and is not traced
   }
   const double scale = DEPOSIT ? *fx_scale : 0.0;
   d3 x, v;
-  ray_init(g, t, a, zr, x, v);
+  ray_init<SOLVER, PARAM>(g, t, a, zr, x, v);
   const double hw = g.step / g.dx;  // step_size_idx_unit (pyTorchRayTrace.py:840-842)
   double A = 0.0, integral = 0.0;
   int step = 0;
-  bool active = true;
-  for (; step < g.max_steps; ++step) {
-    const d3 xo = x;
-    advance(g, x, v);
-    const d3 d = sub(x, xo);
-    const double ds = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
-    A += att_alpha(g, xo) * ds;                 // attenuance (767-769)
-    const double E = E0 * exp(-A);              // ray energy (770-774)
-    const Taps tp = taps_at(g, x);
-    if (tp.x0 >= -1 && tp.x0 < g.nx && tp.y0 >= -1 && tp.y0 < g.ny && tp.z0 >= -1 && tp.z0 < g.nz) {
-      const double base = hw * E;
+  bool active = live;
+  if (live) {
+    for (; step < g.max_steps; ++step) {
+      const d3 xo = x;
+      advance<SOLVER, PARAM>(g, x, v);
+      const d3 d = sub(x, xo);
+      const double ds = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+      A += att_alpha(g, xo) * ds;                 // attenuance (767-769)
+      const double E = E0 * exp(-A);              // ray energy (770-774)
+      const Taps tp = taps_at(g, x);
+      if (tp.x0 >= -1 && tp.x0 < g.nx && tp.y0 >= -1 && tp.y0 < g.ny && tp.z0 >= -1 && tp.z0 < g.nz) {
+        const double base = hw * E;
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const int dzc = (c >> 2) & 1;
-        if (dzc >= tp.nzt) continue;
-        const int xi = tp.x0 + (c & 1), yi = tp.y0 + ((c >> 1) & 1), zi = tp.z0 + dzc;
-        if (xi < 0 || yi < 0 || zi < 0 || xi >= g.nx || yi >= g.ny || zi >= g.nz) continue;
-        const double w = ((c & 1) ? tp.wx : 1.0 - tp.wx) * (((c >> 1) & 1) ? tp.wy : 1.0 - tp.wy) *
-                         (tp.nzt == 1 ? 1.0 : (dzc ? tp.wz : 1.0 - tp.wz));
-        if (w == 0.0) continue;
-        const size_t vi = ((size_t)xi * g.ny + yi) * g.nz + zi;
-        if (DEPOSIT) {
-          const long long q = llrint(base * w * scale);
-          atomicAdd(acc + vi, (unsigned long long)q);
-        } else {
-          integral += (double)feff[vi] * (base * w);
+        for (int c = 0; c < 8; ++c) {
+          const int dzc = (c >> 2) & 1;
+          if (dzc >= tp.nzt) continue;
+          const int xi = tp.x0 + (c & 1), yi = tp.y0 + ((c >> 1) & 1), zi = tp.z0 + dzc;
+          if (xi < 0 || yi < 0 || zi < 0 || xi >= g.nx || yi >= g.ny || zi >= g.nz) continue;
+          const double w = ((c & 1) ? tp.wx : 1.0 - tp.wx) * (((c >> 1) & 1) ? tp.wy : 1.0 - tp.wy) *
+                           (tp.nzt == 1 ? 1.0 : (dzc ? tp.wz : 1.0 - tp.wz));
+          if (w == 0.0) continue;
+          const size_t vi = ((size_t)xi * g.ny + yi) * g.nz + zi;
+          if (DEPOSIT) {
+            const long long q = llrint(base * w * scale);
+            atomicAdd(acc + vi, (unsigned long long)q);
+          } else {
+            integral += (double)feff[vi] * (base * w);
+          }
         }
       }
-    }
-    if (step % g.exit_every == 0 && exited(g, x, v)) {
-      active = false;
-      ++step;
-      break;
+      if (step % g.exit_every == 0 && exited(g, x, v)) {
+        active = false;
+        ++step;
+        break;
+      }
     }
   }
-  if (!DEPOSIT) gout[rid] = (float)integral;
-  atomicAdd(stats, (unsigned long long)step);
-  if (active) atomicAdd(stats + 1, 1ull);
+  if (!DEPOSIT && rid < nrays) gout[rid] = (float)integral;
+  // wavefront sums of the step count and of the rays still active at max_steps
+  unsigned long long st = live ? (unsigned long long)step : 0ull;
+  unsigned long long un = active ? 1ull : 0ull;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    st += __shfl_xor(st, off);
+    un += __shfl_xor(un, off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(stats, st);
+    if (un) atomicAdd(stats + 1, un);
+  }
 }
 
 // f_eff = alpha_abs * f  (pyTorchRayTrace.py:515-519)
@@ -268,13 +293,33 @@ __global__ __launch_bounds__(256) void rt_feff_kernel(const float* __restrict__ 
   if (i < nv) out[i] = ab ? f[i] * ab[i] : f[i];
 }
 
-// max |g| for the fixed-point scale; one block, fixed order
-__global__ __launch_bounds__(256) void rt_gmax_kernel(const float* __restrict__ gin, long long n, double na,
-                                                      double* __restrict__ scale_out,
-                                                      unsigned long long* __restrict__ acc_zero_stats) {
+// max |g| for the fixed-point scale, pass 1: block b reduces a contiguous
+// range of g into part[b] (max is order-independent, so the result is exact)
+constexpr int kGmaxParts = 512;
+
+__global__ __launch_bounds__(256) void rt_gmax_part(const float* __restrict__ gin, long long n,
+                                                    float* __restrict__ part) {
+  __shared__ float sh[256];
+  const long long chunk = (n + kGmaxParts - 1) / kGmaxParts;
+  const long long i0 = (long long)blockIdx.x * chunk, i1 = i0 + chunk < n ? i0 + chunk : n;
+  float m = 0.0f;
+  for (long long i = i0 + threadIdx.x; i < i1; i += 256) m = fmaxf(m, fabsf(gin[i]));
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] = fmaxf(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// pass 2 (one block): the scale 2^e and the cleared step statistics
+__global__ __launch_bounds__(256) void rt_gmax_final(const float* __restrict__ part, double na,
+                                                     double* __restrict__ scale_out,
+                                                     unsigned long long* __restrict__ acc_zero_stats) {
   __shared__ float sh[256];
   float m = 0.0f;
-  for (long long i = threadIdx.x; i < n; i += 256) m = fmaxf(m, fabsf(gin[i]));
+  for (int i = threadIdx.x; i < kGmaxParts; i += 256) m = fmaxf(m, part[i]);
   sh[threadIdx.x] = m;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
@@ -315,6 +360,40 @@ __global__ __launch_bounds__(256) void rt_finalize_kernel(const unsigned long lo
   dose[i] = (float)(ab ? v * (double)ab[i] : v);
 }
 
+// compile-time solver and parametrisation: each instantiation keeps only its
+// own stepper's registers live
+template <bool DEPOSIT, int SOLVER>
+int launch_trace_s(const RtDev& g, dim3 grid, hipStream_t s, const float* feff, float* gout, const float* gin,
+                   unsigned long long* acc, const double* scale, unsigned long long* stats) {
+  if (g.param == VAMX_RT_CANONICAL)
+    hipLaunchKernelGGL((rt_trace_kernel<DEPOSIT, SOLVER, VAMX_RT_CANONICAL>), grid, dim3(256), 0, s, g, feff, gout,
+                       gin, acc, scale, stats);
+  else if (g.param == VAMX_RT_PHYSICAL)
+    hipLaunchKernelGGL((rt_trace_kernel<DEPOSIT, SOLVER, VAMX_RT_PHYSICAL>), grid, dim3(256), 0, s, g, feff, gout,
+                       gin, acc, scale, stats);
+  else
+    hipLaunchKernelGGL((rt_trace_kernel<DEPOSIT, SOLVER, VAMX_RT_OPTICAL>), grid, dim3(256), 0, s, g, feff, gout,
+                       gin, acc, scale, stats);
+  return VAMX_OK;
+}
+
+template <bool DEPOSIT>
+int launch_trace(const RtDev& g, dim3 grid, hipStream_t s, const float* feff, float* gout, const float* gin,
+                 unsigned long long* acc, const double* scale, unsigned long long* stats) {
+  switch (g.solver) {
+    case VAMX_RT_SYMPLECTIC_EULER:
+      return launch_trace_s<DEPOSIT, VAMX_RT_SYMPLECTIC_EULER>(g, grid, s, feff, gout, gin, acc, scale, stats);
+    case VAMX_RT_FORWARD_EULER:
+      return launch_trace_s<DEPOSIT, VAMX_RT_FORWARD_EULER>(g, grid, s, feff, gout, gin, acc, scale, stats);
+    case VAMX_RT_LEAPFROG:
+      return launch_trace_s<DEPOSIT, VAMX_RT_LEAPFROG>(g, grid, s, feff, gout, gin, acc, scale, stats);
+    case VAMX_RT_RK4:
+      return launch_trace_s<DEPOSIT, VAMX_RT_RK4>(g, grid, s, feff, gout, gin, acc, scale, stats);
+    default:
+      return set_error(VAMX_ERR_ARG, "ray tracer: unknown solver");
+  }
+}
+
 }  // namespace vamx
 
 using namespace vamx;
@@ -341,6 +420,10 @@ extern "C" int vamx_rt_create(const vamx_rt_geom* geom, vamx_plan** out_plan) {
   RtDev& g = p->rt;
   g.nx = geom->nx; g.ny = geom->ny; g.nz = geom->nz;
   g.dx = geom->dx; g.dy = geom->dy; g.dz = geom->nz > 1 ? geom->dz : 1.0;
+  // grid indices are taken as (x - min) * (1 / d): a multiply instead of a
+  // float64 division per axis and sample (differs from the division by at
+  // most an ulp; trilinear sampling is continuous across voxel faces)
+  g.inv_dx = 1.0 / g.dx; g.inv_dy = 1.0 / g.dy; g.inv_dz = 1.0 / g.dz;
   g.xmin = geom->xmin; g.ymin = geom->ymin; g.zmin = geom->zmin;
   g.xmax = g.xmin + (g.nx - 1) * g.dx;
   g.ymax = g.ymin + (g.ny - 1) * g.dy;
@@ -382,7 +465,8 @@ extern "C" int vamx_rt_create(const vamx_rt_geom* geom, vamx_plan** out_plan) {
   hipError_t e = hipMalloc((void**)&p->d_rt_tables, tab.size() * sizeof(double));
   if (e == hipSuccess) e = hipMemcpy(p->d_rt_tables, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMalloc((void**)&p->d_accum, nv * sizeof(unsigned long long));
-  if (e == hipSuccess) e = hipMalloc((void**)&p->d_gmax, 2 * sizeof(double));
+  // [0]: scale (double); from byte 64: kGmaxParts per-block maxima (float)
+  if (e == hipSuccess) e = hipMalloc((void**)&p->d_gmax, 64 + kGmaxParts * sizeof(float));
   if (e == hipSuccess) e = hipMalloc((void**)&p->d_stats, 2 * sizeof(unsigned long long));
   if (e != hipSuccess) {
     vamx_destroy(p);
@@ -405,9 +489,9 @@ extern "C" int vamx_rt_integrate(vamx_plan* plan, const float* f, float* gout, v
   hipLaunchKernelGGL(rt_feff_kernel, dim3(blocks_for(nv, 256)), dim3(256), 0, s, f, g.abs_grid, feff, nv);
   hipLaunchKernelGGL(rt_zero_stats, dim3(1), dim3(1), 0, s, plan->d_stats);
   const long long nrays = (long long)g.nt * g.na * g.nzr;
-  hipLaunchKernelGGL(rt_trace_kernel<false>, dim3(blocks_for(nrays, 256)), dim3(256), 0, s, g, (const float*)feff,
-                     gout, (const float*)nullptr, (unsigned long long*)nullptr, (const double*)nullptr,
-                     plan->d_stats);
+  const int rc = launch_trace<false>(g, dim3(blocks_for(nrays, 256)), s, feff, gout, nullptr, nullptr, nullptr,
+                                     plan->d_stats);
+  if (rc) return rc;
   VAMX_HIP(hipGetLastError());
   plan->last_stream = s;
   return VAMX_OK;
@@ -421,10 +505,13 @@ extern "C" int vamx_rt_deposit(vamx_plan* plan, const float* gin, float* dose, v
   const long long nv = (long long)g.nx * g.ny * g.nz;
   const long long nrays = (long long)g.nt * g.na * g.nzr;
   double* scale = (double*)plan->d_gmax;
-  hipLaunchKernelGGL(rt_gmax_kernel, dim3(1), dim3(256), 0, s, gin, nrays, (double)g.na, scale, plan->d_stats);
+  float* gmax_part = (float*)((char*)plan->d_gmax + 64);
+  hipLaunchKernelGGL(rt_gmax_part, dim3(kGmaxParts), dim3(256), 0, s, gin, nrays, gmax_part);
+  hipLaunchKernelGGL(rt_gmax_final, dim3(1), dim3(256), 0, s, gmax_part, (double)g.na, scale, plan->d_stats);
   hipLaunchKernelGGL(rt_zero_u64, dim3(blocks_for(nv, 256)), dim3(256), 0, s, plan->d_accum, nv);
-  hipLaunchKernelGGL(rt_trace_kernel<true>, dim3(blocks_for(nrays, 256)), dim3(256), 0, s, g, (const float*)nullptr,
-                     (float*)nullptr, gin, plan->d_accum, (const double*)scale, plan->d_stats);
+  const int rc = launch_trace<true>(g, dim3(blocks_for(nrays, 256)), s, nullptr, nullptr, gin, plan->d_accum, scale,
+                                    plan->d_stats);
+  if (rc) return rc;
   hipLaunchKernelGGL(rt_finalize_kernel, dim3(blocks_for(nv, 256)), dim3(256), 0, s, plan->d_accum, g.abs_grid,
                      (const double*)scale, dose, nv);
   VAMX_HIP(hipGetLastError());

@@ -35,6 +35,7 @@ struct ParDev {
 struct RtDev {
   int nx, ny, nz;
   double xmin, ymin, zmin, dx, dy, dz;
+  double inv_dx, inv_dy, inv_dz;
   double span_x, span_y, span_z;
   double xmax, ymax, zmax;
   int nt, na, nzr;
