@@ -53,6 +53,11 @@ typedef struct vamx_par_geom {
                                   computed by the caller with the reference's
                                   formula; copied into the plan                 */
   int32_t flags;            /* VAMX_PAR_* bits                                   */
+  const uint8_t* occlusion; /* host array [n*n*nz], non-zero where the attenuation
+                               field is > 0 (an insert); NULL = no occlusion.
+                               Binary shadowing of Projector3DParallel.py:72-111,
+                               147-153, 192-199, 209-216; needs incl_deg == 0;
+                               the exact adjoints are then unavailable         */
 } vamx_par_geom;
 
 /* Replaces the construction of Projector3DParallelPython / Projector3DParallelCUDAAstra

@@ -191,12 +191,14 @@ def _z_taps(zpos: np.ndarray, nz: int):
 # --------------------------------------------------------------------------
 def forward(vol: np.ndarray, angles_deg, det_offset: float = 0.0, mu: float = 0.0,
             radius: float | None = None, incl_deg: float = 0.0,
-            clip_output: bool = True) -> np.ndarray:
+            clip_output: bool = True, occ: np.ndarray | None = None) -> np.ndarray:
     """Sinogram (N, n_angles, nz) of volume (N, N, nz).
 
     ``clip_output`` applies skimage's per-slice output clip (the reference
     always does, ``warp(..., clip=True)``); the adjoint is defined for the
-    linear part, ``clip_output=False``.
+    linear part, ``clip_output=False``.  ``occ`` is an occlusion sinogram
+    (``occlusion_sinogram``): samples behind the first occluding row of their
+    column are dropped (Projector3DParallel.py:147-153).
     """
     vol = np.asarray(vol, dtype=np.float64)
     if vol.ndim == 2:
@@ -232,7 +234,50 @@ def forward(vol: np.ndarray, angles_deg, det_offset: float = 0.0, mu: float = 0.
             for zi, w in _z_taps(zpos, nz):
                 rot += w[:, None, :] * np.take_along_axis(
                     plane, np.broadcast_to(zi[:, None, :], plane.shape), axis=2)
+        if occ is not None:
+            if incl_deg != 0.0:
+                raise NotImplementedError("occlusion with inclination")
+            col = occ[:, i, :]                                # (k, nz)
+            # the shadow applies only when the column holds a finite non-zero
+            # first hit (count_nonzero - isnan != 0, Projector3DParallel.py:150)
+            active = (np.count_nonzero(col, axis=0) - np.isnan(col).sum(axis=0)) != 0
+            with np.errstate(invalid="ignore"):
+                shadow = s[:, None, None] > col[None, :, :]   # y > occ (NaN: False)
+            rot = rot * ~(shadow & active[None, None, :])
         out[:, i, :] = (W[..., None] * rot).sum(axis=0)
+    return out
+
+
+# --------------------------------------------------------------------------
+# occlusion sinogram: Projector3DParallel.py:77-111
+# --------------------------------------------------------------------------
+def occlusion_sinogram(insert: np.ndarray, angles_deg, det_offset: float = 0.0) -> np.ndarray:
+    """First occluding row coordinate y = r - N//2 per detector column.
+
+    ``insert`` (N, N, nz) is 1 where the attenuation field is > 0.  Per slice
+    and angle the insert is rotated like the forward projection (bilinear,
+    zero padded, clipped); a sample occludes when its rotated value is > 0,
+    i.e. when a tap of positive weight is an occluding voxel.  Returns
+    (N, n_angles, nz) float64 with NaN where a column meets no occluder.
+    """
+    ins = (np.asarray(insert) > 0).astype(np.float64)
+    if ins.ndim == 2:
+        ins = ins[:, :, None]
+    N, nx, nz = ins.shape
+    angles_deg = np.atleast_1d(np.asarray(angles_deg, dtype=np.float64))
+    c = N // 2
+    kk = np.arange(N, dtype=np.float64) - det_offset
+    rr = np.arange(N, dtype=np.float64)
+    K, RR = np.meshgrid(kk, rr, indexing="xy")
+    smin = ins.min(axis=(0, 1))
+    smax = ins.max(axis=(0, 1))
+    out = np.full((N, angles_deg.size, nz), np.nan)
+    for i, a in enumerate(angles_deg):
+        row, col = sample_coords(warp_matrix(a, c), K, RR)
+        rot = clip_warp_output(smin, smax, bilinear_zero_pad(ins, row, col))   # (r, k, nz)
+        hit = rot > 0
+        first = np.argmax(hit, axis=0)                                          # (k, nz)
+        out[:, i, :] = np.where(hit.any(axis=0), (first - c).astype(np.float64), np.nan)
     return out
 
 
@@ -258,8 +303,13 @@ def _interp_taps(t: np.ndarray, xp0: float, xp_last: float, n: int, c: int, det_
 
 def backward(sino: np.ndarray, angles_deg, det_offset: float = 0.0, mu: float = 0.0,
              radius: float | None = None, incl_deg: float = 0.0,
-             clip: bool = True) -> np.ndarray:
-    """Reconstruction (N, N, nz) of sinogram (N, n_angles, nz)."""
+             clip: bool = True, occ: np.ndarray | None = None) -> np.ndarray:
+    """Reconstruction (N, N, nz) of sinogram (N, n_angles, nz).
+
+    With an occlusion sinogram ``occ`` a voxel gets nothing from an angle when
+    ``s > floor(np.interp(t, xp, occ column, left=nan, right=nan))``
+    (Projector3DParallel.py:192-199, 209-216; NaN compares False).
+    """
     sino = np.asarray(sino, dtype=np.float64)
     if sino.ndim == 2:
         sino = sino[:, :, None]
@@ -294,6 +344,15 @@ def backward(sino: np.ndarray, angles_deg, det_offset: float = 0.0, mu: float = 
             val = acc
         if mu != 0.0:
             val = val * absorption_weight(t, s, mu, R, incl_deg)[..., None]
+        if occ is not None:
+            if incl_deg != 0.0:
+                raise NotImplementedError("occlusion with inclination")
+            xp = (np.arange(N) - c) - det_offset
+            for z in range(nz):
+                oz = np.interp(t, xp, occ[:, i, z], left=np.nan, right=np.nan)
+                with np.errstate(invalid="ignore"):
+                    shadow = s > np.floor(oz)
+                val[:, :, z] = val[:, :, z] * ~shadow
         out += val
     if clip:
         out = clip_to_circle(out)

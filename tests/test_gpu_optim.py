@@ -215,3 +215,26 @@ def test_bclp_fused_wave_path_matches_oracle(gpu, nz):
     print("BCLP wave path rel L2 g %.2e dose %.2e loss %.2e" % (e_g, e_d, e_l), loss_ref[[0, -1]])
     assert loss_ref[-1] < loss_ref[0]
     assert e_g <= 1e-4 and e_d <= 1e-4 and e_l <= 1e-4
+
+
+def test_osmo_with_insert_matches_oracle(gpu):
+    """OSMO through projectorconstructor with an insert (occlusion shadowing)."""
+    import oracle.d1 as d1
+    import oracle.optim as oo
+    import vamtoolbox_amd as vam
+    n, nz, na = 32, 4, 60
+    t = _ellipsoids(n, nz, 6)
+    yy, xx = np.mgrid[:n, :n] - (n - 1) / 2
+    insert = np.repeat(((xx + 9) ** 2 + yy ** 2 <= 9)[:, :, None], nz, axis=2).astype(np.uint8)
+    t = t * (1 - insert)
+    tg = vam.geometry.TargetGeometry(target=t, insert=insert)
+    ang = np.linspace(0, 360, na, endpoint=False)
+    pg = vam.geometry.ProjectionGeometry(ang, "parallel", CUDA=True)
+    opts = vam.optimize.Options(method="OSMO", n_iter=6, d_h=0.85, d_l=0.6, filter="ram-lak")
+    sino, recon, err = vam.optimize.optimize(tg, pg, opts)
+    occ = d1.occlusion_sinogram(insert, ang)
+    b_ref, x_ref, err_ref = oo.osmo(tg.array, lambda v: d1.forward(v, ang, occ=occ),
+                                    lambda s: d1.backward(s, ang, occ=occ), 6, d_h=0.85, d_l=0.6)
+    e_b, e_x = rel_l2(sino.array, b_ref), rel_l2(recon.array, x_ref)
+    print("OSMO with insert: rel L2 sinogram %.2e recon %.2e" % (e_b, e_x))
+    assert e_b <= 1e-4 and e_x <= 1e-4

@@ -197,3 +197,58 @@ def test_tuning_settings_agree(gpu):
     rep = plan.autotune(tv, ts, reps=1)
     assert set(rep["choice"]) == {"fp_tile_angles", "fp_tile_rays", "fp_zv", "fp_reuse", "bp_tile", "bp_zv"}
     assert plan.get_tuning() == rep["choice"]
+
+
+# --------------------------------------------------------------------------
+# occlusion shadowing (Projector3DParallel.py:72-111, 147-153, 192-216)
+# --------------------------------------------------------------------------
+def test_occlusion_golden_vectors(gpu):
+    """The reference's occlusion path, from scikit-image / numpy golden vectors."""
+    import os
+    import torch
+    from vamtoolbox_amd.parallel import ParallelPlan
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "d1_occlusion_vectors.npz"))
+    keys = sorted({k.rsplit("_", 1)[0] for k in g.files})
+    for key in keys:
+        ins, ang = g[key + "_insert"], g[key + "_ang"]
+        n, _, nz = ins.shape
+        print("case", key, flush=True)
+        plan = ParallelPlan(n, nz, ang, occlusion=ins, device=gpu)
+        f = plan.forward(torch.tensor(g[key + "_vol"], dtype=torch.float32, device=gpu)).cpu().numpy()
+        b = plan.backward(torch.tensor(g[key + "_sino"], dtype=torch.float32, device=gpu)).cpu().numpy()
+        e1, e2 = rel_l2(f, g[key + "_fwd"]), rel_l2(b, g[key + "_bwd"])
+        print("  rel L2 forward %.2e backward %.2e" % (e1, e2))
+        assert e1 <= TOL and e2 <= TOL, key
+
+
+OCC_CASES = [
+    (40, 5, np.arange(0, 360, 9.0), dict(mu=0.02, det_offset=1.5)),
+    (33, 64, np.arange(0, 180, 7.0), {}),                              # wave-eligible shape, occlusion -> per lane
+    (24, 1, np.arange(0, 360, 5.0), dict(det_offset=-2.0)),
+]
+
+
+@pytest.mark.parametrize("case", range(len(OCC_CASES)))
+def test_occlusion_parity(gpu, case):
+    import torch
+    import oracle.d1 as d1
+    from vamtoolbox_amd.parallel import ParallelPlan
+    n, nz, ang, kw = OCC_CASES[case]
+    print("case", case, n, nz, len(ang), kw, flush=True)
+    rng = np.random.default_rng(400 + case)
+    yy, xx = np.mgrid[:n, :n] - (n - 1) / 2
+    rod = (xx - 0.2 * n) ** 2 + (yy - 0.05 * n) ** 2 <= (0.1 * n) ** 2
+    ins = np.repeat(rod[:, :, None], nz, axis=2) & (rng.uniform(size=(1, 1, nz)) > 0.2)
+    vol = rng.uniform(0.1, 1.0, (n, n, nz))
+    sino = rng.uniform(0, 1, (n, len(ang), nz))
+    plan = ParallelPlan(n, nz, ang, occlusion=ins, device=gpu, **kw)
+    occ = d1.occlusion_sinogram(ins, ang, det_offset=kw.get("det_offset", 0.0))
+    f = plan.forward(torch.tensor(vol, dtype=torch.float32, device=gpu)).cpu().numpy()
+    b = plan.backward(torch.tensor(sino, dtype=torch.float32, device=gpu)).cpu().numpy()
+    e1 = rel_l2(f, d1.forward(vol, ang, occ=occ, **kw))
+    e2 = rel_l2(b, d1.backward(sino, ang, occ=occ, **kw))
+    print("rel L2 forward %.2e backward %.2e" % (e1, e2))
+    assert e1 <= TOL and e2 <= TOL
+    # the shadow removes something: differs from the unoccluded projection
+    free = ParallelPlan(n, nz, ang, device=gpu, **kw)
+    assert rel_l2(f, free.forward(torch.tensor(vol, dtype=torch.float32, device=gpu)).cpu().numpy()) > 1e-3

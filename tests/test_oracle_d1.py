@@ -182,3 +182,58 @@ def test_clip_to_circle_counts():
     # symmetric and strictly inside-out
     assert np.array_equal(m, m.T) and np.array_equal(m, m[::-1]) and m[0, 0] and not m[128, 128]
     assert m.sum() == 256 * 256 - d1.inside_circle_mask(256, 256).sum()
+
+
+# --------------------------------------------------------------------------
+# occlusion shadowing (Projector3DParallel.py:77-111, 147-153, 192-199, 209-216)
+# --------------------------------------------------------------------------
+def _occ_cases():
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "d1_occlusion_vectors.npz"))
+    keys = sorted({k.rsplit("_", 1)[0] for k in g.files})
+    return g, keys
+
+
+def test_occlusion_sinogram_matches_golden():
+    g, keys = _occ_cases()
+    assert len(keys) == 8
+    for k in keys:
+        occ = d1.occlusion_sinogram(g[k + "_insert"], g[k + "_ang"])
+        ref = g[k + "_occ"]
+        assert np.array_equal(np.isnan(occ), np.isnan(ref)), k
+        assert np.array_equal(occ[~np.isnan(occ)], ref[~np.isnan(ref)]), k
+
+
+def test_occluded_forward_backward_match_golden():
+    g, keys = _occ_cases()
+    for k in keys:
+        ang, occ = g[k + "_ang"], g[k + "_occ"]
+        f = d1.forward(g[k + "_vol"], ang, occ=occ)
+        b = d1.backward(g[k + "_sino"], ang, occ=occ)
+        assert np.max(np.abs(f - g[k + "_fwd"])) <= 1e-12 * max(1.0, np.abs(g[k + "_fwd"]).max()), k
+        assert np.max(np.abs(b - g[k + "_bwd"])) <= 1e-12 * max(1.0, np.abs(g[k + "_bwd"]).max()), k
+
+
+def test_occlusion_without_occluder_is_neutral():
+    rng = np.random.default_rng(5)
+    n, nz = 13, 2
+    ang = np.arange(0.0, 360.0, 17.0)
+    occ = d1.occlusion_sinogram(np.zeros((n, n, nz)), ang)
+    assert np.isnan(occ).all()
+    vol, sino = rng.uniform(0.1, 1, (n, n, nz)), rng.normal(size=(n, ang.size, nz))
+    assert np.array_equal(d1.forward(vol, ang, occ=occ), d1.forward(vol, ang))
+    assert np.array_equal(d1.backward(sino, ang, occ=occ), d1.backward(sino, ang))
+
+
+def test_occlusion_first_hit_brute_force():
+    """At 0 degrees the rotation is the identity: the first hit of column k is
+    the first occluding row r (minus N//2), found here by a plain scan."""
+    rng = np.random.default_rng(8)
+    n = 11
+    ins = (rng.uniform(size=(n, n, 1)) > 0.85).astype(np.uint8)
+    occ = d1.occlusion_sinogram(ins, [0.0])[:, 0, 0]
+    for k in range(n):
+        rows = [r for r in range(n) if ins[r, k, 0]]
+        if rows:
+            assert occ[k] == rows[0] - n // 2
+        else:
+            assert np.isnan(occ[k])

@@ -33,6 +33,7 @@ class ParGeom(ctypes.Structure):
         ("radius", c_double),
         ("outside_mask", ctypes.POINTER(ctypes.c_uint8)),
         ("flags", c_int32),
+        ("occlusion", ctypes.POINTER(ctypes.c_uint8)),
     ]
 
 

@@ -124,9 +124,13 @@ def _plan_for(target_geo, proj_geo, device):
             (proj_geo.absorption_coeff is not None and getattr(proj_geo, "absorption_mask", None) is not None):
         raise NotImplementedError("batched OSMO: zero-dose constraints and absorption masks are per-problem "
                                   "options of minimizeOSMO; use absorption_per_voxel for absorption")
+    occ = None
+    if getattr(proj_geo, "attenuation_field", None) is not None:
+        af = np.asarray(proj_geo.attenuation_field)
+        occ = np.broadcast_to(af.reshape(af.shape[0], af.shape[1], -1), (n, n, nz)) > 0
     return ParallelPlan(n, nz, np.asarray(proj_geo.angles, dtype=np.float64), det_offset=proj_geo.det_offset,
                         incl_deg=float(proj_geo.inclination_angle or 0.0), mu=proj_geo.absorption_per_voxel,
-                        radius=proj_geo.absorption_radius, device=device)
+                        radius=proj_geo.absorption_radius, occlusion=occ, device=device)
 
 
 class OSMOBatch:

@@ -338,7 +338,7 @@ int launch_epilogue(vamx_plan* p, const RespDev& r, const BclpDev& b, const floa
                                                  (uintptr_t)operand) % 16 == 0 &&
                   (!b.eps_arr || (uintptr_t)b.eps_arr % 16 == 0) && (!b.w_arr || (uintptr_t)b.w_arr % 16 == 0);
   int wz = 0;
-  if (!(p->flags & VAMX_PAR_PER_LANE_GEOMETRY) && !gd.incl && gd.n >= 2) {
+  if (!(p->flags & VAMX_PAR_PER_LANE_GEOMETRY) && !gd.incl && gd.n >= 2 && !gd.occ) {
     const uintptr_t al = (uintptr_t)g | (uintptr_t)dose | (uintptr_t)target | (uintptr_t)operand |
                          (uintptr_t)b.eps_arr | (uintptr_t)b.w_arr;
     wz = wave_zv_choice(gd.nz, al, gd.bp_zv);

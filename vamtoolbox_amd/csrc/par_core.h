@@ -138,6 +138,17 @@ __device__ __forceinline__ typename VecT<ZV>::T fp_ray(const ParDev& g, const fl
   }
   V acc = vzero<ZV>();
   const size_t zstride = (size_t)g.nz;
+  // occlusion: slice z+q drops the samples with y = r - c > lim[q]
+  // (Projector3DParallel.py:147-153); lim = +inf where no shadow applies
+  float lim[ZV];
+#pragma unroll
+  for (int q = 0; q < ZV; ++q) {
+    lim[q] = INFINITY;
+    if (g.occ && g.occ_active[(size_t)i * g.nz + z + q]) {
+      const float o = g.occ[((size_t)k * g.na + i) * zstride + z + q];
+      if (!isnan(o)) lim[q] = o;
+    }
+  }
   for (int r = 0; r < n; ++r) {
     const double rr = (double)r;
     const double col = (a0 + M01 * rr) + M02;
@@ -164,6 +175,12 @@ __device__ __forceinline__ typename VecT<ZV>::T fp_ray(const ParDev& g, const fl
       const bool colw = vc0 || (dc > 0.0 && vc1);
       const bool anyin = (vr0 && colw) || (dr > 0.0 && vr1 && colw);
       if (anyin) v = vclamp(v, lo, hi);
+    }
+    if (g.occ) {
+      const float yr = (float)(r - g.c);
+#pragma unroll
+      for (int q = 0; q < ZV; ++q)
+        if (yr > lim[q]) vset(v, q, 0.0f);
     }
     acc = vfma(abs_weight(g, t, (double)(r - g.c)), v, acc);
   }
@@ -217,6 +234,44 @@ __device__ __forceinline__ float fp_ray_incl(const ParDev& g, const float* __res
 // back projection at voxel column pix, z .. z+ZV-1 (no circle clip here)
 // (Projector3DParallel.py:175-205)
 // ---------------------------------------------------------------------------
+// Occlusion of a voxel at detector coordinate t, along-ray coordinate s,
+// for angle i and slices z..z+ZV-1: the voxel gets nothing from this angle
+// when s > floor(np.interp(t, xp, occ column, left=nan, right=nan))
+// (Projector3DParallel.py:192-199, 209-216).  np.interp returns fp[j]
+// exactly when t equals a node xp[j] (also the last one) and otherwise
+// evaluates slope * (t - xp[j]) + fp[j] on the interval containing t (NaN if
+// a node is NaN); NaN compares false.  t is inside [xp[0], xp[n-1]] here
+// (interp_node accepted it).
+template <int ZV>
+__device__ __forceinline__ void occ_shadow_mask(const ParDev& g, int i, int j, double t, double s, int z,
+                                                typename VecT<ZV>::T& val) {
+  const size_t zs = (size_t)g.nz;
+  const float* oc = g.occ + (size_t)i * zs + z;
+  const size_t step_j = (size_t)g.na * zs;
+  // numpy's interval: xp[jn] <= t < xp[jn + 1], found by comparing t with
+  // the node values (floor(t - xp[0]) can be off by one when t is within an
+  // ulp of a node)
+  int jn = j;
+  if (jn > 0 && t < (double)(jn - g.c) - g.det_offset) {
+    --jn;
+  } else if (jn < g.n - 1 && t >= (double)(jn + 1 - g.c) - g.det_offset) {
+    ++jn;
+  }
+#pragma unroll
+  for (int q = 0; q < ZV; ++q) {
+    const double x0 = (double)(jn - g.c) - g.det_offset;
+    const double o0 = (double)oc[(size_t)jn * step_j + q];
+    double o = o0;
+    if (jn < g.n - 1 && t != x0) {
+      const double o1 = (double)oc[(size_t)(jn + 1) * step_j + q];
+      const double x1 = (double)(jn + 1 - g.c) - g.det_offset;
+      const double slope = (o1 - o0) / (x1 - x0);
+      o = slope * (t - x0) + o0;
+    }
+    if (s > floor(o)) vset(val, q, 0.0f);
+  }
+}
+
 template <int ZV>
 __device__ __forceinline__ typename VecT<ZV>::T bp_voxel(const ParDev& g, const float* __restrict__ sino, int pix,
                                                           int z) {
@@ -235,6 +290,7 @@ __device__ __forceinline__ typename VecT<ZV>::T bp_voxel(const ParDev& g, const 
     const V b1 = vload<ZV>(sino + ((size_t)j1 * g.na + i) * zstride + z);
     V val = vfma(w, vsub(b1, b0), b0);
     if (g.mu != 0.0) val = vmul(abs_weight(g, t, x * sn + y * co), val);
+    if (g.occ) occ_shadow_mask<ZV>(g, i, j, t, x * sn + y * co, z, val);
     acc = vadd(acc, val);
   }
   return acc;

@@ -79,6 +79,65 @@ __global__ __launch_bounds__(1024) void slice_minmax_final(const float* __restri
   }
 }
 
+// Occlusion sinogram (Projector3DParallel.py:77-111): for detector bin k,
+// angle i and slice z, the row coordinate y = r - c of the first sample along
+// the ray whose bilinear rotation of the insert is positive, i.e. that has a
+// tap of positive weight on an occluding voxel (the warp clip keeps positive
+// values positive); NaN when the ray meets no occluder.  One thread per
+// (k, i, z), z fastest; runs once at plan creation.
+__global__ __launch_bounds__(256) void occ_sino_kernel(ParDev g, const uint8_t* __restrict__ ins,
+                                                       float* __restrict__ occ) {
+  const long long total = (long long)g.n * g.na * g.nz;
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= total) return;
+  const int z = (int)(tid % g.nz);
+  const long long ki = tid / g.nz;
+  const int i = (int)(ki % g.na), k = (int)(ki / g.na);
+  const int n = g.n;
+  const double* M = g.mats + 6 * i;
+  const double kp = (double)k - g.det_offset;
+  const double a0 = M[0] * kp, a1 = M[3] * kp;
+  const size_t zs = (size_t)g.nz;
+  float first = NAN;
+  for (int r = 0; r < n; ++r) {
+    const double rr = (double)r;
+    const double col = (a0 + M[1] * rr) + M[2];
+    const double row = (a1 + M[4] * rr) + M[5];
+    if (!(row > -1.0 && row < (double)n && col > -1.0 && col < (double)n)) continue;
+    const double fr = floor(row), fc = floor(col);
+    const double dr = row - fr, dc = col - fc;
+    const int r0 = (int)fr, c0 = (int)fc, r1 = (int)ceil(row), c1 = (int)ceil(col);
+    const bool vr0 = r0 >= 0, vr1 = r1 < n, vc0 = c0 >= 0, vc1 = c1 < n;
+    // tap weights: (1-dr)(1-dc) > 0 always; the others need dc > 0 / dr > 0
+    const bool hit = (vr0 && vc0 && ins[((size_t)r0 * n + c0) * zs + z]) ||
+                     (dc > 0.0 && vr0 && vc1 && ins[((size_t)r0 * n + c1) * zs + z]) ||
+                     (dr > 0.0 && vr1 && vc0 && ins[((size_t)r1 * n + c0) * zs + z]) ||
+                     (dr > 0.0 && dc > 0.0 && vr1 && vc1 && ins[((size_t)r1 * n + c1) * zs + z]);
+    if (hit) {
+      first = (float)(r - g.c);
+      break;
+    }
+  }
+  occ[((size_t)k * g.na + i) * zs + z] = first;
+}
+
+// whether the forward projection applies the shadow of angle i, slice z:
+// the occlusion column holds a finite non-zero entry
+// (count_nonzero - isnan != 0, Projector3DParallel.py:150)
+__global__ __launch_bounds__(256) void occ_active_kernel(ParDev g, const float* __restrict__ occ,
+                                                         uint8_t* __restrict__ active) {
+  const long long total = (long long)g.na * g.nz;
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= total) return;
+  const int z = (int)(tid % g.nz), i = (int)(tid / g.nz);
+  uint8_t a = 0;
+  for (int k = 0; k < g.n && !a; ++k) {
+    const float o = occ[((size_t)k * g.na + i) * g.nz + z];
+    if (!isnan(o) && o != 0.0f) a = 1;
+  }
+  active[(size_t)i * g.nz + z] = a;
+}
+
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
@@ -230,7 +289,7 @@ int vamx::par_wave_zv(const vamx_plan* p, const void* a, const void* b, int requ
   // zv, buffers aligned to it, and the best lane utilisation
   // nz / (chunks * 64 * zv) (ties -> the wider vector, fewer waves); 0 (use
   // the per-lane kernels) when even the best leaves half the lanes idle
-  if ((p->flags & VAMX_PAR_PER_LANE_GEOMETRY) || p->par.incl || p->par.n < 2) return 0;
+  if ((p->flags & VAMX_PAR_PER_LANE_GEOMETRY) || p->par.incl || p->par.n < 2 || p->par.occ) return 0;
   return wave_zv_choice(p->par.nz, (uintptr_t)a | (uintptr_t)b, requested);
 }
 
@@ -262,6 +321,8 @@ extern "C" int vamx_par_create(const vamx_par_geom* geom, vamx_plan** out_plan) 
   if (!(std::fabs(geom->incl_deg) < 89.0))
     return set_error(VAMX_ERR_ARG, "vamx_par_create: |inclination| must be < 89 degrees");
   if (geom->mu < 0.0) return set_error(VAMX_ERR_ARG, "vamx_par_create: mu must be >= 0");
+  if (geom->occlusion && geom->incl_deg != 0.0)
+    return set_error(VAMX_ERR_ARG, "vamx_par_create: occlusion shadowing needs zero inclination");
   vamx_plan* p = new vamx_plan();
   p->kind = PLAN_PARALLEL;
   if (hipGetDevice(&p->device) != hipSuccess) {
@@ -324,6 +385,33 @@ extern "C" int vamx_par_create(const vamx_par_geom* geom, vamx_plan** out_plan) 
   g.mats = p->d_mats;
   g.cs = p->d_cs;
   g.outside = p->d_outside;
+  g.occ = nullptr;
+  g.occ_active = nullptr;
+  if (geom->occlusion) {
+    // occlusion sinogram of the insert, computed once like the reference's
+    // constructor does (Projector3DParallel.py:72-75)
+    const size_t nvox = (size_t)n * n * nz, nsino = (size_t)n * na * nz;
+    uint8_t* d_ins = nullptr;
+    e = hipMalloc((void**)&d_ins, nvox);
+    if (e == hipSuccess) e = hipMalloc((void**)&p->d_occ, nsino * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&p->d_occ_active, (size_t)na * nz);
+    if (e == hipSuccess) e = hipMemcpy(d_ins, geom->occlusion, nvox, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(occ_sino_kernel, dim3(blocks_for((long long)nsino, 256)), dim3(256), 0, 0, g, d_ins,
+                         p->d_occ);
+      hipLaunchKernelGGL(occ_active_kernel, dim3(blocks_for((long long)na * nz, 256)), dim3(256), 0, 0, g,
+                         p->d_occ, p->d_occ_active);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (d_ins) hipFree(d_ins);
+    if (e != hipSuccess) {
+      vamx_destroy(p);
+      return hip_check(e, "vamx_par_create: occlusion sinogram");
+    }
+    g.occ = p->d_occ;
+    g.occ_active = p->d_occ_active;
+  }
   *out_plan = p;
   return VAMX_OK;
 }
@@ -448,6 +536,7 @@ extern "C" int vamx_par_forward_adjoint(vamx_plan* plan, const float* sino, floa
   if (!check_plan(plan, PLAN_PARALLEL))
     return set_error(VAMX_ERR_STATE, "vamx_par_forward_adjoint: not a parallel plan");
   if (!vol || !sino) return set_error(VAMX_ERR_ARG, "vamx_par_forward_adjoint: null buffer");
+  if (plan->par.occ) return set_error(VAMX_ERR_STATE, "vamx_par_forward_adjoint: not available with occlusion");
   const ParDev& g = plan->par;
   hipStream_t s = (hipStream_t)stream;
   if (g.incl) {
@@ -468,6 +557,7 @@ extern "C" int vamx_par_backward_adjoint(vamx_plan* plan, const float* vol, floa
   if (!check_plan(plan, PLAN_PARALLEL))
     return set_error(VAMX_ERR_STATE, "vamx_par_backward_adjoint: not a parallel plan");
   if (!vol || !sino) return set_error(VAMX_ERR_ARG, "vamx_par_backward_adjoint: null buffer");
+  if (plan->par.occ) return set_error(VAMX_ERR_STATE, "vamx_par_backward_adjoint: not available with occlusion");
   const ParDev& g = plan->par;
   hipStream_t s = (hipStream_t)stream;
   const int clipc = (plan->flags & VAMX_PAR_NO_CIRCLE_CLIP) ? 0 : 1;

@@ -25,6 +25,11 @@ struct ParDev {
   const double* mats;        // [na][6]: R00 R01 R02 R10 R11 R12 (Projector3DParallel.py:137-143)
   const double* cs;          // [na][2]: cos, sin of deg2rad(angle)
   const uint8_t* outside;    // [n*n] clipToCircle mask
+  // occlusion shadowing (null = none): first occluding row coordinate per
+  // (detector bin, angle, slice), NaN where none, and per (angle, slice)
+  // whether the forward projection applies it
+  const float* occ;          // [n][na][nz]
+  const uint8_t* occ_active; // [na][nz]
   // work order of the wavefront-uniform kernels (set by vamx_par_set_tuning)
   int fp_ta, fp_tk, fp_zv;   // forward: tile of fp_ta angles x fp_tk rays, z values per lane
   int fp_reuse;              // forward: keep the previous sample's taps in registers
@@ -67,6 +72,8 @@ struct vamx_plan {
   uint8_t* d_outside = nullptr;
   float* d_minmax_part = nullptr;   // [parts][2*nz]
   float* d_minmax = nullptr;        // [2*nz]
+  float* d_occ = nullptr;           // [n*na*nz] occlusion sinogram
+  uint8_t* d_occ_active = nullptr;  // [na*nz]
   int minmax_parts = 0;
   std::vector<double> h_angles;
   // ray tracer

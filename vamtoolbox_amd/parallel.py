@@ -32,11 +32,14 @@ class ParallelPlan:
     clip switches (both on = reference behaviour).  ``per_lane_geometry``
     selects the kernels that evaluate the geometry in every lane instead of
     the wavefront-uniform ones (same results to float32 rounding).
+    ``occlusion`` (n, n, nz) or (n, n, 1), non-zero where the attenuation field
+    is > 0, turns on the reference's binary occlusion shadowing
+    (Projector3DParallel.py:72-111, 147-153, 192-216).
     """
 
     def __init__(self, n: int, nz: int, angles_deg, det_offset: float = 0.0, incl_deg: float = 0.0,
                  mu: float = 0.0, radius: float | None = None, warp_clip: bool = True,
-                 circle_clip: bool = True, per_lane_geometry: bool = False, device=None):
+                 circle_clip: bool = True, per_lane_geometry: bool = False, occlusion=None, device=None):
         self.lib = _lib.load()
         self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
         self.n, self.nz = int(n), int(nz)
@@ -47,11 +50,19 @@ class ParallelPlan:
         mask = np.ascontiguousarray(outside_circle_mask(self.n).astype(np.uint8))
         flags = (0 if warp_clip else _lib.PAR_NO_WARP_CLIP) | (0 if circle_clip else _lib.PAR_NO_CIRCLE_CLIP) | \
             (_lib.PAR_PER_LANE_GEOMETRY if per_lane_geometry else 0)
+        occ = None
+        if occlusion is not None:
+            occ = np.ascontiguousarray(np.broadcast_to(np.asarray(occlusion).reshape(
+                (self.n, self.n, -1)), (self.n, self.n, self.nz)) > 0, dtype=np.uint8)
+            if not occ.any():
+                occ = None            # an empty insert shadows nothing
+        self.occluded = occ is not None
         geom = _lib.ParGeom(
             n=self.n, nz=self.nz, n_angles=self.na,
             angles_deg=self.angles_deg.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
             det_offset=self.det_offset, incl_deg=self.incl_deg, mu=self.mu, radius=self.radius,
-            outside_mask=mask.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), flags=flags)
+            outside_mask=mask.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), flags=flags,
+            occlusion=None if occ is None else occ.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
         handle = ctypes.c_void_p()
         with torch.cuda.device(self.device):
             _lib.check(self.lib.vamx_par_create(ctypes.byref(geom), ctypes.byref(handle)), "vamx_par_create")

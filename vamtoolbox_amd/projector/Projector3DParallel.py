@@ -8,8 +8,10 @@ sinogram (nR, nTheta, nZ)).  NumPy in -> NumPy out (float64, as the
 reference returns); torch GPU tensor in -> float32 GPU tensor out (no host
 copies).  The discretisation is the reference's Python projector (bilinear
 rotate-and-sum forward, np.interp back projection, circle clip), evaluated
-by the HIP kernels of libvamx_hip.so.  Additionally ``forward_adjoint`` and
-``backward_adjoint`` are the exact transposes used for autograd.
+by the HIP kernels of libvamx_hip.so, with the reference's binary occlusion
+shadowing when ``proj_geo.attenuation_field`` has positive entries.
+Additionally ``forward_adjoint`` and ``backward_adjoint`` are the exact
+transposes used for autograd (not available with occlusion).
 """
 
 from __future__ import annotations
@@ -33,8 +35,16 @@ class Projector3DParallelHIP:
         self.angles = np.asarray(proj_geo.angles, dtype=np.float64)
         incl = proj_geo.inclination_angle or 0.0
         radius = proj_geo.absorption_radius
+        # occlusion shadowing by the voxels of positive attenuation
+        # (Projector3DParallel.py:72-75)
+        occ = None
+        if getattr(proj_geo, "attenuation_field", None) is not None:
+            af = np.asarray(proj_geo.attenuation_field)
+            occ = np.broadcast_to(af.reshape(af.shape[0], af.shape[1], -1), (nY, nX, nZ)) > 0
         self.plan = ParallelPlan(nY, nZ, self.angles, det_offset=proj_geo.det_offset, incl_deg=float(incl),
-                                 mu=proj_geo.absorption_per_voxel, radius=radius, device=self.device)
+                                 mu=proj_geo.absorption_per_voxel, radius=radius, occlusion=occ,
+                                 device=self.device)
+        self.occluded = self.plan.occluded
         self.vol_shape = (nY, nX, nZ)
         self.sino_shape = (nY, self.angles.size, nZ)
         self._mask_t = None

@@ -13,10 +13,10 @@ runs on the GPU through libvamx_hip.so:
   propagation is not part of this package)
 
 As in the reference, an insert turns into infinite attenuation (62-70) and a
-zero-dose body into ``proj_geo.zero_dose_sino`` (162-163).  A non-zero
-``attenuation_field`` (occlusion shadowing, Projector3DParallel.py:77-111)
-raises NotImplementedError; an all-zero one is numerically neutral and
-accepted.
+zero-dose body into ``proj_geo.zero_dose_sino`` (162-163).  With an
+``attenuation_field`` the parallel projector applies the reference's binary
+occlusion shadowing (voxels with attenuation > 0 occlude,
+Projector3DParallel.py:72-111, 147-153, 192-216).
 """
 
 from __future__ import annotations
@@ -58,8 +58,6 @@ def projectorconstructor(target_geo: geometry.TargetGeometry, proj_geo: geometry
         from .projector.pyTorchRayTrace import PyTorchRayTracingPropagator
         A = PyTorchRayTracingPropagator(target_geo, proj_geo)
     elif proj_geo.ray_type == "parallel":
-        if proj_geo.attenuation_field is not None and np.any(np.asarray(proj_geo.attenuation_field) > 0):
-            raise NotImplementedError("occlusion shadowing (attenuation_field > 0) is not implemented")
         from .projector.Projector3DParallel import Projector3DParallelHIP
         A = Projector3DParallelHIP(target_geo, proj_geo)
     else:
