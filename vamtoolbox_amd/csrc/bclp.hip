@@ -151,7 +151,7 @@ __global__ __launch_bounds__(256) void bclp_bp_epilogue(ParDev g, int clipc, Res
 // A (wavefront-uniform variant): one wavefront = one 2x2 quad of voxel
 // columns x 64*ZV z
 template <int ZV>
-__global__ __launch_bounds__(256) void bclp_bp_epilogue_wave(ParDev g, int clipc, RespDev r, BclpDev b,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void bclp_bp_epilogue_wave(ParDev g, int clipc, RespDev r, BclpDev b,
                                                              const float* __restrict__ gsino,
                                                              const float* __restrict__ target,
                                                              float* __restrict__ dose, float* __restrict__ operand,

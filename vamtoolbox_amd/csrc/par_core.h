@@ -630,6 +630,26 @@ __device__ __forceinline__ typename VecT<ZV>::T fp_ray_wave(const ParDev& g, con
           t11 = t01;
           t00 = vload<ZV>(p);
           t01 = vload<ZV>(p + step_c);
+        } else if (d == n + 1) {            // diagonal steps keep one tap
+          t00 = t11;
+          t01 = vload<ZV>(p + step_c);
+          t10 = vload<ZV>(p + step_r);
+          t11 = vload<ZV>(p + step_r + step_c);
+        } else if (d == n - 1) {
+          t01 = t10;
+          t00 = vload<ZV>(p);
+          t10 = vload<ZV>(p + step_r);
+          t11 = vload<ZV>(p + step_r + step_c);
+        } else if (d == -n + 1) {
+          t10 = t01;
+          t00 = vload<ZV>(p);
+          t01 = vload<ZV>(p + step_c);
+          t11 = vload<ZV>(p + step_r + step_c);
+        } else if (d == -n - 1) {
+          t11 = t00;
+          t00 = vload<ZV>(p);
+          t01 = vload<ZV>(p + step_c);
+          t10 = vload<ZV>(p + step_r);
         } else if (d != 0) {
           t00 = vload<ZV>(p);
           t01 = vload<ZV>(p + step_c);
