@@ -9,6 +9,8 @@ TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
 * ``cal``                                   vamtoolbox/optimizer/CAL.py:10-218 (d_h given)
 * ``bclp``                                  vamtoolbox/optimizer/BCLP.py:138-641
 * ``ver``                                   vamtoolbox/metrics.py:47-87
+* ``fbp``                                   vamtoolbox/optimizer/FBP.py:9-76
+* ``pm_loss_grad`` / ``pm``                 vamtoolbox/optimizer/PM.py:25-229
 
 The projector is passed in as a pair of callables (forward, backward) so the
 loops run on the D1 oracle or on an explicit matrix.
@@ -217,3 +219,53 @@ def bclp_init(target, fwd, resp, n_angles, filt="ram-lak", glb=0.0, gub=None):
     g0 = fwd(resp.map_inv(target))
     g0 = filter_sinogram(g0, filt) * (np.pi / (2 * n_angles))
     return np.clip(g0, glb, gub)
+
+
+def fbp(target, fwd, bwd, filt="ram-lak", offset=False):
+    """FBP.py:42-63: filtered forward projection, non-negative, normalised; dose normalised."""
+    gel, void = circle_inds(target)
+    b = filter_sinogram(fwd(target), filt)
+    if offset:
+        b = b + np.broadcast_to(np.abs(np.min(b, axis=0)), b.shape)
+    else:
+        b = np.clip(b, 0, None)
+    b = b / np.max(b)
+    x = bwd(b)
+    x = x / np.max(x)
+    return b, x, ver(gel, void, x)
+
+
+def pm_loss_grad(b, target, bwd, fwd, d_h=0.8, d_l=0.7, rho_1=1.0, rho_2=1.0, p=1.0):
+    """Loss and gradient of PM.py:76-140 at sinogram b (gradient through the
+    forward projector, both scaled by pi/180)."""
+    gel, void = circle_inds(target)
+    core = np.zeros(target.shape, dtype=bool)
+    core[gel] = True
+    background = np.zeros(target.shape, dtype=bool)
+    background[void] = True
+    dose = bwd(b) * np.pi / 180
+    V1 = core & ~(dose >= d_h)
+    V2 = background & ~(dose <= d_l)
+    loss = float(np.sum(rho_1 * np.power(dose - d_l, p) * V2 + rho_2 * np.power(d_h - dose, p) * V1))
+    gd = rho_1 * V2 * p * np.power(dose - d_l, p - 1) - rho_2 * V1 * p * np.power(d_h - dose, p - 1)
+    return loss, fwd(gd) * (np.pi / 180), dose
+
+
+def pm(target, fwd, bwd, n_iter, filt="ram-lak", **kw):
+    """PM.py:168-229: L-BFGS-B from the filtered initial sinogram, bounds [0, 1]."""
+    from scipy import optimize
+    b0 = np.clip(filter_sinogram(fwd(target), filt), 0, None)
+    b0 = b0 / np.max(b0)
+    shape = b0.shape
+
+    def f(bf):
+        return pm_loss_grad(bf.reshape(shape), target, bwd, fwd, **kw)[0]
+
+    def g(bf):
+        return pm_loss_grad(bf.reshape(shape), target, bwd, fwd, **kw)[1].reshape(-1)
+
+    res = optimize.minimize(fun=f, x0=b0.reshape(-1), jac=g, method="L-BFGS-B",
+                            bounds=np.tile(np.array([[0.0, 1.0]]), (b0.size, 1)),
+                            options={"maxiter": n_iter, "ftol": 1e-12, "disp": False, "maxcor": 10, "maxls": 30})
+    b = res.x.reshape(shape)
+    return b, bwd(b) * np.pi / 180

@@ -49,9 +49,9 @@ def test_unsupported_paths_raise():
     tg = vam.geometry.TargetGeometry(target=np.ones((8, 8, 2)))
     with pytest.raises(NotImplementedError):
         vam.projectorconstructor.projectorconstructor(tg, vam.geometry.ProjectionGeometry(np.arange(4), "algebraic"))
-    with pytest.raises(NotImplementedError):
+    with pytest.raises(ValueError):
         vam.optimize.optimize(tg, vam.geometry.ProjectionGeometry(np.arange(4), "parallel"),
-                              vam.optimize.Options(method="PM"))
+                              vam.optimize.Options(method="XYZ"))
 
 
 def test_no_gpu_fails_loudly():

@@ -238,3 +238,36 @@ def test_osmo_with_insert_matches_oracle(gpu):
     e_b, e_x = rel_l2(sino.array, b_ref), rel_l2(recon.array, x_ref)
     print("OSMO with insert: rel L2 sinogram %.2e recon %.2e" % (e_b, e_x))
     assert e_b <= 1e-4 and e_x <= 1e-4
+
+
+def test_fbp_matches_oracle(gpu):
+    import oracle.optim as oo
+    import vamtoolbox_amd as vam
+    tg, pg, ang = _problem(seed=3)
+    for offset in (False, True):
+        sino, recon, err = vam.optimize.optimize(tg, pg, vam.optimize.Options(method="FBP", offset=offset))
+        fwd, bwd = _oracle_ops(ang)
+        b_ref, x_ref, ver_ref = oo.fbp(tg.array, fwd, bwd, offset=offset)
+        e_b, e_x = rel_l2(sino.array, b_ref), rel_l2(recon.array, x_ref)
+        print("FBP offset=%s rel L2 sinogram %.2e recon %.2e VER %.4f / %.4f" % (offset, e_b, e_x, err[0], ver_ref))
+        assert e_b <= 1e-5 and e_x <= 1e-5
+        assert abs(err[0] - ver_ref) <= 1e-3
+
+
+def test_pm_matches_oracle(gpu):
+    """PM (L-BFGS-B on the host, projections on the GPU) against the same
+    scipy iteration on the float64 oracle operators."""
+    import oracle.optim as oo
+    import vamtoolbox_amd as vam
+    tg, pg, ang = _problem(n=24, nz=2, na=40, seed=5)
+    opts = vam.optimize.Options(method="PM", n_iter=4, d_h=0.8, d_l=0.7, p=2)
+    sino, recon, err = vam.optimize.optimize(tg, pg, opts)
+    fwd, bwd = _oracle_ops(ang)
+    b_ref, x_ref = oo.pm(tg.array, fwd, bwd, 4, p=2.0, d_h=0.8, d_l=0.7)
+    e_b, e_x = rel_l2(sino.array, b_ref), rel_l2(recon.array, x_ref)
+    L0, _, _ = oo.pm_loss_grad(np.clip(oo.filter_sinogram(fwd(tg.array)), 0, None) /
+                               np.max(np.clip(oo.filter_sinogram(fwd(tg.array)), 0, None)), tg.array, bwd, fwd, p=2.0)
+    L1, _, _ = oo.pm_loss_grad(sino.array, tg.array, bwd, fwd, p=2.0)
+    print("PM rel L2 sinogram %.2e recon %.2e; loss %.4g -> %.4g" % (e_b, e_x, L0, L1))
+    assert L1 < L0
+    assert e_b <= 1e-3 and e_x <= 1e-3

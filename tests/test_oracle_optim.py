@@ -126,3 +126,26 @@ def test_osmo_cal_oracle_tiny_run():
     assert err[-1] <= err[0]
     _, _, err2 = oo.cal(t, fwd, bwd, 6, lr=0.002)
     assert np.all(np.isfinite(err2))
+
+
+def test_pm_gradient_finite_differences():
+    """PM loss gradient (PM.py:117-140) on an explicit linear pair where the
+    forward operator is the transpose of the backward one: the gradient is
+    then exact and must match central differences of the loss."""
+    import oracle.optim as oo
+    rng = np.random.default_rng(12)
+    n = 6
+    target = np.zeros((n, n, 1))
+    target[2:4, 2:4, 0] = 1.0
+    Bm = rng.uniform(0, 1, (n * n, 20)) * 40.0
+    bwd = lambda b: (Bm @ b.reshape(-1)).reshape(n, n, 1)
+    fwd = lambda v: (Bm.T @ v.reshape(-1)).reshape(20)
+    b = rng.uniform(0, 1, 20)
+    for p in (1.0, 2.0):
+        L, G, _ = oo.pm_loss_grad(b, target, bwd, fwd, p=p, d_h=0.8, d_l=0.3)
+        for j in range(0, 20, 3):
+            e = np.zeros(20)
+            e[j] = 1e-6
+            fd = (oo.pm_loss_grad(b + e, target, bwd, fwd, p=p, d_h=0.8, d_l=0.3)[0] -
+                  oo.pm_loss_grad(b - e, target, bwd, fwd, p=p, d_h=0.8, d_l=0.3)[0]) / 2e-6
+            assert abs(fd - G[j]) <= 1e-4 * max(1.0, abs(G[j])), (p, j, fd, G[j])

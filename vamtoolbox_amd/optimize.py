@@ -11,7 +11,7 @@ BCLP keywords of this package: ``fused`` (default True), ``hip_graph``
 
 from __future__ import annotations
 
-from .optimizer import BCLP, CAL, OSMO
+from .optimizer import BCLP, CAL, FBP, OSMO, PM
 
 
 class Options:
@@ -79,6 +79,8 @@ def optimize(target_geo, proj_geo, options: Options, output="packaged"):
         return OSMO.minimizeOSMO(target_geo, proj_geo, options)
     if options.method == "BCLP":
         return BCLP.minimizeBCLP(target_geo, proj_geo, options, output)
-    if options.method in ("FBP", "PM"):
-        raise NotImplementedError("method %s is not part of vamtoolbox_amd (CAL, OSMO, BCLP are)" % options.method)
+    if options.method == "FBP":
+        return FBP.minimizeFBP(target_geo, proj_geo, options)
+    if options.method == "PM":
+        return PM.minimizePM(target_geo, proj_geo, options)
     raise ValueError("unknown method %r" % (options.method,))
