@@ -252,3 +252,27 @@ def test_occlusion_parity(gpu, case):
     # the shadow removes something: differs from the unoccluded projection
     free = ParallelPlan(n, nz, ang, device=gpu, **kw)
     assert rel_l2(f, free.forward(torch.tensor(vol, dtype=torch.float32, device=gpu)).cpu().numpy()) > 1e-3
+
+
+def test_zero_dose_constraint(gpu):
+    """Zero-dose body: the sinogram entries whose rays cross it are zeroed
+    before every back projection (geometry.py:157-159,
+    Projector3DParallel.py:43-44 via projectorconstructor.py:162-163)."""
+    import oracle.d1 as d1
+    import vamtoolbox_amd as vam
+    n, nz = 28, 3
+    ang = np.arange(0.0, 180.0, 6.0)
+    yy, xx = np.mgrid[:n, :n] - (n - 1) / 2
+    zd = np.repeat(((xx - 7) ** 2 + (yy + 3) ** 2 <= 6)[:, :, None], nz, axis=2).astype(np.float64)
+    rng = np.random.default_rng(21)
+    tg = vam.geometry.TargetGeometry(target=rng.uniform(0, 1, (n, n, nz)), zero_dose=zd)
+    pg = vam.geometry.ProjectionGeometry(ang, "parallel", CUDA=True)
+    A = vam.projectorconstructor.projectorconstructor(tg, pg)
+    zs = d1.forward(zd, ang) != 0
+    assert np.array_equal(np.asarray(pg.zero_dose_sino), zs)
+    sino = rng.uniform(0, 1, (n, len(ang), nz))
+    got = A.backward(sino)
+    ref = d1.backward(np.where(zs, 0.0, sino), ang)
+    e = rel_l2(got, ref)
+    print("zero-dose backward rel L2 %.2e, %d of %d sinogram entries zeroed" % (e, zs.sum(), zs.size))
+    assert e <= TOL and zs.sum() > 0
