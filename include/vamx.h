@@ -214,6 +214,32 @@ int vamx_rt_deposit(vamx_plan* plan, const float* g, float* dose, void* stream);
  * Synchronises the plan's stream. */
 int vamx_rt_stats(vamx_plan* plan, int64_t* steps);
 
+/* Propagation matrix of the ray tracer (PyTorchRayTracingPropagator.buildPropagationMatrix,
+ * pyTorchRayTrace.py:168-197 -> recordEnergyUntilExit 573-690), in two passes
+ * over the same traces:
+ *   vamx_rt_record_counts: counts[ray] = number of interaction taps of the ray
+ *                          (device int64 [n_rays]);
+ *   vamx_rt_record:        with offsets = exclusive prefix sum of counts, writes the
+ *                          ray's taps (voxel index, weight = step * E * trilinear weight)
+ *                          to voxel/weight[offsets[ray] ...] in step order.
+ * Summing equal (ray, voxel) pairs and scaling column v by alpha_abs[v] gives the
+ * matrix P with P f = vamx_rt_integrate(f) and P^T g = vamx_rt_deposit(g). */
+int vamx_rt_record_counts(vamx_plan* plan, int64_t* counts, void* stream);
+int vamx_rt_record(vamx_plan* plan, const int64_t* offsets, int32_t* voxel, float* weight, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Algebraic (sparse-matrix) propagation
+ * ------------------------------------------------------------------------- */
+/* y = P x for a CSR matrix P (n_rows x n_cols; device row_ptr [n_rows+1] int64,
+ * col_idx / values [nnz] int32 / float32) applied to each of z_tiling z layers:
+ * x is (n_cols, z_tiling) and y (n_rows, z_tiling), z fastest.  Replaces
+ * torch.sparse.mm in PyTorchAlgebraicPropagator.forward / backward
+ * (vamtoolbox/projector/pyTorchAlgebraicPropagation.py:152-182; the backward
+ * passes the CSR transpose).  Column indices must be < n_cols (not checked on
+ * the device).  Deterministic (fixed summation order). */
+int vamx_csr_spmm(const int64_t* row_ptr, const int32_t* col_idx, const float* values, int64_t n_rows,
+                  int64_t n_cols, int32_t z_tiling, const float* x, float* y, void* stream);
+
 /* ---------------------------------------------------------------------------
  * Batched uneven parallel-beam problems in one launch
  * ------------------------------------------------------------------------- */

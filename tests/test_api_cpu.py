@@ -48,7 +48,7 @@ def test_projection_geometry_keywords():
 def test_unsupported_paths_raise():
     tg = vam.geometry.TargetGeometry(target=np.ones((8, 8, 2)))
     with pytest.raises(NotImplementedError):
-        vam.projectorconstructor.projectorconstructor(tg, vam.geometry.ProjectionGeometry(np.arange(4), "algebraic"))
+        vam.projectorconstructor.projectorconstructor(tg, vam.geometry.ProjectionGeometry(np.arange(4), "cone"))
     with pytest.raises(ValueError):
         vam.optimize.optimize(tg, vam.geometry.ProjectionGeometry(np.arange(4), "parallel"),
                               vam.optimize.Options(method="XYZ"))
@@ -73,3 +73,22 @@ def test_medium_models():
     assert a.max() == 2.0 and a.min() == 0.0
     hm = vam.medium.IndexModel(cv, type="analytical", form="homogeneous", n_sur=1.4)
     assert np.all(hm.presampled_scalar_field_3D == 1.4) and np.all(hm.presampled_vector_field_4D == 0)
+
+
+def test_algebraic_matrix_formats(tmp_path):
+    """The propagation matrix loads from SciPy .npz and torch .pt files alike
+    (pyTorchAlgebraicPropagation.py:50-117)."""
+    import scipy.sparse
+    import torch
+    from vamtoolbox_amd.projector.pyTorchAlgebraicPropagation import _load_matrix
+    rng = np.random.default_rng(0)
+    m = scipy.sparse.random(30, 20, density=0.2, random_state=rng, format="csr")
+    scipy.sparse.save_npz(tmp_path / "m.npz", m)
+    coo = m.tocoo()
+    t = torch.sparse_coo_tensor(np.vstack((coo.row, coo.col)), coo.data, size=m.shape)
+    torch.save(t, tmp_path / "m.pt")
+    a = _load_matrix(str(tmp_path / "m.npz"))
+    b = _load_matrix(str(tmp_path / "m.pt"))
+    assert (abs(a - m)).max() == 0 and (abs(b - m)).max() == 0
+    with pytest.raises(Exception):
+        _load_matrix(str(tmp_path / "m.txt"))

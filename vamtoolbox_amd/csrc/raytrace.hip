@@ -208,12 +208,23 @@ __device__ __forceinline__ Taps taps_at(const RtDev& g, d3 x) {
   return t;
 }
 
-template <bool DEPOSIT, int SOLVER, int PARAM>
-__global__ __launch_bounds__(256) void rt_trace_kernel(RtDev g, const float* __restrict__ feff, float* __restrict__ gout,
-                                                       const float* __restrict__ gin,
-                                                       unsigned long long* __restrict__ acc,
-                                                       const double* __restrict__ fx_scale,
-                                                       unsigned long long* __restrict__ stats) {
+// what a trace does at each interaction tap
+enum RtMode { RT_INTEGRATE = 0, RT_DEPOSIT = 1, RT_COUNT = 2, RT_RECORD = 3 };
+
+struct RtIo {
+  const float* feff;              // integrate: alpha_abs * f
+  float* gout;                    // integrate: one value per ray
+  const float* gin;               // deposit: ray energies
+  unsigned long long* acc;        // deposit: fixed-point accumulator
+  const double* fx_scale;         // deposit: fixed-point scale
+  unsigned long long* stats;      // [2] step statistics
+  long long* counts;              // count: taps per ray (out); record: first slot per ray (in)
+  int32_t* rec_vox;               // record: voxel of each tap
+  float* rec_w;                   // record: weight of each tap
+};
+
+template <int MODE, int SOLVER, int PARAM>
+__global__ __launch_bounds__(256) void rt_trace_kernel(RtDev g, RtIo io) {
   const long long nrays = (long long)g.nt * g.na * g.nzr;
   const long long rid = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   // every lane runs to the end (no early return): the step statistics are
@@ -224,11 +235,13 @@ __global__ __launch_bounds__(256) void rt_trace_kernel(RtDev g, const float* __r
   const long long ta = r / g.nzr;
   const int a = (int)(ta % g.na), t = (int)(ta / g.na);
   double E0 = (double)g.energy0;
-  if (DEPOSIT && live) {
-    E0 = (double)gin[r];
+  if (MODE == RT_DEPOSIT && live) {
+    E0 = (double)io.gin[r];
     live = E0 != 0.0;  // a ray of zero energy contributes nothing and is not traced
   }
-  const double scale = DEPOSIT ? *fx_scale : 0.0;
+  const double scale = MODE == RT_DEPOSIT ? *io.fx_scale : 0.0;
+  long long slot = (MODE == RT_RECORD && live) ? io.counts[r] : 0;   // record: this ray's first slot
+  long long ntaps = 0;
   d3 x, v;
   ray_init<SOLVER, PARAM>(g, t, a, zr, x, v);
   const double hw = g.step / g.dx;  // step_size_idx_unit (pyTorchRayTrace.py:840-842)
@@ -256,11 +269,17 @@ __global__ __launch_bounds__(256) void rt_trace_kernel(RtDev g, const float* __r
                            (tp.nzt == 1 ? 1.0 : (dzc ? tp.wz : 1.0 - tp.wz));
           if (w == 0.0) continue;
           const size_t vi = ((size_t)xi * g.ny + yi) * g.nz + zi;
-          if (DEPOSIT) {
+          if (MODE == RT_DEPOSIT) {
             const long long q = llrint(base * w * scale);
-            atomicAdd(acc + vi, (unsigned long long)q);
+            atomicAdd(io.acc + vi, (unsigned long long)q);
+          } else if (MODE == RT_INTEGRATE) {
+            integral += (double)io.feff[vi] * (base * w);
+          } else if (MODE == RT_COUNT) {
+            ++ntaps;
           } else {
-            integral += (double)feff[vi] * (base * w);
+            io.rec_vox[slot] = (int32_t)vi;
+            io.rec_w[slot] = (float)(base * w);
+            ++slot;
           }
         }
       }
@@ -271,7 +290,8 @@ __global__ __launch_bounds__(256) void rt_trace_kernel(RtDev g, const float* __r
       }
     }
   }
-  if (!DEPOSIT && rid < nrays) gout[rid] = (float)integral;
+  if (MODE == RT_INTEGRATE && rid < nrays) io.gout[rid] = (float)integral;
+  if (MODE == RT_COUNT && rid < nrays) io.counts[rid] = ntaps;
   // wavefront sums of the step count and of the rays still active at max_steps
   unsigned long long st = live ? (unsigned long long)step : 0ull;
   unsigned long long un = active ? 1ull : 0ull;
@@ -281,8 +301,8 @@ __global__ __launch_bounds__(256) void rt_trace_kernel(RtDev g, const float* __r
     un += __shfl_xor(un, off);
   }
   if ((threadIdx.x & 63) == 0) {
-    atomicAdd(stats, st);
-    if (un) atomicAdd(stats + 1, un);
+    atomicAdd(io.stats, st);
+    if (un) atomicAdd(io.stats + 1, un);
   }
 }
 
@@ -362,35 +382,25 @@ __global__ __launch_bounds__(256) void rt_finalize_kernel(const unsigned long lo
 
 // compile-time solver and parametrisation: each instantiation keeps only its
 // own stepper's registers live
-template <bool DEPOSIT, int SOLVER>
-int launch_trace_s(const RtDev& g, dim3 grid, hipStream_t s, const float* feff, float* gout, const float* gin,
-                   unsigned long long* acc, const double* scale, unsigned long long* stats) {
+template <int MODE, int SOLVER>
+int launch_trace_s(const RtDev& g, dim3 grid, hipStream_t s, const RtIo& io) {
   if (g.param == VAMX_RT_CANONICAL)
-    hipLaunchKernelGGL((rt_trace_kernel<DEPOSIT, SOLVER, VAMX_RT_CANONICAL>), grid, dim3(256), 0, s, g, feff, gout,
-                       gin, acc, scale, stats);
+    hipLaunchKernelGGL((rt_trace_kernel<MODE, SOLVER, VAMX_RT_CANONICAL>), grid, dim3(256), 0, s, g, io);
   else if (g.param == VAMX_RT_PHYSICAL)
-    hipLaunchKernelGGL((rt_trace_kernel<DEPOSIT, SOLVER, VAMX_RT_PHYSICAL>), grid, dim3(256), 0, s, g, feff, gout,
-                       gin, acc, scale, stats);
+    hipLaunchKernelGGL((rt_trace_kernel<MODE, SOLVER, VAMX_RT_PHYSICAL>), grid, dim3(256), 0, s, g, io);
   else
-    hipLaunchKernelGGL((rt_trace_kernel<DEPOSIT, SOLVER, VAMX_RT_OPTICAL>), grid, dim3(256), 0, s, g, feff, gout,
-                       gin, acc, scale, stats);
+    hipLaunchKernelGGL((rt_trace_kernel<MODE, SOLVER, VAMX_RT_OPTICAL>), grid, dim3(256), 0, s, g, io);
   return VAMX_OK;
 }
 
-template <bool DEPOSIT>
-int launch_trace(const RtDev& g, dim3 grid, hipStream_t s, const float* feff, float* gout, const float* gin,
-                 unsigned long long* acc, const double* scale, unsigned long long* stats) {
+template <int MODE>
+int launch_trace(const RtDev& g, dim3 grid, hipStream_t s, const RtIo& io) {
   switch (g.solver) {
-    case VAMX_RT_SYMPLECTIC_EULER:
-      return launch_trace_s<DEPOSIT, VAMX_RT_SYMPLECTIC_EULER>(g, grid, s, feff, gout, gin, acc, scale, stats);
-    case VAMX_RT_FORWARD_EULER:
-      return launch_trace_s<DEPOSIT, VAMX_RT_FORWARD_EULER>(g, grid, s, feff, gout, gin, acc, scale, stats);
-    case VAMX_RT_LEAPFROG:
-      return launch_trace_s<DEPOSIT, VAMX_RT_LEAPFROG>(g, grid, s, feff, gout, gin, acc, scale, stats);
-    case VAMX_RT_RK4:
-      return launch_trace_s<DEPOSIT, VAMX_RT_RK4>(g, grid, s, feff, gout, gin, acc, scale, stats);
-    default:
-      return set_error(VAMX_ERR_ARG, "ray tracer: unknown solver");
+    case VAMX_RT_SYMPLECTIC_EULER: return launch_trace_s<MODE, VAMX_RT_SYMPLECTIC_EULER>(g, grid, s, io);
+    case VAMX_RT_FORWARD_EULER: return launch_trace_s<MODE, VAMX_RT_FORWARD_EULER>(g, grid, s, io);
+    case VAMX_RT_LEAPFROG: return launch_trace_s<MODE, VAMX_RT_LEAPFROG>(g, grid, s, io);
+    case VAMX_RT_RK4: return launch_trace_s<MODE, VAMX_RT_RK4>(g, grid, s, io);
+    default: return set_error(VAMX_ERR_ARG, "ray tracer: unknown solver");
   }
 }
 
@@ -489,8 +499,11 @@ extern "C" int vamx_rt_integrate(vamx_plan* plan, const float* f, float* gout, v
   hipLaunchKernelGGL(rt_feff_kernel, dim3(blocks_for(nv, 256)), dim3(256), 0, s, f, g.abs_grid, feff, nv);
   hipLaunchKernelGGL(rt_zero_stats, dim3(1), dim3(1), 0, s, plan->d_stats);
   const long long nrays = (long long)g.nt * g.na * g.nzr;
-  const int rc = launch_trace<false>(g, dim3(blocks_for(nrays, 256)), s, feff, gout, nullptr, nullptr, nullptr,
-                                     plan->d_stats);
+  RtIo io{};
+  io.feff = feff;
+  io.gout = gout;
+  io.stats = plan->d_stats;
+  const int rc = launch_trace<RT_INTEGRATE>(g, dim3(blocks_for(nrays, 256)), s, io);
   if (rc) return rc;
   VAMX_HIP(hipGetLastError());
   plan->last_stream = s;
@@ -509,11 +522,53 @@ extern "C" int vamx_rt_deposit(vamx_plan* plan, const float* gin, float* dose, v
   hipLaunchKernelGGL(rt_gmax_part, dim3(kGmaxParts), dim3(256), 0, s, gin, nrays, gmax_part);
   hipLaunchKernelGGL(rt_gmax_final, dim3(1), dim3(256), 0, s, gmax_part, (double)g.na, scale, plan->d_stats);
   hipLaunchKernelGGL(rt_zero_u64, dim3(blocks_for(nv, 256)), dim3(256), 0, s, plan->d_accum, nv);
-  const int rc = launch_trace<true>(g, dim3(blocks_for(nrays, 256)), s, nullptr, nullptr, gin, plan->d_accum, scale,
-                                    plan->d_stats);
+  RtIo io{};
+  io.gin = gin;
+  io.acc = plan->d_accum;
+  io.fx_scale = scale;
+  io.stats = plan->d_stats;
+  const int rc = launch_trace<RT_DEPOSIT>(g, dim3(blocks_for(nrays, 256)), s, io);
   if (rc) return rc;
   hipLaunchKernelGGL(rt_finalize_kernel, dim3(blocks_for(nv, 256)), dim3(256), 0, s, plan->d_accum, g.abs_grid,
                      (const double*)scale, dose, nv);
+  VAMX_HIP(hipGetLastError());
+  plan->last_stream = s;
+  return VAMX_OK;
+}
+
+extern "C" int vamx_rt_record_counts(vamx_plan* plan, int64_t* counts, void* stream) {
+  if (!check_plan(plan, PLAN_RAYTRACE))
+    return set_error(VAMX_ERR_STATE, "vamx_rt_record_counts: not a ray-trace plan");
+  if (!counts) return set_error(VAMX_ERR_ARG, "vamx_rt_record_counts: null buffer");
+  hipStream_t s = (hipStream_t)stream;
+  const RtDev& g = plan->rt;
+  const long long nrays = (long long)g.nt * g.na * g.nzr;
+  hipLaunchKernelGGL(rt_zero_stats, dim3(1), dim3(1), 0, s, plan->d_stats);
+  RtIo io{};
+  io.counts = (long long*)counts;
+  io.stats = plan->d_stats;
+  const int rc = launch_trace<RT_COUNT>(g, dim3(blocks_for(nrays, 256)), s, io);
+  if (rc) return rc;
+  VAMX_HIP(hipGetLastError());
+  plan->last_stream = s;
+  return VAMX_OK;
+}
+
+extern "C" int vamx_rt_record(vamx_plan* plan, const int64_t* offsets, int32_t* voxel, float* weight,
+                              void* stream) {
+  if (!check_plan(plan, PLAN_RAYTRACE)) return set_error(VAMX_ERR_STATE, "vamx_rt_record: not a ray-trace plan");
+  if (!offsets || !voxel || !weight) return set_error(VAMX_ERR_ARG, "vamx_rt_record: null buffer");
+  hipStream_t s = (hipStream_t)stream;
+  const RtDev& g = plan->rt;
+  const long long nrays = (long long)g.nt * g.na * g.nzr;
+  hipLaunchKernelGGL(rt_zero_stats, dim3(1), dim3(1), 0, s, plan->d_stats);
+  RtIo io{};
+  io.counts = (long long*)offsets;
+  io.rec_vox = voxel;
+  io.rec_w = weight;
+  io.stats = plan->d_stats;
+  const int rc = launch_trace<RT_RECORD>(g, dim3(blocks_for(nrays, 256)), s, io);
+  if (rc) return rc;
   VAMX_HIP(hipGetLastError());
   plan->last_stream = s;
   return VAMX_OK;

@@ -37,7 +37,7 @@ def minimizeOSMO(target_geo, proj_geo, options):
         return torch.where(b < min_proj_val, torch.full_like(b, min_proj_val), b)
 
     def reconstruct(b):
-        x = A.backward(b)
+        x = A.backward(b).reshape(T.shape)      # the algebraic propagator returns flat arrays
         x = x / torch.amax(x)
         return torch.clamp(x, min=0.0)
 
@@ -45,7 +45,7 @@ def minimizeOSMO(target_geo, proj_geo, options):
     x_model = data.filterTargetOSMO(T, options.filter)
     x_model = (x_model.real if torch.is_complex(x_model) else x_model).to(torch.float32)
     b = torch.clamp(A.forward(x_model), min=0.0)                 # OSMO.py:133-134
-    x = A.backward(b)
+    x = A.backward(b).reshape(T.shape)
     x = x / torch.amax(x)
     _error[0] = metrics.calcVER(target_geo, x)
     iter_times[0] = time.perf_counter() - t0

@@ -153,6 +153,42 @@ class PyTorchRayTracingPropagator:
                 f"Rays active: {buf[1]}/{self.num_rays}.")
         return int(buf[0]), int(buf[1])
 
+    def buildPropagationMatrix(self):
+        """Propagation matrix P (rays x voxels) with P f = forward(f) and
+        P^T g = backward(g) (pyTorchRayTrace.py:168-197, recordEnergyUntilExit
+        573-690): every interaction tap of every ray is recorded on the GPU
+        (two passes over the same traces: counts, then taps), equal
+        (ray, voxel) pairs are summed on the host in float64, and column v is
+        scaled by the absorption coefficient at voxel v.  Returned as a torch
+        sparse COO tensor when ``output_torch_tensor`` is set, else as a
+        ``scipy.sparse.coo_array``."""
+        import scipy.sparse
+        dev = self.device
+        n = self.num_rays
+        counts = torch.empty(n, dtype=torch.int64, device=dev)
+        with torch.cuda.device(dev):
+            _lib.check(self.lib.vamx_rt_record_counts(self.handle, counts.data_ptr(), _lib.stream_ptr(dev)),
+                       "vamx_rt_record_counts")
+            offsets = (torch.cumsum(counts, 0) - counts).contiguous()
+            total = int(counts.sum().item())
+            vox = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
+            w = torch.empty(max(total, 1), dtype=torch.float32, device=dev)
+            _lib.check(self.lib.vamx_rt_record(self.handle, offsets.data_ptr(), vox.data_ptr(), w.data_ptr(),
+                                               _lib.stream_ptr(dev)), "vamx_rt_record")
+        self.stats()
+        rows = np.repeat(np.arange(n, dtype=np.int64), counts.cpu().numpy())
+        nvox = int(np.prod(self.vol_shape))
+        P = scipy.sparse.coo_matrix((w[:total].cpu().numpy().astype(np.float64), (rows, vox[:total].cpu().numpy())),
+                                    shape=(n, nvox)).tocsr()            # sums equal (ray, voxel) pairs
+        if self.alpha_abs is not None:
+            P = P @ scipy.sparse.diags(np.asarray(self.alpha_abs, dtype=np.float64).ravel())
+        P = scipy.sparse.coo_array(P)
+        if self.output_torch_tensor:
+            idx = torch.as_tensor(np.vstack((P.row, P.col)).astype(np.int64), device=dev)
+            return torch.sparse_coo_tensor(idx, torch.as_tensor(P.data, dtype=torch.float32, device=dev),
+                                           size=P.shape).coalesce()
+        return P
+
     def inverseBackward(self, f, filter_option="ram-lak", **kwargs):
         """Initial sinogram: f / alpha_abs^2, forward, ramp filter, * pi/(2 n_angles) (135-166)."""
         t, torch_in = self._vol_in(f)

@@ -9,8 +9,8 @@ runs on the GPU through libvamx_hip.so:
 * ``ray_type == "parallel"``   -> ``Projector3DParallelHIP`` for both ``CUDA``
   settings (the reference's Python-projector discretisation; ASTRA's
   ``line``/``parallel3d`` kernels are not reproduced)
-* ``ray_type == "algebraic"``  -> NotImplementedError (sparse-matrix
-  propagation is not part of this package)
+* ``ray_type == "algebraic"``  -> ``PyTorchAlgebraicPropagator`` (CSR matrix
+  from ``proj_geo.loading_path_for_matrix``, HIP CSR kernel)
 
 As in the reference, an insert turns into infinite attenuation (62-70) and a
 zero-dose body into ``proj_geo.zero_dose_sino`` (162-163).  With an
@@ -53,8 +53,9 @@ def projectorconstructor(target_geo: geometry.TargetGeometry, proj_geo: geometry
             proj_geo.attenuation_field = np.where(target_geo.insert == 1, np.inf, 0)
 
     if proj_geo.ray_type == "algebraic":
-        raise NotImplementedError("algebraic (sparse-matrix) propagation is not part of vamtoolbox_amd")
-    if proj_geo.ray_type == "ray_trace":
+        from .projector.pyTorchAlgebraicPropagation import PyTorchAlgebraicPropagator
+        A = PyTorchAlgebraicPropagator(target_geo, proj_geo)
+    elif proj_geo.ray_type == "ray_trace":
         from .projector.pyTorchRayTrace import PyTorchRayTracingPropagator
         A = PyTorchRayTracingPropagator(target_geo, proj_geo)
     elif proj_geo.ray_type == "parallel":
