@@ -241,6 +241,21 @@ int vamx_csr_spmm(const int64_t* row_ptr, const int32_t* col_idx, const float* v
                   int64_t n_cols, int32_t z_tiling, const float* x, float* y, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Sinogram resampling (fan-beam rebinning)
+ * ------------------------------------------------------------------------- */
+/* out[p, z] = factors[2p+1] * (bilinear(sino[., ., z]) * factors[2p]) at the
+ * grid cell (cell[2p], cell[2p+1]) with weights (weights[2p] along the
+ * detector, weights[2p+1] along the angles); sino is (n_r, n_a, nz) float32,
+ * z fastest, out is (m, nz).  A cell with cell[2p] == -1 samples 0 (a point
+ * outside the grid: interpn's fill value); any other cell outside
+ * 0 <= cell[2p] <= n_r-2, 0 <= cell[2p+1] <= n_a-2 samples NaN.  The factors
+ * are the differential-area factor and the inverse Fresnel transmission.
+ * The resampling of rebinFanBeam (vamtoolbox/geometry.py:736-944; interpn at
+ * line 805, the factors at 814-821). */
+int vamx_sino_resample(const float* sino, int32_t n_r, int32_t n_a, int32_t nz, const int32_t* cell,
+                       const float* weights, const float* factors, int64_t m, float* out, void* stream);
+
+/* ---------------------------------------------------------------------------
  * Batched uneven parallel-beam problems in one launch
  * ------------------------------------------------------------------------- */
 /* Forward-project n_plans independent problems (each a parallel plan created

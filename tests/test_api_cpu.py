@@ -92,3 +92,41 @@ def test_algebraic_matrix_formats(tmp_path):
     assert (abs(a - m)).max() == 0 and (abs(b - m)).max() == 0
     with pytest.raises(Exception):
         _load_matrix(str(tmp_path / "m.txt"))
+
+
+def test_fan_beam_sampling_matches_oracle():
+    """The host geometry of rebinFanBeam equals the oracle's restatement."""
+    import oracle.rebin as rb
+    from vamtoolbox_amd.geometry import _fan_beam_sampling
+    ang = np.arange(0.0, 360.0, 7.5)
+    for w, n, tr in ((64, 1.48, 1.6), (33, 1.33, 0.8), (20, 1.0, 1e9)):
+        with np.errstate(divide="ignore", invalid="ignore"):
+            x_s, th_s, dxv, t_inv, xp_o = rb.sampling(ang, w, 1920, n, tr)
+            xp, xv, theta, area, tinv = _fan_beam_sampling(ang, w, 1920, n, tr)
+        np.testing.assert_array_equal(xp, xp_o)
+        np.testing.assert_allclose(xv, x_s[:, 0], rtol=0, atol=1e-12)
+        np.testing.assert_allclose(theta, th_s, rtol=0, atol=1e-12)
+        np.testing.assert_allclose(area, dxv[:, 0], rtol=1e-12)
+        np.testing.assert_allclose(tinv, t_inv[:, 0], rtol=1e-12)
+
+
+def test_grid_cells_reproduce_interpn():
+    """Cells and weights of the host lookup, combined bilinearly, give scipy's
+    linear interpn with fill value 0 -- at nodes, the last node, outside the
+    grid and at NaN positions."""
+    from scipy import interpolate
+    from vamtoolbox_amd.geometry import _grid_cells
+    rng = np.random.default_rng(5)
+    gx = np.linspace(-5.0, 5.0, 11)
+    ga = np.arange(0.0, 360.0, 30.0)
+    v = rng.normal(size=(gx.size, ga.size))
+    x = np.concatenate([rng.uniform(-6, 6, 200), gx, [gx[-1], gx[0], np.nan, 1.0]])
+    a = np.concatenate([rng.uniform(-10, 340, 200), ga[:11], [ga[-1], 330.0, 10.0, np.nan]])
+    ref = interpolate.interpn((gx, ga), v, (x, a), method="linear", bounds_error=False, fill_value=0)
+    ix, wx, inx, nanx = _grid_cells(gx, x)
+    ia, wa, ina, nana = _grid_cells(ga, a)
+    top = v[ix, ia] + wa * (v[ix, ia + 1] - v[ix, ia])
+    bot = v[ix + 1, ia] + wa * (v[ix + 1, ia + 1] - v[ix + 1, ia])
+    got = np.where(inx & ina, top + wx * (bot - top), 0.0)
+    got[nanx | nana] = np.nan
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12)

@@ -149,3 +149,54 @@ def test_pm_gradient_finite_differences():
             fd = (oo.pm_loss_grad(b + e, target, bwd, fwd, p=p, d_h=0.8, d_l=0.3)[0] -
                   oo.pm_loss_grad(b - e, target, bwd, fwd, p=p, d_h=0.8, d_l=0.3)[0]) / 2e-6
             assert abs(fd - G[j]) <= 1e-4 * max(1.0, abs(G[j])), (p, j, fd, G[j])
+
+
+def test_fan_beam_rebin_telecentric_limit():
+    """With n_write = 1 (no refraction) and a nearly telecentric projector
+    (throw ratio 1e9) each ray keeps its pixel's position, x_samp = xp, and
+    is turned by thetav = -arctan(xp / throw) only, T = 1 away from the
+    grazing edge pixels, and dxv/dxp is the pixel pitch w / (w - 1).  The
+    reference's wrap rule sends every angle above max - step to the first
+    angle: the last angle always, the one before it for the pixels xp < 0,
+    whose thetav is positive (but for the grazing edge pixels).  The rebinned sinogram is then the input, with
+    those columns replaced by the first one, times the pitch; the grazing
+    edge pixels (T = 0) give inf, or NaN where the sample is 0."""
+    import oracle.rebin as rb
+    rng = np.random.default_rng(2)
+    w, na, nz = 24, 12, 3
+    ang = np.arange(0.0, 360.0, 360.0 / na)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        x_samp, theta_samp, dxv, t_inv, xp = rb.sampling(ang, w, 1920, 1.0, 1e9)
+        sino = rng.uniform(0.5, 1, (w, na, nz))
+        sino[0, 0, 0] = 0.0
+        out = rb.rebin_fan_beam(sino, ang, w, (1920, 1080), 1.0, 1e9)
+    np.testing.assert_allclose(x_samp, np.repeat(xp[:, None], na, axis=1), atol=1e-6)
+    expect_theta = np.tile(ang, (w, 1))
+    expect_theta[:, -1] = ang[0]
+    expect_theta[xp < 0, -2] = ang[0]
+    np.testing.assert_allclose(theta_samp[1:-1], expect_theta[1:-1], atol=1e-6)   # grazing edges reflect
+    np.testing.assert_allclose(t_inv[1:-1], 1.0, atol=1e-12)
+    assert np.all(np.isinf(t_inv[[0, -1]]))
+    pitch = w / (w - 1)
+    np.testing.assert_allclose(dxv, pitch, rtol=1e-9)
+    cols = np.where(expect_theta[..., None] == ang[0], sino[:, :1, :], sino)
+    np.testing.assert_allclose(out[1:-1], cols[1:-1] * pitch, rtol=1e-5)
+    assert np.isnan(out[0, 0, 0]) and np.all(~np.isfinite(out[[0, -1]])) and np.isinf(out[-1]).all()
+
+
+def test_fan_beam_snell_and_centre_ray():
+    """Refraction at the vial wall conserves the ray's impact parameter over
+    the refractive index (Bouguer's invariant for a circle): a pixel at xp,
+    diverging by phi = arctan(xp / throw), passes the axis at xp cos(phi), so
+    inside the vial it runs at xv = xp cos(phi) / n_write.  At normal
+    incidence (centre pixel) the Fresnel transmission is 4 n / (1 + n)^2."""
+    import oracle.rebin as rb
+    w, n, tr = 101, 1.48, 1.6
+    with np.errstate(divide="ignore", invalid="ignore"):
+        x_samp, theta_samp, dxv, t_inv, xp = rb.sampling(np.arange(0.0, 360.0, 10.0), w, 1920, n, tr)
+    phi = np.arctan(xp / (tr * 1920))
+    np.testing.assert_allclose(x_samp[:, 0], xp * np.cos(phi) / n, atol=1e-9)
+    c = w // 2
+    assert abs(xp[c]) < 1e-12 and abs(theta_samp[c, 3] - 30.0) < 1e-12
+    np.testing.assert_allclose(t_inv[c, 0], (1 + n) ** 2 / (4 * n), rtol=1e-12)
+    assert np.all(t_inv[1:-1] >= 1.0)               # transmission never exceeds 1

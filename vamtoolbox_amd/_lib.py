@@ -134,6 +134,8 @@ SIGNATURES = {
     "vamx_rt_stats": (c_int32, [_PLAN_P, ctypes.POINTER(ctypes.c_int64)]),
     "vamx_rt_record_counts": (c_int32, [_PLAN_P, c_void_p, c_void_p]),
     "vamx_rt_record": (c_int32, [_PLAN_P, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "vamx_sino_resample": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p,
+                                     ctypes.c_int64, c_void_p, c_void_p]),
     "vamx_csr_spmm": (c_int32, [c_void_p, c_void_p, c_void_p, ctypes.c_int64, ctypes.c_int64, c_int32, c_void_p,
                                 c_void_p, c_void_p]),
     "vamx_par_forward_batched": (c_int32, [ctypes.POINTER(c_void_p), c_int32, ctypes.POINTER(c_void_p),

@@ -225,3 +225,127 @@ def getInds(target):
     target = np.asarray(target)
     circle_mask = getCircleMask(target)
     return np.logical_and(target > 0, circle_mask), np.logical_and(target == 0, circle_mask)
+
+
+def _fan_beam_sampling(angles, vial_width: int, n_u: int, n_write: float, throw_ratio: float):
+    """Where a non-telecentric projector's refracted rays sample Radon space
+    (geometry.py:823-918; Orth et al., Opt. Express 29, 11037 (2021)).
+
+    Returns, per projector pixel xp of the apparent vial width, the ray's
+    offset xv inside the vial, its angle turn thetav in degrees, the
+    differential-area factor dxv/dxp and the inverse Fresnel transmission,
+    and the (pixel, angle) grid of sampled angles after the reference's
+    wrap-around rule.  All float64, computed on the host (O(width * angles)).
+    """
+    angles = np.asarray(angles, dtype=np.float64)
+    w = int(vial_width)
+    d = throw_ratio * n_u                                   # projector distance in pixels
+    r_vial = (w / 2) * np.sqrt(1 + (w / (2 * d)) ** 2)      # true vial radius (pixels)
+    xp = np.linspace(-w / 2, w / 2, w)
+    phi = np.arctan(xp / d)                                 # pixel divergence
+    q = 1 + (xp / d) ** 2
+    x_hit = (xp - xp * np.sqrt(1 - q * (1 - (r_vial / d) ** 2))) / q   # wall hit point
+    normal = np.arcsin(x_hit / r_vial)
+    inc = normal + phi                                      # incidence angle
+    trans = np.arcsin(np.sin(inc) / n_write)                # Snell, air index 1
+    turn = normal - trans                                   # ray angle inside the vial
+    ci, ct = np.cos(inc), np.cos(trans)
+    t_s = 1 - ((ci - n_write * ct) / (ci + n_write * ct)) ** 2
+    t_p = 1 - ((ct - n_write * ci) / (ct + n_write * ci)) ** 2
+    t = (t_s + t_p) / 2
+    t[t < 0] = 0
+    xv = x_hit * np.cos(turn) - np.sin(turn) * np.sqrt(r_vial ** 2 - x_hit ** 2)
+    area = np.gradient(xv)
+    area[area < 0] = 0                                      # sign flips of the area element
+    with np.errstate(divide="ignore"):
+        t_inv = 1 / t
+    theta = angles[None, :] + np.degrees(turn)[:, None]
+    lo, hi = angles[0], angles[-1]
+    step = abs(angles[1] - angles[0])
+    theta[theta > hi] -= 360
+    theta[theta < lo] += 360
+    theta[theta > hi - step] = lo                           # geometry.py:914 (the later rule at 915 is void)
+    return xp, xv, theta, area, t_inv
+
+
+def _grid_cells(grid, x):
+    """Cell index and weight of x on an ascending grid, as scipy's linear
+    interpn finds them: grid[i] <= x < grid[i+1], the last node in the last
+    cell; out-of-range and NaN positions flagged separately."""
+    i = np.clip(np.searchsorted(grid, x, side="right") - 1, 0, grid.size - 2)
+    with np.errstate(invalid="ignore"):
+        wt = (x - grid[i]) / (grid[i + 1] - grid[i])
+        inside = (x >= grid[0]) & (x <= grid[-1])
+    return i, wt, inside, np.isnan(x)
+
+
+def rebinFanBeam(sinogram, vial_width, N_screen, n_write, throw_ratio):
+    """Rebin a parallel-beam sinogram for a non-telecentric projector whose
+    light refracts at an air/vial wall (geometry.py:736-944).
+
+    The sinogram (nR, nTheta, nZ) is zero-padded to ``vial_width`` detector
+    pixels and resampled, by scipy's linear ``interpn`` rule with fill value
+    0, at the (ray offset, angle) each physical pixel actually samples, then
+    multiplied by the differential-area factor and the inverse Fresnel
+    transmission.  Grazing edge pixels with zero transmission give inf (NaN
+    where the sample is 0), as in the reference.  The geometry is float64 on
+    the host; the resampling of every z slice is one HIP launch
+    (``vamx_sino_resample``) in float32.  Unlike the reference, the input's
+    array is not padded in place.  NumPy in -> NumPy out of the input's
+    float dtype; a torch GPU tensor in -> float32 GPU tensor out.
+    """
+    import torch
+    from . import _lib
+    from ._device import require_gpu
+
+    proj_geo = getattr(sinogram, "proj_geo", None)
+    if proj_geo is None:
+        raise ValueError("rebinFanBeam needs a Sinogram with its projection geometry (the angles)")
+    angles = np.asarray(proj_geo.angles, dtype=np.float64)
+    if angles.size < 2 or np.any(np.diff(angles) <= 0):
+        raise ValueError("rebinFanBeam needs at least two strictly ascending angles")
+    arr = sinogram.array
+    torch_in = isinstance(arr, torch.Tensor)
+    if len(arr.shape) != 3:
+        raise ValueError("rebinFanBeam needs a 3-D sinogram (nR, nTheta, nZ)")
+    n_r, n_a, n_z = (int(s) for s in arr.shape)
+    w = int(vial_width)
+    if n_a != angles.size:
+        raise ValueError("sinogram has %d angles, the geometry %d" % (n_a, angles.size))
+    if n_r > w or w < 2:
+        raise ValueError("vial_width (%d) must be at least the sinogram's %d detector pixels" % (w, n_r))
+
+    xp, xv, theta, area, t_inv = _fan_beam_sampling(angles, w, N_screen[0], n_write, throw_ratio)
+    ix, wx, in_x, nan_x = _grid_cells(xp, xv)
+    ia, wa, in_a, nan_a = _grid_cells(angles, theta)
+    cell = np.empty((w, n_a, 2), dtype=np.int32)
+    cell[..., 0] = ix[:, None]
+    cell[..., 1] = ia
+    cell[..., 0][~(in_x[:, None] & in_a)] = -1
+    cell[..., 0][nan_x[:, None] | nan_a] = -2
+    wts = np.empty((w, n_a, 2), dtype=np.float32)
+    wts[..., 0] = wx[:, None]
+    wts[..., 1] = wa
+    fac = np.empty((w, n_a, 2), dtype=np.float32)
+    fac[..., 0] = area[:, None]
+    fac[..., 1] = t_inv[:, None]
+
+    dev = arr.device if torch_in and arr.is_cuda else require_gpu()
+    lib = _lib.load()
+    src = torch.as_tensor(arr if torch_in else np.asarray(arr), device=dev).to(torch.float32)
+    pd = (w - n_r) // 2                                     # geometry.py:919-926
+    padded = torch.zeros((w, n_a, n_z), dtype=torch.float32, device=dev)
+    padded[pd:pd + n_r] = src
+    out = torch.empty((w, n_a, n_z), dtype=torch.float32, device=dev)
+    d_cell = torch.as_tensor(cell, device=dev)
+    d_wts = torch.as_tensor(wts, device=dev)
+    d_fac = torch.as_tensor(fac, device=dev)
+    with torch.cuda.device(dev):
+        _lib.check(lib.vamx_sino_resample(padded.data_ptr(), w, n_a, n_z, d_cell.data_ptr(), d_wts.data_ptr(),
+                                          d_fac.data_ptr(), w * n_a, out.data_ptr(), _lib.stream_ptr(dev)),
+                   "vamx_sino_resample")
+    if torch_in:
+        return Sinogram(out, proj_geo)
+    dt = np.asarray(arr).dtype
+    res = out.cpu().numpy().astype(dt if np.issubdtype(dt, np.floating) else np.float64)
+    return Sinogram(res, proj_geo)
