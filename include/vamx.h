@@ -89,14 +89,19 @@ int vamx_par_backward_adjoint(vamx_plan* plan, const float* vol, float* sino, vo
 /* Work order of the parallel-beam kernels (chosen by an autotuner on the host,
  * see vamtoolbox_amd/parallel.py ParallelPlan.autotune).  No reference
  * counterpart: ASTRA picks its launch shapes internally.  Results are
- * identical for every setting up to float32 rounding of the z-vector width. */
+ * identical for every setting up to float32 rounding: tiles and tap reuse
+ * give bitwise equal sinograms; opposite-angle pairing sums the second ray
+ * of a pair in reverse sample order. */
 typedef struct vamx_par_tuning {
-  int32_t fp_tile_angles;   /* forward: angles per tile of consecutive waves (default 64) */
+  int32_t fp_tile_angles;   /* forward: angles per tile of consecutive waves (default 16) */
   int32_t fp_tile_rays;     /* forward: rays per tile (default 8)                        */
   int32_t fp_zv;            /* forward: z values per lane, 1, 2 or 4 (default 4)         */
   int32_t fp_reuse;         /* forward: 1 = keep the previous sample's taps in registers (default 1) */
   int32_t bp_tile;          /* back projection: tile edge in 2x2 quads of voxel columns (default 8) */
   int32_t bp_zv;            /* back projection: z values per lane (default 4)            */
+  int32_t fp_pair;          /* forward: 1 = project angles a and a + 180 deg in one pass over the
+                               taps when 2 * det_offset is an integer and there is no inclination
+                               or occlusion (default 1); fp_tile_angles then counts angle pairs */
 } vamx_par_tuning;
 int vamx_par_set_tuning(vamx_plan* plan, const vamx_par_tuning* tuning);
 int vamx_par_get_tuning(vamx_plan* plan, vamx_par_tuning* tuning);

@@ -175,7 +175,7 @@ def test_tuning_settings_agree(gpu):
     import oracle.d1 as d1
     from vamtoolbox_amd.parallel import ParallelPlan
     n, nz = 40, 128
-    ang = np.arange(0.0, 360.0, 7.0)
+    ang = np.arange(0.0, 360.0, 6.0)      # opposite angles: the paired candidates pair them
     rng = np.random.default_rng(9)
     vol = rng.uniform(-1, 1, (n, n, nz))
     sino = rng.uniform(0, 1, (n, len(ang), nz))
@@ -183,20 +183,60 @@ def test_tuning_settings_agree(gpu):
     ts = torch.tensor(sino, dtype=torch.float32, device=gpu)
     plan = ParallelPlan(n, nz, ang, device=gpu, mu=0.02)
     ref_f, ref_b = d1.forward(vol, ang, mu=0.02), d1.backward(sino, ang, mu=0.02)
-    for (ta, tk, zv, ru), (tile, bzv) in zip(ParallelPlan.FP_CANDIDATES, itertools.cycle(ParallelPlan.BP_CANDIDATES)):
-        plan.set_tuning(fp_tile_angles=ta, fp_tile_rays=tk, fp_zv=zv, fp_reuse=ru, bp_tile=tile, bp_zv=bzv)
+    for (ta, tk, zv, ru, pr), (tile, bzv) in zip(ParallelPlan.FP_CANDIDATES,
+                                                 itertools.cycle(ParallelPlan.BP_CANDIDATES)):
+        plan.set_tuning(fp_tile_angles=ta, fp_tile_rays=tk, fp_zv=zv, fp_reuse=ru, fp_pair=pr, bp_tile=tile,
+                        bp_zv=bzv)
         e1 = rel_l2(plan.forward(tv).cpu().numpy(), ref_f)
         e2 = rel_l2(plan.backward(ts).cpu().numpy(), ref_b)
-        assert e1 <= TOL and e2 <= TOL, (ta, tk, zv, ru, tile, bzv, e1, e2)
+        assert e1 <= TOL and e2 <= TOL, (ta, tk, zv, ru, pr, tile, bzv, e1, e2)
     # the tile shape changes the order waves run in and tap reuse which
     # instruction loads a tap, not any value
-    plan.set_tuning(fp_tile_angles=32, fp_tile_rays=32, fp_zv=2, fp_reuse=0, bp_tile=16, bp_zv=2)
-    a, b = plan.forward(tv), plan.backward(ts)
-    plan.set_tuning(fp_tile_angles=8, fp_tile_rays=64, fp_reuse=1, bp_tile=7)
-    assert torch.equal(a, plan.forward(tv)) and torch.equal(b, plan.backward(ts))
+    for pr in (0, 1):
+        plan.set_tuning(fp_tile_angles=32, fp_tile_rays=32, fp_zv=2, fp_reuse=0, fp_pair=pr, bp_tile=16, bp_zv=2)
+        a, b = plan.forward(tv), plan.backward(ts)
+        plan.set_tuning(fp_tile_angles=8, fp_tile_rays=64, fp_reuse=1, bp_tile=7)
+        assert torch.equal(a, plan.forward(tv)) and torch.equal(b, plan.backward(ts))
     rep = plan.autotune(tv, ts, reps=1)
-    assert set(rep["choice"]) == {"fp_tile_angles", "fp_tile_rays", "fp_zv", "fp_reuse", "bp_tile", "bp_zv"}
+    assert set(rep["choice"]) == {"fp_tile_angles", "fp_tile_rays", "fp_zv", "fp_reuse", "fp_pair", "bp_tile",
+                                  "bp_zv"}
     assert plan.get_tuning() == rep["choice"]
+
+
+PAIR_CASES = [
+    # n, nz, angles, det_offset, mu, data sign
+    (40, 128, np.arange(0.0, 360.0, 3.0), 0.0, 0.02, "signed"),
+    (41, 64, np.arange(0.0, 360.0, 4.0), 2.5, 0.0, "positive"),     # odd n, half-pixel offset; clipped
+    (41, 64, np.arange(0.0, 360.0, 4.0), 2.5, 0.0, "signed"),       # waves project the rays separately
+    (64, 40, np.arange(0.0, 360.0, 3.0), -3.0, 0.01, "signed"),     # partial z chunk
+    (33, 256, np.concatenate([np.arange(0.0, 360.0, 10.0), [5.0, 95.0]]), 4.0, 0.01, "signed"),  # + single angles
+]
+
+
+@pytest.mark.parametrize("case", range(len(PAIR_CASES)))
+def test_opposite_angle_pairing(gpu, case):
+    """Projecting rays k at angle a and 2c + 2 det_offset - k at a + 180 deg
+    from one pass over the taps (unclipped waves): the result matches the
+    oracle and the unpaired kernel to float32 rounding; clipped waves project
+    the two rays separately, bitwise as unpaired."""
+    import torch
+    import oracle.d1 as d1
+    from vamtoolbox_amd.parallel import ParallelPlan
+    n, nz, ang, off, mu, sign = PAIR_CASES[case]
+    rng = np.random.default_rng(40 + case)
+    vol = rng.uniform(-1, 1, (n, n, nz)) if sign == "signed" else rng.uniform(0.2, 1, (n, n, nz))
+    tv = torch.tensor(vol, dtype=torch.float32, device=gpu)
+    plan = ParallelPlan(n, nz, ang, det_offset=off, mu=mu, device=gpu)
+    assert plan.get_tuning()["fp_pair"] == 1
+    paired = plan.forward(tv).cpu().numpy()
+    plan.set_tuning(fp_pair=0)
+    single = plan.forward(tv).cpu().numpy()
+    ref = d1.forward(vol, ang, det_offset=off, mu=mu)
+    e_ref, e_pair = rel_l2(paired, ref), rel_l2(paired, single)
+    print("case %d: rel L2 vs oracle %.2e, paired vs unpaired %.2e" % (case, e_ref, e_pair))
+    assert e_ref <= TOL and e_pair <= 1e-6
+    if sign == "positive":
+        assert np.array_equal(paired, single)
 
 
 # --------------------------------------------------------------------------

@@ -45,6 +45,7 @@ class ParTuning(ctypes.Structure):
         ("fp_reuse", c_int32),
         ("bp_tile", c_int32),
         ("bp_zv", c_int32),
+        ("fp_pair", c_int32),
     ]
 
 

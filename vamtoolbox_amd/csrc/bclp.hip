@@ -214,6 +214,50 @@ __global__ __launch_bounds__(256) void bclp_fp_update_wave(ParDev g, const float
   vstore<ZV>(gp, gv);
 }
 
+// C, paired: the opposite ray (k2, i2) of a pair is projected from the same
+// taps and updated too
+template <int ZV, bool REUSE>
+__global__ __launch_bounds__(256) void bclp_fp_update_pair(ParDev g, const float* __restrict__ operand,
+                                                           float* __restrict__ gsino,
+                                                           const float* __restrict__ minmax, int clip,
+                                                           const double* __restrict__ scale_p, float lr, float glb,
+                                                           float gub) {
+  using V = typename VecT<ZV>::T;
+  const int lane = threadIdx.x & 63;
+  const long long wave = wave_index(xcd_remap(blockIdx.x, gridDim.x));
+  const PairItem it = fp_pair_item<ZV>(wave, g, lane);
+  if (!it.ok) return;
+  const int k = it.k, i = it.i, k2 = it.k2, i2 = it.i2, z = it.z;
+  const int zl = z < g.nz - ZV ? z : g.nz - ZV;
+  const bool clip_on = clip && wave_clip_active<ZV>(minmax, g.nz, zl);
+  const float scale = (float)(*scale_p);
+  auto update = [&](int kh, int ih, const V& fp) {
+    float* gp = gsino + ((size_t)kh * g.na + ih) * g.nz + z;
+    V gv = vload<ZV>(gp);
+#pragma unroll
+    for (int q = 0; q < ZV; ++q) vset(gv, q, clampf(vget(gv, q) - lr * (scale * vget(fp, q)), glb, gub));
+    vstore<ZV>(gp, gv);
+  };
+  if (clip_on) {
+    // the clipped sample is not continuous in its position: the two rays are
+    // projected one after the other
+    for (int h = 0; h < (k2 >= 0 ? 2 : 1); ++h) {
+      const int kh = h ? k2 : k, ih = h ? i2 : i;
+      const V fp = fp_ray_wave<ZV, true, REUSE>(g, operand, minmax, kh, ih, zl, lane);
+      if (z < g.nz) update(kh, ih, fp);
+    }
+    return;
+  }
+  V fp, fp2 = vzero<ZV>();
+  if (k2 >= 0)
+    fp = fp_ray_wave<ZV, false, REUSE, true>(g, operand, minmax, k, i, zl, lane, &fp2);
+  else
+    fp = fp_ray_wave<ZV, false, REUSE>(g, operand, minmax, k, i, zl, lane);
+  if (z >= g.nz) return;
+  update(k, i, fp);
+  if (k2 >= 0) update(k2, i2, fp2);
+}
+
 // B: fixed-order reduction of the partials, loss log and gradient scale
 __global__ __launch_bounds__(256) void bclp_reduce(const double* __restrict__ partials, int nparts, double dvol,
                                                    double p, double q, double* loss_dst, int32_t* counter,
@@ -417,6 +461,26 @@ extern "C" int vamx_bclp_step(vamx_plan* plan, const vamx_response* resp, const 
       if (rc) return rc;
     }
     const int wz = par_wave_zv(plan, g, operand, gd.fp_zv);
+    if (wz && gd.fp_pair && gd.pair_ok) {
+      const long long threads = fp_pair_tiled(gd) * wave_zchunks(gd.nz, wz) * 64;
+      const dim3 grid(blocks_for(threads, 256)), block(256);
+      const int cl = clip ? 1 : 0;
+#define VAMX_FPP(Z, R)                                                                                 \
+  hipLaunchKernelGGL((bclp_fp_update_pair<Z, R>), grid, block, 0, s, gd, operand, g, plan->d_minmax, cl, scale, \
+                     lr, glb, gub)
+      if (gd.fp_reuse) {
+        if (wz == 4) VAMX_FPP(4, true);
+        else if (wz == 2) VAMX_FPP(2, true);
+        else VAMX_FPP(1, true);
+      } else {
+        if (wz == 4) VAMX_FPP(4, false);
+        else if (wz == 2) VAMX_FPP(2, false);
+        else VAMX_FPP(1, false);
+      }
+#undef VAMX_FPP
+      VAMX_HIP(hipGetLastError());
+      return VAMX_OK;
+    }
     if (wz) {
       const long long threads = fp_tiled_rays(gd) * wave_zchunks(gd.nz, wz) * 64;
       const dim3 grid(blocks_for(threads, 256)), block(256);

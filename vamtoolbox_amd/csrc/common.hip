@@ -37,6 +37,7 @@ extern "C" void vamx_destroy(vamx_plan* p) {
   if (p->d_minmax) hipFree(p->d_minmax);
   if (p->d_occ) hipFree(p->d_occ);
   if (p->d_occ_active) hipFree(p->d_occ_active);
+  if (p->d_units) hipFree(p->d_units);
   if (p->d_rt_tables) hipFree(p->d_rt_tables);
   if (p->d_accum) hipFree(p->d_accum);
   if (p->d_gmax) hipFree(p->d_gmax);

@@ -554,11 +554,29 @@ __device__ __forceinline__ void axis_slots(double coord, int n, int& b, double& 
 // the floor corner b moves by at most one pixel per axis, and when it moves
 // along one axis only, two of the four taps are the previous sample's.  b is
 // wave-uniform, so the case test is a scalar branch.
-template <int ZV, bool CLIP, bool REUSE>
+//
+// PAIR also projects the opposite ray, k2 = 2c + D - k at angle i + 180 deg,
+// into *acc2: its sample r2 lies on sample r = 2c - r2 of ray k (rotating by
+// 180 deg negates the matrix's linear part, and D = 2 det_offset maps the
+// detector coordinate t to -t), so the taps are loaded once and weighted
+// twice.  Ray k's samples are r = 0..n-1, the partner's r = 2c-n+1..2c; the
+// loop runs over their union, each ray weighting only its own samples (the
+// other's extra end sample adds an exact zero).  Each sample is interpolated
+// once and multiplied by each ray's absorption weight (unpaired, the weight
+// folds into the taps' weights: the sums agree to float32 rounding).  The
+// partner's samples are summed in reverse order, at positions computed from
+// angle i's matrix -- equal to the partner's own to ~1e-13 pixel, which moves
+// the result by as little since the unclipped sample is continuous in its
+// position.  The warp clip is not: a sample a
+// rounding error inside the -1 or n border clips up to the slice minimum
+// while one on the border stays 0, so PAIR excludes CLIP and clipped waves
+// project the two rays separately.
+template <int ZV, bool CLIP, bool REUSE, bool PAIR = false>
 __device__ __forceinline__ typename VecT<ZV>::T fp_ray_wave(const ParDev& g, const float* __restrict__ vol,
                                                              const float* __restrict__ minmax, int k, int i, int z,
-                                                             int lane) {
+                                                             int lane, typename VecT<ZV>::T* acc2 = nullptr) {
   using V = typename VecT<ZV>::T;
+  static_assert(!(PAIR && CLIP), "the clipped sample is discontinuous at the border: no pairing");
   const int n = g.n;
   const double* M = g.mats + 6 * i;
   const double M01 = M[1], M02 = M[2], M11 = M[4], M12 = M[5];
@@ -572,15 +590,18 @@ __device__ __forceinline__ typename VecT<ZV>::T fp_ray_wave(const ParDev& g, con
   }
   const size_t nzs = (size_t)g.nz;
   const size_t step_c = nzs, step_r = (size_t)n * nzs;
-  V acc = vzero<ZV>();
+  V acc = vzero<ZV>(), accp = vzero<ZV>();
   V t00 = vzero<ZV>(), t01 = vzero<ZV>(), t10 = vzero<ZV>(), t11 = vzero<ZV>();
   int prev = -2 * n - 2;   // no taps held
-  for (int r0 = 0; r0 < n; r0 += 64) {
+  const int r_end = PAIR ? (2 * g.c + 1 > n ? 2 * g.c + 1 : n) : n;
+  const int r2_lo = 2 * g.c - n + 1;   // partner's first sample (PAIR)
+  for (int r0 = 0; r0 < r_end; r0 += 64) {
     const int r = r0 + lane;
     int base = 0;
     float w00 = 0.f, w01 = 0.f, w10 = 0.f, w11 = 0.f, aw = 0.f;
+    float aw2 = 0.f;
     bool valid = false, anyin = false;
-    if (r < n) {
+    if (r < r_end) {
       const double rr = (double)r;
       const double col = (a0 + M01 * rr) + M02;
       const double row = (a1 + M11 * rr) + M12;
@@ -591,9 +612,14 @@ __device__ __forceinline__ typename VecT<ZV>::T fp_ray_wave(const ParDev& g, con
         axis_slots(col, n, bc, c0w, c1w);
         base = br * n + bc;
         anyin = (r0w > 0.0 || r1w > 0.0) && (c0w > 0.0 || c1w > 0.0);
-        aw = abs_weight(g, t, (double)(r - g.c));
-        valid = anyin && (CLIP || aw != 0.0f);
-        const float f = CLIP ? 1.0f : aw;   // without the clip the absorption folds into the weights
+        if (!PAIR || r < n) aw = abs_weight(g, t, (double)(r - g.c));
+        if constexpr (PAIR) {
+          if (r >= r2_lo) aw2 = abs_weight(g, -t, (double)(g.c - r));
+        }
+        valid = anyin && (CLIP || aw != 0.0f || aw2 != 0.0f);
+        // unpaired without the clip, the absorption folds into the weights;
+        // paired, the sample is interpolated once and weighted per ray
+        const float f = (CLIP || PAIR) ? 1.0f : aw;
         w00 = f * (float)(r0w * c0w);
         w01 = f * (float)(r0w * c1w);
         w10 = f * (float)(r1w * c0w);
@@ -666,11 +692,16 @@ __device__ __forceinline__ typename VecT<ZV>::T fp_ray_wave(const ParDev& g, con
         V v = vfma(q11, t11, vfma(q10, t10, vfma(q01, t01, vmul(q00, t00))));
         v = vclamp(v, lo, hi);  // every enumerated sample has an in-range tap of positive weight
         acc = vfma(rl_f(aw, j), v, acc);
+      } else if constexpr (PAIR) {
+        const V v = vfma(q11, t11, vfma(q10, t10, vfma(q01, t01, vmul(q00, t00))));
+        acc = vfma(rl_f(aw, j), v, acc);
+        accp = vfma(rl_f(aw2, j), v, accp);
       } else {
         acc = vfma(q11, t11, vfma(q10, t10, vfma(q01, t01, vfma(q00, t00, acc))));
       }
     }
   }
+  if constexpr (PAIR) *acc2 = accp;
   return acc;
 }
 
@@ -841,6 +872,54 @@ __device__ __forceinline__ bool fp_wave_item(long long wave, const ParDev& g, in
   return true;
 }
 
+// Paired forward work order: unit u (an angle pair, or a single angle) has
+// n + pair_u extended rays e: e < n is ray e of the primary angle (with its
+// partner ray when that exists), e >= n a partner-angle ray without a
+// primary.  Tiles are fp_ta units x fp_tk extended rays, z chunk slowest.
+__host__ __device__ inline int fp_pair_next(const ParDev& g) { return g.n + g.pair_u; }
+__host__ __device__ inline int fp_pair_ta(const ParDev& g) { return g.fp_ta < g.n_units ? g.fp_ta : g.n_units; }
+__host__ __device__ inline int fp_pair_tk(const ParDev& g) {
+  return g.fp_tk < fp_pair_next(g) ? g.fp_tk : fp_pair_next(g);
+}
+__host__ __device__ inline long long fp_pair_tiled(const ParDev& g) {
+  const int ta = fp_pair_ta(g), tk = fp_pair_tk(g), ne = fp_pair_next(g);
+  return (long long)((g.n_units + ta - 1) / ta) * ((ne + tk - 1) / tk) * ta * tk;
+}
+// wave -> ray (k, i), partner ray (k2, i2) or k2 = -1, and z; ok = false for
+// padding waves
+struct PairItem {
+  int k, i, k2, i2, z;
+  bool ok;
+};
+template <int ZV>
+__host__ __device__ __forceinline__ PairItem fp_pair_item(long long wave, const ParDev& g, int lane) {
+  PairItem it{0, 0, -1, -1, 0, false};
+  const long long tiled = fp_pair_tiled(g);
+  const int nch = wave_zchunks(g.nz, ZV);
+  if (wave >= tiled * nch) return it;
+  const int zc = (int)(wave / tiled);
+  const long long q = wave % tiled;
+  const int ta = fp_pair_ta(g), tk = fp_pair_tk(g), ne = fp_pair_next(g);
+  const int ntk = (ne + tk - 1) / tk;
+  const int tsz = ta * tk;
+  const long long tile = q / tsz;
+  const int within = (int)(q % tsz);
+  const int u = (int)(tile / ntk) * ta + within / tk;
+  const int e = (int)(tile % ntk) * tk + within % tk;
+  if (u >= g.n_units || e >= ne) return it;
+  const int u1 = g.units[2 * u], u2 = g.units[2 * u + 1];
+  const int kk = 2 * g.c + g.pair_D - e;
+  const bool has2 = e < g.n && u2 >= 0 && kk >= 0 && kk < g.n;
+  if (e >= g.n && u2 < 0) return it;
+  it.k = e < g.n ? e : g.pair_first2 + (e - g.n);
+  it.i = e < g.n ? u1 : u2;
+  it.k2 = has2 ? kk : -1;
+  it.i2 = has2 ? u2 : -1;
+  it.z = zc * 64 * ZV + lane * ZV;
+  it.ok = true;
+  return it;
+}
+
 // wave -> (voxel quad, z) of the back-projection kernels; false for padding
 template <int ZV>
 __device__ __forceinline__ bool bp_wave_quad(long long wave, const ParDev& g, int lane, int& r0, int& c0, int& z) {
@@ -874,5 +953,7 @@ int par_slice_minmax(vamx_plan* p, const float* vol, hipStream_t s);
 bool par_use_vec4(const vamx_plan* p, const void* a, const void* b);
 int par_wave_zv(const vamx_plan* p, const void* a, const void* b, int requested);
 int wave_zv_choice(int nz, uintptr_t align_bits, int requested);
+void par_pair_units(const std::vector<double>& cs, int na, int n, int c, double D, std::vector<int>& units,
+                    int& npaired_rays, int& first2);
 
 }  // namespace vamx

@@ -4,6 +4,8 @@
 // owns one ray (forward, B^T) or one voxel column (backward, A^T) and ZV
 // consecutive z values; every output element is written by exactly one lane
 // after a fixed-order accumulation (no atomics, bitwise deterministic).
+#include <algorithm>
+
 #include "par_core.h"
 
 namespace vamx {
@@ -170,6 +172,41 @@ __global__ __launch_bounds__(256) void par_fp_wave_kernel(ParDev g, const float*
   if (z < g.nz) vstore<ZV>(sino + ((size_t)k * g.na + i) * g.nz + z, acc);
 }
 
+// paired forward projection: one wavefront = ray (k, i) and, when it exists,
+// the opposite ray (k2, i2) x 64*ZV z, from one pass over the taps
+template <int ZV, bool REUSE>
+__global__ __launch_bounds__(256) void par_fp_pair_kernel(ParDev g, const float* __restrict__ vol,
+                                                          float* __restrict__ sino, const float* __restrict__ minmax,
+                                                          int clip) {
+  using V = typename VecT<ZV>::T;
+  const int lane = threadIdx.x & 63;
+  const long long wave = wave_index(xcd_remap(blockIdx.x, gridDim.x));
+  const PairItem it = fp_pair_item<ZV>(wave, g, lane);
+  if (!it.ok) return;
+  const int k = it.k, i = it.i, k2 = it.k2, i2 = it.i2, z = it.z;
+  const int zl = z < g.nz - ZV ? z : g.nz - ZV;
+  const bool clip_on = clip && wave_clip_active<ZV>(minmax, g.nz, zl);
+  if (clip_on) {
+    // the clipped sample is not continuous in its position: the two rays are
+    // projected one after the other (one call site keeps the register count
+    // of the unpaired kernel)
+    for (int h = 0; h < (k2 >= 0 ? 2 : 1); ++h) {
+      const int kh = h ? k2 : k, ih = h ? i2 : i;
+      const V acc = fp_ray_wave<ZV, true, REUSE>(g, vol, minmax, kh, ih, zl, lane);
+      if (z < g.nz) vstore<ZV>(sino + ((size_t)kh * g.na + ih) * g.nz + z, acc);
+    }
+    return;
+  }
+  V acc, acc2 = vzero<ZV>();
+  if (k2 >= 0)
+    acc = fp_ray_wave<ZV, false, REUSE, true>(g, vol, minmax, k, i, zl, lane, &acc2);
+  else
+    acc = fp_ray_wave<ZV, false, REUSE>(g, vol, minmax, k, i, zl, lane);
+  if (z >= g.nz) return;
+  vstore<ZV>(sino + ((size_t)k * g.na + i) * g.nz + z, acc);
+  if (k2 >= 0) vstore<ZV>(sino + ((size_t)k2 * g.na + i2) * g.nz + z, acc2);
+}
+
 // wavefront-uniform back projection: one wavefront = one 2x2 quad of voxel
 // columns x 64*ZV z
 template <int ZV>
@@ -311,6 +348,63 @@ bool vamx::par_use_vec4(const vamx_plan* p, const void* a, const void* b) {
   return (p->par.nz % 4 == 0) && !p->par.incl && ((uintptr_t)a % 16 == 0) && ((uintptr_t)b % 16 == 0);
 }
 
+// Opposite-angle work units of the paired forward projection (see ParDev):
+// angle b is angle a + 180 deg when its cos / sin are the negated ones to
+// 1e-9; the detector shift D = 2 det_offset must be an integer.  Units are
+// (primary, partner) with the lower index primary, or (angle, -1); empty when
+// nothing pairs.  npaired_rays rays of a primary angle have a partner ray;
+// the partner angle's rays without one start at first2.
+void vamx::par_pair_units(const std::vector<double>& cs, int na, int n, int c, double D, std::vector<int>& units,
+                          int& npaired_rays, int& first2) {
+  units.clear();
+  npaired_rays = 0;
+  first2 = 0;
+  if (n >= 2 && D == std::nearbyint(D) && std::fabs(D) < n) {
+    const int Di = (int)D;
+    const int lo2 = std::max(0, 2 * c + Di - (n - 1)), hi2 = std::min(n - 1, 2 * c + Di);
+    npaired_rays = hi2 >= lo2 ? hi2 - lo2 + 1 : 0;
+    first2 = lo2 > 0 ? 0 : hi2 + 1;
+    std::vector<int> order(na);
+    std::vector<double> phase(na);
+    for (int i = 0; i < na; ++i) {
+      order[i] = i;
+      phase[i] = std::atan2(cs[2 * i + 1], cs[2 * i]);
+    }
+    std::sort(order.begin(), order.end(), [&](int x, int y) { return phase[x] < phase[y]; });
+    std::vector<int> partner(na, -1);
+    auto opposite = [&](int i, int j) {
+      return std::fabs(cs[2 * i] + cs[2 * j]) < 1e-9 && std::fabs(cs[2 * i + 1] + cs[2 * j + 1]) < 1e-9;
+    };
+    // search the phases within 1e-7 of `want` for an unpaired opposite angle
+    auto find = [&](int i, double want) {
+      auto it = std::lower_bound(order.begin(), order.end(), want - 1e-7,
+                                 [&](int x, double v) { return phase[x] < v; });
+      for (; it != order.end() && phase[*it] <= want + 1e-7; ++it)
+        if (*it != i && partner[*it] < 0 && opposite(i, *it)) return *it;
+      return -1;
+    };
+    for (int i = 0; i < na; ++i) {
+      if (partner[i] >= 0) continue;
+      const double want = phase[i] > 0.0 ? phase[i] - M_PI : phase[i] + M_PI;
+      int j = find(i, want);
+      if (j < 0 && want + M_PI < 1e-6) j = find(i, want + 2 * M_PI);   // near -pi: also look near +pi
+      if (j < 0 && M_PI - want < 1e-6) j = find(i, want - 2 * M_PI);
+      if (j >= 0) {
+        partner[i] = j;
+        partner[j] = i;
+      }
+    }
+    bool any = false;
+    for (int i = 0; i < na; ++i) {
+      if (partner[i] >= 0 && partner[i] < i) continue;   // the primary is the lower index
+      units.push_back(i);
+      units.push_back(partner[i]);
+      any |= partner[i] >= 0;
+    }
+    if (!any || npaired_rays == 0) units.clear();
+  }
+}
+
 extern "C" int vamx_par_create(const vamx_par_geom* geom, vamx_plan** out_plan) {
   if (!geom || !out_plan) return set_error(VAMX_ERR_ARG, "vamx_par_create: null argument");
   *out_plan = nullptr;
@@ -361,8 +455,20 @@ extern "C" int vamx_par_create(const vamx_par_geom* geom, vamx_plan** out_plan) 
   g.incl = geom->incl_deg != 0.0;
   g.mu = geom->mu;
   g.radius = geom->radius > 0.0 ? geom->radius : (n - 1) / 2.0;
+  std::vector<int> units;
+  int npaired_rays = 0, first2 = 0;
+  const double D = 2.0 * geom->det_offset;
+  if (geom->incl_deg == 0.0 && !geom->occlusion)
+    par_pair_units(cs, na, n, c, D, units, npaired_rays, first2);
+  g.pair_ok = units.empty() ? 0 : 1;
+  g.n_units = (int)(units.size() / 2);
+  g.pair_D = (int)D;
+  g.pair_u = n - npaired_rays;
+  g.pair_first2 = first2;
+  g.fp_pair = 1;
   // defaults: ParallelPlan.autotune's choice at 256^3 x 360 angles on MI355X
-  g.fp_ta = 64;
+  // (16 angle pairs x 8 rays, paired)
+  g.fp_ta = 16;
   g.fp_tk = 8;
   g.fp_zv = 4;
   g.fp_reuse = 1;
@@ -378,12 +484,17 @@ extern "C" int vamx_par_create(const vamx_par_geom* geom, vamx_plan** out_plan) 
   if (e == hipSuccess) e = hipMemcpy(p->d_mats, mats.data(), mats.size() * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(p->d_cs, cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(p->d_outside, geom->outside_mask, (size_t)n * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess && !units.empty()) {
+    e = hipMalloc((void**)&p->d_units, units.size() * sizeof(int));
+    if (e == hipSuccess) e = hipMemcpy(p->d_units, units.data(), units.size() * sizeof(int), hipMemcpyHostToDevice);
+  }
   if (e != hipSuccess) {
     vamx_destroy(p);
     return hip_check(e, "vamx_par_create: allocation / upload");
   }
   g.mats = p->d_mats;
   g.cs = p->d_cs;
+  g.units = p->d_units;
   g.outside = p->d_outside;
   g.occ = nullptr;
   g.occ_active = nullptr;
@@ -440,6 +551,24 @@ int par_forward_launch(vamx_plan* p, const float* vol, float* sino, hipStream_t 
     if (rc) return rc;
   }
   const int wz = par_wave_zv(p, vol, sino, g.fp_zv);
+  if (wz && g.fp_pair && g.pair_ok) {
+    const long long threads = fp_pair_tiled(g) * wave_zchunks(g.nz, wz) * 64;
+    const dim3 grid(blocks_for(threads, 256)), block(256);
+    const int clip = warp_clip ? 1 : 0;
+#define VAMX_FPP(Z, R) hipLaunchKernelGGL((par_fp_pair_kernel<Z, R>), grid, block, 0, s, g, vol, sino, p->d_minmax, clip)
+    if (g.fp_reuse) {
+      if (wz == 4) VAMX_FPP(4, true);
+      else if (wz == 2) VAMX_FPP(2, true);
+      else VAMX_FPP(1, true);
+    } else {
+      if (wz == 4) VAMX_FPP(4, false);
+      else if (wz == 2) VAMX_FPP(2, false);
+      else VAMX_FPP(1, false);
+    }
+#undef VAMX_FPP
+    VAMX_HIP(hipGetLastError());
+    return VAMX_OK;
+  }
   if (wz) {
     const long long threads = fp_tiled_rays(g) * wave_zchunks(g.nz, wz) * 64;
     const dim3 grid(blocks_for(threads, 256)), block(256);
@@ -580,15 +709,17 @@ extern "C" int vamx_par_set_tuning(vamx_plan* plan, const vamx_par_tuning* t) {
   if (!t) return set_error(VAMX_ERR_ARG, "vamx_par_set_tuning: null tuning");
   auto pow2 = [](int v) { return v == 1 || v == 2 || v == 4; };
   if (t->fp_tile_angles < 1 || t->fp_tile_rays < 1 || t->bp_tile < 1 || !pow2(t->fp_zv) || !pow2(t->bp_zv) ||
-      t->fp_tile_angles * t->fp_tile_rays > (1 << 20) || t->bp_tile > 1024 || t->fp_reuse < 0 || t->fp_reuse > 1)
+      t->fp_tile_angles * t->fp_tile_rays > (1 << 20) || t->bp_tile > 1024 || t->fp_reuse < 0 || t->fp_reuse > 1 ||
+      t->fp_pair < 0 || t->fp_pair > 1)
     return set_error(VAMX_ERR_ARG,
-                     "vamx_par_set_tuning: tiles must be >= 1, z widths 1, 2 or 4 and fp_reuse 0 or 1");
+                     "vamx_par_set_tuning: tiles must be >= 1, z widths 1, 2 or 4, fp_reuse and fp_pair 0 or 1");
   plan->par.fp_ta = t->fp_tile_angles;
   plan->par.fp_tk = t->fp_tile_rays;
   plan->par.fp_zv = t->fp_zv;
   plan->par.fp_reuse = t->fp_reuse;
   plan->par.bp_tile = t->bp_tile;
   plan->par.bp_zv = t->bp_zv;
+  plan->par.fp_pair = t->fp_pair;
   return VAMX_OK;
 }
 
@@ -601,5 +732,6 @@ extern "C" int vamx_par_get_tuning(vamx_plan* plan, vamx_par_tuning* t) {
   t->fp_reuse = plan->par.fp_reuse;
   t->bp_tile = plan->par.bp_tile;
   t->bp_zv = plan->par.bp_zv;
+  t->fp_pair = plan->par.fp_pair;
   return VAMX_OK;
 }

@@ -34,6 +34,15 @@ struct ParDev {
   int fp_ta, fp_tk, fp_zv;   // forward: tile of fp_ta angles x fp_tk rays, z values per lane
   int fp_reuse;              // forward: keep the previous sample's taps in registers
   int bp_tile, bp_zv;        // back projection: bp_tile x bp_tile voxel columns, z values per lane
+  // Opposite-angle pairing of the forward projection.  Ray k at angle a and
+  // ray 2c + D - k at angle a + 180 deg (D = 2 det_offset, an integer) sample
+  // the same points in reverse order, so one pass over the taps serves both.
+  // Work units are angles: (primary, partner) or (angle, -1).  Of the partner
+  // angle's rays, pair_u have no primary ray; they start at pair_first2.
+  int fp_pair;               // tuning: pair opposite angles when pair_ok
+  int pair_ok;               // the geometry has opposite angles and D is an integer
+  int n_units, pair_D, pair_u, pair_first2;
+  const int* units;          // [n_units][2]
 };
 
 // Geometry of the curved-ray tracer, passed by value.
@@ -74,6 +83,7 @@ struct vamx_plan {
   float* d_minmax = nullptr;        // [2*nz]
   float* d_occ = nullptr;           // [n*na*nz] occlusion sinogram
   uint8_t* d_occ_active = nullptr;  // [na*nz]
+  int* d_units = nullptr;           // [n_units][2] forward work units (opposite-angle pairs)
   int minmax_parts = 0;
   std::vector<double> h_angles;
   // ray tracer

@@ -133,10 +133,11 @@ class ParallelPlan:
         t = _lib.ParTuning(**cur)
         _lib.check(self.lib.vamx_par_set_tuning(self.handle, ctypes.byref(t)), "vamx_par_set_tuning")
 
-    # (angles per tile, rays per tile, z values per lane, register tap reuse)
-    FP_CANDIDATES = [(64, 16, 4, 1), (128, 8, 4, 1), (64, 8, 4, 1), (128, 16, 4, 1), (64, 32, 4, 1), (32, 32, 4, 1),
-                     (16, 64, 4, 1), (64, 16, 2, 1), (32, 32, 2, 1), (64, 16, 4, 0), (32, 32, 4, 0), (16, 32, 4, 0),
-                     (32, 32, 1, 0)]
+    # (angles -- or opposite-angle pairs -- per tile, rays per tile, z values
+    # per lane, register tap reuse, opposite-angle pairing)
+    FP_CANDIDATES = [(16, 8, 4, 1, 1), (8, 16, 4, 1, 1), (16, 16, 4, 1, 1), (32, 8, 4, 1, 1), (64, 8, 4, 1, 1),
+                     (8, 32, 4, 1, 1), (16, 8, 2, 1, 1), (16, 16, 4, 0, 1), (16, 8, 4, 1, 0), (8, 16, 4, 1, 0),
+                     (64, 8, 4, 1, 0), (32, 32, 4, 1, 0), (64, 16, 2, 1, 0), (32, 32, 4, 0, 0), (32, 32, 1, 0, 0)]
     # (tile edge in 2x2 voxel quads, z values per lane)
     BP_CANDIDATES = [(8, 4), (16, 4), (4, 4), (32, 4), (8, 2), (16, 2)]
 
@@ -144,8 +145,10 @@ class ParallelPlan:
         """Time the forward and back projection for each candidate work order
         (tile shape x z values per lane) on this plan's shape and keep the
         fastest of each; returns {setting: time_ms} plus the choice.  Results
-        are cached per (n, nz, n_angles, absorption on/off)."""
-        key = (self.n, self.nz, self.na, self.mu != 0.0, self.incl_deg != 0.0)
+        are cached per (n, nz, angles, absorption on/off, inclination on/off,
+        detector offset) -- the angles and the offset decide the pairing."""
+        key = (self.n, self.nz, self.na, self.mu != 0.0, self.incl_deg != 0.0, self.det_offset,
+               hash(self.angles_deg.tobytes()))
         if key in _TUNING_CACHE:
             self.set_tuning(**_TUNING_CACHE[key]["choice"])
             return _TUNING_CACHE[key]
@@ -168,16 +171,16 @@ class ParallelPlan:
             return a.elapsed_time(b) / reps
 
         report = {"forward": {}, "backward": {}}
-        for ta, tk, zv, ru in self.FP_CANDIDATES:
-            self.set_tuning(fp_tile_angles=ta, fp_tile_rays=tk, fp_zv=zv, fp_reuse=ru)
-            report["forward"][(ta, tk, zv, ru)] = timed(lambda: self.forward(vol, out=out_s))
+        for ta, tk, zv, ru, pr in self.FP_CANDIDATES:
+            self.set_tuning(fp_tile_angles=ta, fp_tile_rays=tk, fp_zv=zv, fp_reuse=ru, fp_pair=pr)
+            report["forward"][(ta, tk, zv, ru, pr)] = timed(lambda: self.forward(vol, out=out_s))
         for tile, zv in self.BP_CANDIDATES:
             self.set_tuning(bp_tile=tile, bp_zv=zv)
             report["backward"][(tile, zv)] = timed(lambda: self.backward(sino, out=out_v))
         fbest = min(report["forward"], key=report["forward"].get)
         bbest = min(report["backward"], key=report["backward"].get)
         choice = dict(fp_tile_angles=fbest[0], fp_tile_rays=fbest[1], fp_zv=fbest[2], fp_reuse=fbest[3],
-                      bp_tile=bbest[0], bp_zv=bbest[1])
+                      fp_pair=fbest[4], bp_tile=bbest[0], bp_zv=bbest[1])
         self.set_tuning(**choice)
         report["choice"] = choice
         _TUNING_CACHE[key] = report
