@@ -35,6 +35,31 @@ template <> __device__ __forceinline__ float vload<1>(const float* p) { return *
 template <> __device__ __forceinline__ float2 vload<2>(const float* p) { return *reinterpret_cast<const float2*>(p); }
 template <> __device__ __forceinline__ float4 vload<4>(const float* p) { return *reinterpret_cast<const float4*>(p); }
 
+// Buffer loads: a wave-uniform base in a buffer descriptor (scalar
+// registers) plus a per-lane byte offset and a scalar byte offset, so the
+// per-load address arithmetic is scalar work.  The descriptor covers
+// `bytes` from the base (raw buffer, no stride); word 3 is the gfx9 raw
+// 32-bit data format.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const float* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)bytes, 0x00020000);
+}
+template <int ZV>
+__device__ __forceinline__ typename VecT<ZV>::T bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff);
+template <>
+__device__ __forceinline__ float bload<1>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+template <>
+__device__ __forceinline__ float2 bload<2>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0);
+  return make_float2(__uint_as_float(v[0]), __uint_as_float(v[1]));
+}
+template <>
+__device__ __forceinline__ float4 bload<4>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0);
+  return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), __uint_as_float(v[3]));
+}
+
 template <int ZV> __device__ __forceinline__ void vstore(float* p, typename VecT<ZV>::T v);
 template <> __device__ __forceinline__ void vstore<1>(float* p, float v) { *p = v; }
 template <> __device__ __forceinline__ void vstore<2>(float* p, float2 v) { *reinterpret_cast<float2*>(p) = v; }
@@ -590,6 +615,12 @@ __device__ __forceinline__ typename VecT<ZV>::T fp_ray_wave(const ParDev& g, con
   }
   const size_t nzs = (size_t)g.nz;
   const size_t step_c = nzs, step_r = (size_t)n * nzs;
+  // taps are buffer loads from a per-sample descriptor at the cell's corner
+  // voxel column: lane byte offset 4z, scalar byte offsets of the other
+  // three corners; the descriptor spans the two rows of the cell
+  const uint32_t zb = 4u * (uint32_t)z;
+  const uint32_t oc = (uint32_t)(4 * step_c), orr = (uint32_t)(4 * step_r), orc = oc + orr;
+  const uint32_t span = orc + oc;
   V acc = vzero<ZV>(), accp = vzero<ZV>();
   V t00 = vzero<ZV>(), t01 = vzero<ZV>(), t10 = vzero<ZV>(), t11 = vzero<ZV>();
   int prev = -2 * n - 2;   // no taps held
@@ -632,61 +663,61 @@ __device__ __forceinline__ typename VecT<ZV>::T fp_ray_wave(const ParDev& g, con
       m &= m - 1;
       const int b = rl_i(base, j);
       const float q00 = rl_f(w00, j), q01 = rl_f(w01, j), q10 = rl_f(w10, j), q11 = rl_f(w11, j);
-      const float* p = vol + (size_t)b * nzs + z;
+      const __amdgpu_buffer_rsrc_t rs = buf_rsrc(vol + (size_t)b * nzs, span);
       if constexpr (REUSE) {
         const int d = b - prev;
         prev = b;
         if (d == 1) {
           t00 = t01;
           t10 = t11;
-          t01 = vload<ZV>(p + step_c);
-          t11 = vload<ZV>(p + step_r + step_c);
+          t01 = bload<ZV>(rs, zb, oc);
+          t11 = bload<ZV>(rs, zb, orc);
         } else if (d == n) {
           t00 = t10;
           t01 = t11;
-          t10 = vload<ZV>(p + step_r);
-          t11 = vload<ZV>(p + step_r + step_c);
+          t10 = bload<ZV>(rs, zb, orr);
+          t11 = bload<ZV>(rs, zb, orc);
         } else if (d == -1) {
           t01 = t00;
           t11 = t10;
-          t00 = vload<ZV>(p);
-          t10 = vload<ZV>(p + step_r);
+          t00 = bload<ZV>(rs, zb, 0);
+          t10 = bload<ZV>(rs, zb, orr);
         } else if (d == -n) {
           t10 = t00;
           t11 = t01;
-          t00 = vload<ZV>(p);
-          t01 = vload<ZV>(p + step_c);
+          t00 = bload<ZV>(rs, zb, 0);
+          t01 = bload<ZV>(rs, zb, oc);
         } else if (d == n + 1) {            // diagonal steps keep one tap
           t00 = t11;
-          t01 = vload<ZV>(p + step_c);
-          t10 = vload<ZV>(p + step_r);
-          t11 = vload<ZV>(p + step_r + step_c);
+          t01 = bload<ZV>(rs, zb, oc);
+          t10 = bload<ZV>(rs, zb, orr);
+          t11 = bload<ZV>(rs, zb, orc);
         } else if (d == n - 1) {
           t01 = t10;
-          t00 = vload<ZV>(p);
-          t10 = vload<ZV>(p + step_r);
-          t11 = vload<ZV>(p + step_r + step_c);
+          t00 = bload<ZV>(rs, zb, 0);
+          t10 = bload<ZV>(rs, zb, orr);
+          t11 = bload<ZV>(rs, zb, orc);
         } else if (d == -n + 1) {
           t10 = t01;
-          t00 = vload<ZV>(p);
-          t01 = vload<ZV>(p + step_c);
-          t11 = vload<ZV>(p + step_r + step_c);
+          t00 = bload<ZV>(rs, zb, 0);
+          t01 = bload<ZV>(rs, zb, oc);
+          t11 = bload<ZV>(rs, zb, orc);
         } else if (d == -n - 1) {
           t11 = t00;
-          t00 = vload<ZV>(p);
-          t01 = vload<ZV>(p + step_c);
-          t10 = vload<ZV>(p + step_r);
+          t00 = bload<ZV>(rs, zb, 0);
+          t01 = bload<ZV>(rs, zb, oc);
+          t10 = bload<ZV>(rs, zb, orr);
         } else if (d != 0) {
-          t00 = vload<ZV>(p);
-          t01 = vload<ZV>(p + step_c);
-          t10 = vload<ZV>(p + step_r);
-          t11 = vload<ZV>(p + step_r + step_c);
+          t00 = bload<ZV>(rs, zb, 0);
+          t01 = bload<ZV>(rs, zb, oc);
+          t10 = bload<ZV>(rs, zb, orr);
+          t11 = bload<ZV>(rs, zb, orc);
         }
       } else {
-        t00 = vload<ZV>(p);
-        t01 = vload<ZV>(p + step_c);
-        t10 = vload<ZV>(p + step_r);
-        t11 = vload<ZV>(p + step_r + step_c);
+        t00 = bload<ZV>(rs, zb, 0);
+        t01 = bload<ZV>(rs, zb, oc);
+        t10 = bload<ZV>(rs, zb, orr);
+        t11 = bload<ZV>(rs, zb, orc);
       }
       if constexpr (CLIP) {
         V v = vfma(q11, t11, vfma(q10, t10, vfma(q01, t01, vmul(q00, t00))));
@@ -777,17 +808,28 @@ __device__ __forceinline__ void bp_quad_wave(const ParDev& g, const float* __res
       V T2 = vzero<ZV>(), T3 = vzero<ZV>();
       if (span >= 1) T2 = vload<ZV>(pb + 2 * step_j);
       if (span >= 2) T3 = vload<ZV>(pb + 3 * step_j);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        if (!((pm >> q) & 1u)) continue;
-        const int off = jq[q] - bmin;
-        if (off == 0)
-          acc[q] = vfma(u1[q], T1, vfma(u0[q], T0, acc[q]));
-        else if (off == 1)
-          acc[q] = vfma(u1[q], T2, vfma(u0[q], T1, acc[q]));
-        else
-          acc[q] = vfma(u1[q], T3, vfma(u0[q], T2, acc[q]));
-      }
+      // one branch per pixel and row offset (the offsets are wave-uniform);
+      // the empty asm markers keep the compiler from turning the branches
+      // into register selects
+#define VAMX_BP_PIXEL(Q)                                                        \
+  if ((pm >> (Q)) & 1u) {                                                      \
+    const int off = jq[Q] - bmin;                                              \
+    if (off == 0) {                                                            \
+      asm volatile("; vamx bp " #Q " 0");                                      \
+      acc[Q] = vfma(u1[Q], T1, vfma(u0[Q], T0, acc[Q]));                       \
+    } else if (off == 1) {                                                     \
+      asm volatile("; vamx bp " #Q " 1");                                      \
+      acc[Q] = vfma(u1[Q], T2, vfma(u0[Q], T1, acc[Q]));                       \
+    } else {                                                                   \
+      asm volatile("; vamx bp " #Q " 2");                                      \
+      acc[Q] = vfma(u1[Q], T3, vfma(u0[Q], T2, acc[Q]));                       \
+    }                                                                          \
+  }
+      VAMX_BP_PIXEL(0)
+      VAMX_BP_PIXEL(1)
+      VAMX_BP_PIXEL(2)
+      VAMX_BP_PIXEL(3)
+#undef VAMX_BP_PIXEL
     }
   }
 }

@@ -417,6 +417,10 @@ extern "C" int vamx_par_create(const vamx_par_geom* geom, vamx_plan** out_plan) 
   if (geom->mu < 0.0) return set_error(VAMX_ERR_ARG, "vamx_par_create: mu must be >= 0");
   if (geom->occlusion && geom->incl_deg != 0.0)
     return set_error(VAMX_ERR_ARG, "vamx_par_create: occlusion shadowing needs zero inclination");
+  // the forward kernels address a cell's two volume rows through 32-bit
+  // buffer offsets
+  if (4.0 * geom->nz * ((double)geom->n + 2) >= 2147483648.0)
+    return set_error(VAMX_ERR_ARG, "vamx_par_create: n * nz too large for 32-bit row offsets");
   vamx_plan* p = new vamx_plan();
   p->kind = PLAN_PARALLEL;
   if (hipGetDevice(&p->device) != hipSuccess) {
