@@ -11,7 +11,10 @@ from tests.conftest import rel_l2
 pytestmark = pytest.mark.gpu
 
 
-def test_batched_equals_individual(gpu):
+@pytest.mark.parametrize("span", [180, 360])
+def test_batched_equals_individual(gpu, span):
+    """Over 360 deg the wavefront forward pairs opposite angles, in the batch
+    as in the single-problem kernel; over 180 deg nothing pairs."""
     import torch
     from vamtoolbox_amd import _lib
     from vamtoolbox_amd.parallel import ParallelPlan
@@ -21,7 +24,7 @@ def test_batched_equals_individual(gpu):
              (32, 64, 40, 0.01), (24, 100, 30, 0.0), (21, 40, 25, 0.02), (18, 256, 12, 0.0)]
     plans, vols, sinos, ref_s, bps, ref_b = [], [], [], [], [], []
     for n, nz, na, mu in specs:
-        ang = np.linspace(0, 180, na, endpoint=False)
+        ang = np.linspace(0, span, na, endpoint=False)
         p = ParallelPlan(n, nz, ang, mu=mu, device=gpu)
         v = torch.tensor(rng.random((n, n, nz)), dtype=torch.float32, device=gpu)
         s = torch.tensor(rng.random((n, na, nz)), dtype=torch.float32, device=gpu)

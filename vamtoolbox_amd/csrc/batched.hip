@@ -3,8 +3,9 @@
 //
 // Each problem is a parallel plan (vamx_par_create) with its own volume and
 // sinogram.  A problem's work is cut into tiles of one 256-thread block:
-// four wavefronts of the wavefront-uniform kernels (one ray or one 2x2 voxel
-// quad and 64 x ZV z values per wavefront, in the plan's L2-aware order) or,
+// four wavefronts of the wavefront-uniform kernels (one ray -- or a ray and
+// its opposite ray, when the plan pairs opposite angles -- or one 2x2 voxel
+// quad, and 64 x ZV z values per wavefront, in the plan's L2-aware order) or,
 // for shapes those do not take (inclination, very thin z), 256 items of the
 // per-lane kernels.  Tiles are ordered largest problem first and handed out
 // by a device-side counter, so the resident blocks of the single launch drain
@@ -29,6 +30,7 @@ struct BatchItem {
   int zv;                // z values per lane: 1, 2, 4 (wave mode) or 1, 4 (per-lane mode)
   int clip;              // forward: warp clip; backward: circle clip
   int wave;              // 1: wavefront-uniform kernels
+  int pair;              // forward, wave mode: opposite-angle pairs (the plan's paired work order)
 };
 
 __global__ void batch_reset(unsigned long long* counter) { *counter = 0ull; }
@@ -43,6 +45,36 @@ __device__ __forceinline__ void batch_fp_wave(const BatchItem& it, long long w, 
   const typename VecT<ZV>::T acc = clip_on ? fp_ray_wave<ZV, true, true>(g, it.in, it.minmax, k, i, zl, lane)
                                            : fp_ray_wave<ZV, false, true>(g, it.in, it.minmax, k, i, zl, lane);
   if (z < g.nz) vstore<ZV>(it.out + ((size_t)k * g.na + i) * g.nz + z, acc);
+}
+
+// paired forward: ray (k, i) and its opposite ray from one pass over the
+// taps, as the single-problem paired kernel (clipped waves project the two
+// rays one after the other)
+template <int ZV>
+__device__ __forceinline__ void batch_fp_pair(const BatchItem& it, long long w, int lane) {
+  using V = typename VecT<ZV>::T;
+  const ParDev& g = it.g;
+  const PairItem pi = fp_pair_item<ZV>(w, g, lane);
+  if (!pi.ok) return;
+  const int z = pi.z;
+  const int zl = z < g.nz - ZV ? z : g.nz - ZV;
+  const bool clip_on = it.clip && wave_clip_active<ZV>(it.minmax, g.nz, zl);
+  if (clip_on) {
+    for (int h = 0; h < (pi.k2 >= 0 ? 2 : 1); ++h) {
+      const int kh = h ? pi.k2 : pi.k, ih = h ? pi.i2 : pi.i;
+      const V acc = fp_ray_wave<ZV, true, true>(g, it.in, it.minmax, kh, ih, zl, lane);
+      if (z < g.nz) vstore<ZV>(it.out + ((size_t)kh * g.na + ih) * g.nz + z, acc);
+    }
+    return;
+  }
+  V acc, acc2 = vzero<ZV>();
+  if (pi.k2 >= 0)
+    acc = fp_ray_wave<ZV, false, true, true>(g, it.in, it.minmax, pi.k, pi.i, zl, lane, &acc2);
+  else
+    acc = fp_ray_wave<ZV, false, true>(g, it.in, it.minmax, pi.k, pi.i, zl, lane);
+  if (z >= g.nz) return;
+  vstore<ZV>(it.out + ((size_t)pi.k * g.na + pi.i) * g.nz + z, acc);
+  if (pi.k2 >= 0) vstore<ZV>(it.out + ((size_t)pi.k2 * g.na + pi.i2) * g.nz + z, acc2);
 }
 
 template <int ZV>
@@ -126,7 +158,11 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchItem* __restrict_
     if (it.wave) {
       const long long w = (tile - it.tile_begin) * 4 + slot;
       if (w < it.items) {
-        if (FORWARD) {
+        if (FORWARD && it.pair) {
+          if (it.zv == 4) batch_fp_pair<4>(it, w, lane);
+          else if (it.zv == 2) batch_fp_pair<2>(it, w, lane);
+          else batch_fp_pair<1>(it, w, lane);
+        } else if (FORWARD) {
           if (it.zv == 4) batch_fp_wave<4>(it, w, lane);
           else if (it.zv == 2) batch_fp_wave<2>(it, w, lane);
           else batch_fp_wave<1>(it, w, lane);
@@ -184,10 +220,13 @@ int run_batched(vamx_plan* const* plans, int n, const float* const* ins, float* 
     it.minmax = p->d_minmax;
     it.clip = forward ? !(p->flags & VAMX_PAR_NO_WARP_CLIP) : !(p->flags & VAMX_PAR_NO_CIRCLE_CLIP);
     const int wz = par_wave_zv(p, ins[i], outs[i], forward ? it.g.fp_zv : it.g.bp_zv);
+    it.pair = 0;
     if (wz) {
       it.wave = 1;
       it.zv = wz;
-      it.items = (forward ? fp_tiled_rays(it.g) : bp_tiled_quads(it.g)) * wave_zchunks(it.g.nz, wz);
+      it.pair = forward && it.g.fp_pair && it.g.pair_ok ? 1 : 0;
+      const long long units = it.pair ? fp_pair_tiled(it.g) : forward ? fp_tiled_rays(it.g) : bp_tiled_quads(it.g);
+      it.items = units * wave_zchunks(it.g.nz, wz);
     } else {
       it.wave = 0;
       it.zv = par_use_vec4(p, ins[i], outs[i]) ? 4 : 1;
