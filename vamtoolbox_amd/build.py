@@ -28,6 +28,11 @@ FLAGS = [
     "-ffp-contract=off",
     # no packed FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # branches on wave-uniform values (the forward's tap-reuse cases, the back
+    # projection's row offsets) need no structurizing: left as they are, the
+    # register allocator keeps the taps in place instead of copying them
+    # through the structurizer's flow blocks
+    "-mllvm", "-structurizecfg-skip-uniform-regions=true",
     "-Wno-unused-result",
 ]
 

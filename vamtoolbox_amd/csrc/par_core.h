@@ -622,7 +622,13 @@ __device__ __forceinline__ typename VecT<ZV>::T fp_ray_wave(const ParDev& g, con
   const uint32_t oc = (uint32_t)(4 * step_c), orr = (uint32_t)(4 * step_r), orc = oc + orr;
   const uint32_t span = orc + oc;
   V acc = vzero<ZV>(), accp = vzero<ZV>();
-  V t00 = vzero<ZV>(), t01 = vzero<ZV>(), t10 = vzero<ZV>(), t11 = vzero<ZV>();
+  // Tap registers by absolute parity: T<a><b> holds the tap of the current
+  // cell whose volume row is even (a = 0) or odd (a = 1) and whose column is
+  // even (b = 0) or odd (b = 1).  When the cell moves by one row or column
+  // the taps it keeps stay where they are and the new ones replace exactly
+  // the ones that left (same parity), so the window never shifts between
+  // registers; each lane delivers its weights in the same parity order.
+  V T00 = vzero<ZV>(), T01 = vzero<ZV>(), T10 = vzero<ZV>(), T11 = vzero<ZV>();
   int prev = -2 * n - 2;   // no taps held
   const int r_end = PAIR ? (2 * g.c + 1 > n ? 2 * g.c + 1 : n) : n;
   const int r2_lo = 2 * g.c - n + 1;   // partner's first sample (PAIR)
@@ -641,7 +647,7 @@ __device__ __forceinline__ typename VecT<ZV>::T fp_ray_wave(const ParDev& g, con
         double r0w, r1w, c0w, c1w;
         axis_slots(row, n, br, r0w, r1w);
         axis_slots(col, n, bc, c0w, c1w);
-        base = br * n + bc;
+        base = (br * n + bc) | ((br & 1) << 29) | ((bc & 1) << 28);   // cell and its row / column parity
         anyin = (r0w > 0.0 || r1w > 0.0) && (c0w > 0.0 || c1w > 0.0);
         if (!PAIR || r < n) aw = abs_weight(g, t, (double)(r - g.c));
         if constexpr (PAIR) {
@@ -651,84 +657,91 @@ __device__ __forceinline__ typename VecT<ZV>::T fp_ray_wave(const ParDev& g, con
         // unpaired without the clip, the absorption folds into the weights;
         // paired, the sample is interpolated once and weighted per ray
         const float f = (CLIP || PAIR) ? 1.0f : aw;
-        w00 = f * (float)(r0w * c0w);
-        w01 = f * (float)(r0w * c1w);
-        w10 = f * (float)(r1w * c0w);
-        w11 = f * (float)(r1w * c1w);
+        const float k00 = f * (float)(r0w * c0w), k01 = f * (float)(r0w * c1w);
+        const float k10 = f * (float)(r1w * c0w), k11 = f * (float)(r1w * c1w);
+        // parity (a, b) holds corner (a ^ row parity, b ^ column parity)
+        const bool pr = br & 1, pc = bc & 1;
+        const float e0 = pc ? k01 : k00, e1 = pc ? k00 : k01;   // row 0 by column parity
+        const float f0 = pc ? k11 : k10, f1 = pc ? k10 : k11;   // row 1
+        w00 = pr ? f0 : e0;
+        w01 = pr ? f1 : e1;
+        w10 = pr ? e0 : f0;
+        w11 = pr ? e1 : f1;
       }
     }
     unsigned long long m = __ballot(valid);
     while (m) {
       const int j = __builtin_ctzll(m);
       m &= m - 1;
-      const int b = rl_i(base, j);
+      const int bp = rl_i(base, j);
+      const int b = bp & 0x0fffffff, par = (bp >> 28) & 3;   // par: row parity << 1 | column parity
       const float q00 = rl_f(w00, j), q01 = rl_f(w01, j), q10 = rl_f(w10, j), q11 = rl_f(w11, j);
       const __amdgpu_buffer_rsrc_t rs = buf_rsrc(vol + (size_t)b * nzs, span);
-      if constexpr (REUSE) {
-        const int d = b - prev;
-        prev = b;
-        if (d == 1) {
-          t00 = t01;
-          t10 = t11;
-          t01 = bload<ZV>(rs, zb, oc);
-          t11 = bload<ZV>(rs, zb, orc);
-        } else if (d == n) {
-          t00 = t10;
-          t01 = t11;
-          t10 = bload<ZV>(rs, zb, orr);
-          t11 = bload<ZV>(rs, zb, orc);
-        } else if (d == -1) {
-          t01 = t00;
-          t11 = t10;
-          t00 = bload<ZV>(rs, zb, 0);
-          t10 = bload<ZV>(rs, zb, orr);
-        } else if (d == -n) {
-          t10 = t00;
-          t11 = t01;
-          t00 = bload<ZV>(rs, zb, 0);
-          t01 = bload<ZV>(rs, zb, oc);
-        } else if (d == n + 1) {            // diagonal steps keep one tap
-          t00 = t11;
-          t01 = bload<ZV>(rs, zb, oc);
-          t10 = bload<ZV>(rs, zb, orr);
-          t11 = bload<ZV>(rs, zb, orc);
-        } else if (d == n - 1) {
-          t01 = t10;
-          t00 = bload<ZV>(rs, zb, 0);
-          t10 = bload<ZV>(rs, zb, orr);
-          t11 = bload<ZV>(rs, zb, orc);
-        } else if (d == -n + 1) {
-          t10 = t01;
-          t00 = bload<ZV>(rs, zb, 0);
-          t01 = bload<ZV>(rs, zb, oc);
-          t11 = bload<ZV>(rs, zb, orc);
-        } else if (d == -n - 1) {
-          t11 = t00;
-          t00 = bload<ZV>(rs, zb, 0);
-          t01 = bload<ZV>(rs, zb, oc);
-          t10 = bload<ZV>(rs, zb, orr);
-        } else if (d != 0) {
-          t00 = bload<ZV>(rs, zb, 0);
-          t01 = bload<ZV>(rs, zb, oc);
-          t10 = bload<ZV>(rs, zb, orr);
-          t11 = bload<ZV>(rs, zb, orc);
-        }
-      } else {
-        t00 = bload<ZV>(rs, zb, 0);
-        t01 = bload<ZV>(rs, zb, oc);
-        t10 = bload<ZV>(rs, zb, orr);
-        t11 = bload<ZV>(rs, zb, orc);
+      // C00..C11 name the registers of the cell's corners (0,0) .. (1,1) for
+      // this parity; each arm loads the corners the previous cell lacked
+#define VAMX_TAP_ARMS(C00, C01, C10, C11, P)                                       \
+  if (d == 1) {             /* one column right: the right column is new */      \
+    asm volatile("; vamx fp " #P " d1");                                        \
+    C01 = bload<ZV>(rs, zb, oc);                                                 \
+    C11 = bload<ZV>(rs, zb, orc);                                                \
+  } else if (d == n) {      /* one row down: the bottom row is new */           \
+    asm volatile("; vamx fp " #P " dn");                                        \
+    C10 = bload<ZV>(rs, zb, orr);                                                \
+    C11 = bload<ZV>(rs, zb, orc);                                                \
+  } else if (d == -1) {                                                         \
+    asm volatile("; vamx fp " #P " dm1");                                       \
+    C00 = bload<ZV>(rs, zb, 0);                                                  \
+    C10 = bload<ZV>(rs, zb, orr);                                                \
+  } else if (d == -n) {                                                         \
+    asm volatile("; vamx fp " #P " dmn");                                       \
+    C00 = bload<ZV>(rs, zb, 0);                                                  \
+    C01 = bload<ZV>(rs, zb, oc);                                                 \
+  } else if (d == n + 1) {  /* diagonal steps keep one tap */                   \
+    asm volatile("; vamx fp " #P " dnp1");                                      \
+    C01 = bload<ZV>(rs, zb, oc);                                                 \
+    C10 = bload<ZV>(rs, zb, orr);                                                \
+    C11 = bload<ZV>(rs, zb, orc);                                                \
+  } else if (d == n - 1) {                                                      \
+    asm volatile("; vamx fp " #P " dnm1");                                      \
+    C00 = bload<ZV>(rs, zb, 0);                                                  \
+    C10 = bload<ZV>(rs, zb, orr);                                                \
+    C11 = bload<ZV>(rs, zb, orc);                                                \
+  } else if (d == -n + 1) {                                                     \
+    asm volatile("; vamx fp " #P " dmnp1");                                     \
+    C00 = bload<ZV>(rs, zb, 0);                                                  \
+    C01 = bload<ZV>(rs, zb, oc);                                                 \
+    C11 = bload<ZV>(rs, zb, orc);                                                \
+  } else if (d == -n - 1) {                                                     \
+    asm volatile("; vamx fp " #P " dmnm1");                                     \
+    C00 = bload<ZV>(rs, zb, 0);                                                  \
+    C01 = bload<ZV>(rs, zb, oc);                                                 \
+    C10 = bload<ZV>(rs, zb, orr);                                                \
+  } else if (d != 0) {                                                          \
+    asm volatile("; vamx fp " #P " full");                                      \
+    C00 = bload<ZV>(rs, zb, 0);                                                  \
+    C01 = bload<ZV>(rs, zb, oc);                                                 \
+    C10 = bload<ZV>(rs, zb, orr);                                                \
+    C11 = bload<ZV>(rs, zb, orc);                                                \
+  }
+      const int d = REUSE ? b - prev : 2 * n + 2;   // without reuse every sample reloads
+      prev = b;
+      switch (par) {
+        case 0: { VAMX_TAP_ARMS(T00, T01, T10, T11, 0) break; }
+        case 1: { VAMX_TAP_ARMS(T01, T00, T11, T10, 1) break; }
+        case 2: { VAMX_TAP_ARMS(T10, T11, T00, T01, 2) break; }
+        default: { VAMX_TAP_ARMS(T11, T10, T01, T00, 3) break; }
       }
+#undef VAMX_TAP_ARMS
       if constexpr (CLIP) {
-        V v = vfma(q11, t11, vfma(q10, t10, vfma(q01, t01, vmul(q00, t00))));
+        V v = vfma(q11, T11, vfma(q10, T10, vfma(q01, T01, vmul(q00, T00))));
         v = vclamp(v, lo, hi);  // every enumerated sample has an in-range tap of positive weight
         acc = vfma(rl_f(aw, j), v, acc);
       } else if constexpr (PAIR) {
-        const V v = vfma(q11, t11, vfma(q10, t10, vfma(q01, t01, vmul(q00, t00))));
+        const V v = vfma(q11, T11, vfma(q10, T10, vfma(q01, T01, vmul(q00, T00))));
         acc = vfma(rl_f(aw, j), v, acc);
         accp = vfma(rl_f(aw2, j), v, accp);
       } else {
-        acc = vfma(q11, t11, vfma(q10, t10, vfma(q01, t01, vfma(q00, t00, acc))));
+        acc = vfma(q11, T11, vfma(q10, T10, vfma(q01, T01, vfma(q00, T00, acc))));
       }
     }
   }
@@ -783,27 +796,32 @@ __device__ __forceinline__ void bp_quad_wave(const ParDev& g, const float* __res
     }
     const unsigned long long m = __ballot(valid);
     unsigned long long am = (m | (m >> 1) | (m >> 2) | (m >> 3)) & 0x1111111111111111ull;
+    // the quad's row range over its valid pixels (the four lanes of one angle
+    // are lanes 4a .. 4a+3), each pixel's row offset in it and the element
+    // offset of its first row, (bmin * na + i) * nz -- lane-parallel for 16
+    // angles at once, so the per-angle loop below only broadcasts them
+    int lo = valid ? jr : 0x7fffffff, hi = valid ? jr : -1;
+    lo = min(lo, __shfl_xor(lo, 1));
+    lo = min(lo, __shfl_xor(lo, 2));
+    hi = max(hi, __shfl_xor(hi, 1));
+    hi = max(hi, __shfl_xor(hi, 2));
+    const int offl = jr - lo, spanl = hi - lo;
+    const unsigned rowl = ((unsigned)lo * (unsigned)g.na + (unsigned)i) * (unsigned)g.nz;
     while (am) {
       const int l0 = __builtin_ctzll(am);   // lane of pixel 0 of this angle
       am &= am - 1;
       const unsigned pm = (unsigned)(m >> l0) & 0xFu;
-      const int ia = i0 + (l0 >> 2);
       int jq[4];
       float u0[4], u1[4];
-      int bmin = g.n, bmax = -1;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        jq[q] = rl_i(jr, l0 + q);
+        jq[q] = rl_i(offl, l0 + q);
         u0[q] = rl_f(w0, l0 + q);
         u1[q] = rl_f(w1, l0 + q);
-        if ((pm >> q) & 1u) {
-          bmin = jq[q] < bmin ? jq[q] : bmin;
-          bmax = jq[q] > bmax ? jq[q] : bmax;
-        }
       }
       // rows bmin .. bmax + 1 <= n - 1 (jr <= n - 2); span <= 2 by the bound above
-      const float* pb = sino + ((size_t)bmin * g.na + ia) * nzs + z;
-      const int span = bmax - bmin;
+      const float* pb = sino + (size_t)(unsigned)rl_i((int)rowl, l0) + z;
+      const int span = rl_i(spanl, l0);
       const V T0 = vload<ZV>(pb), T1 = vload<ZV>(pb + step_j);
       V T2 = vzero<ZV>(), T3 = vzero<ZV>();
       if (span >= 1) T2 = vload<ZV>(pb + 2 * step_j);
@@ -813,7 +831,7 @@ __device__ __forceinline__ void bp_quad_wave(const ParDev& g, const float* __res
       // into register selects
 #define VAMX_BP_PIXEL(Q)                                                        \
   if ((pm >> (Q)) & 1u) {                                                      \
-    const int off = jq[Q] - bmin;                                              \
+    const int off = jq[Q];                                                     \
     if (off == 0) {                                                            \
       asm volatile("; vamx bp " #Q " 0");                                      \
       acc[Q] = vfma(u1[Q], T1, vfma(u0[Q], T0, acc[Q]));                       \

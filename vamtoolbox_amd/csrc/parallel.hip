@@ -421,6 +421,11 @@ extern "C" int vamx_par_create(const vamx_par_geom* geom, vamx_plan** out_plan) 
   // buffer offsets
   if (4.0 * geom->nz * ((double)geom->n + 2) >= 2147483648.0)
     return set_error(VAMX_ERR_ARG, "vamx_par_create: n * nz too large for 32-bit row offsets");
+  // the forward kernels pack a cell index below 2^28 with its parity bits
+  if (geom->n >= 16384) return set_error(VAMX_ERR_ARG, "vamx_par_create: n must be < 16384");
+  // the back projection's sinogram row offsets are 32-bit element counts
+  if ((double)geom->n * geom->n_angles * geom->nz >= 4294967296.0)
+    return set_error(VAMX_ERR_ARG, "vamx_par_create: n * n_angles * nz too large for 32-bit sinogram offsets");
   vamx_plan* p = new vamx_plan();
   p->kind = PLAN_PARALLEL;
   if (hipGetDevice(&p->device) != hipSuccess) {
@@ -471,8 +476,8 @@ extern "C" int vamx_par_create(const vamx_par_geom* geom, vamx_plan** out_plan) 
   g.pair_first2 = first2;
   g.fp_pair = 1;
   // defaults: ParallelPlan.autotune's choice at 256^3 x 360 angles on MI355X
-  // (16 angle pairs x 8 rays, paired)
-  g.fp_ta = 16;
+  // (8 angle pairs x 8 rays, paired)
+  g.fp_ta = 8;
   g.fp_tk = 8;
   g.fp_zv = 4;
   g.fp_reuse = 1;
