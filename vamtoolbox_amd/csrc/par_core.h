@@ -759,7 +759,8 @@ __device__ __forceinline__ typename VecT<ZV>::T fp_ray_wave(const ParDev& g, con
 // the float64 geometry of 16 angles x 4 pixels; the per-angle loop then
 // loads the rows [bmin, bmax + 1] once and every pixel accumulates
 // fma(w1, b[j+1], fma(w0, b[j], acc)) in angle order -- the same operations,
-// in the same order, as one column at a time.  Requires n >= 2.
+// in the same order, as one column at a time.  Requires n >= 2 and all 64
+// lanes of the wavefront (cross-lane swizzles).
 template <int ZV>
 __device__ __forceinline__ void bp_quad_wave(const ParDev& g, const float* __restrict__ sino, int r0, int c0,
                                              unsigned active, int z, int lane, typename VecT<ZV>::T acc[4]) {
