@@ -770,6 +770,7 @@ __device__ __forceinline__ void bp_quad_wave(const ParDev& g, const float* __res
   const double x = (double)(c0 + (p & 1) - g.c), y = (double)(r0 + (p >> 1) - g.c);
   const size_t nzs = (size_t)g.nz;
   const size_t step_j = (size_t)g.na * nzs;
+  const uint32_t zb = 4u * (uint32_t)z, row_b = (uint32_t)(4 * step_j), rows_b = 3 * row_b + 4u * (uint32_t)g.nz;
 #pragma unroll
   for (int q = 0; q < 4; ++q) acc[q] = vzero<ZV>();
   for (int i0 = 0; i0 < g.na; i0 += 16) {
@@ -821,12 +822,14 @@ __device__ __forceinline__ void bp_quad_wave(const ParDev& g, const float* __res
         u1[q] = rl_f(w1, l0 + q);
       }
       // rows bmin .. bmax + 1 <= n - 1 (jr <= n - 2); span <= 2 by the bound above
-      const float* pb = sino + (size_t)(unsigned)rl_i((int)rowl, l0) + z;
+      // buffer loads from a descriptor at row bmin: lane byte offset 4z,
+      // scalar row offsets
+      const __amdgpu_buffer_rsrc_t rs = buf_rsrc(sino + (size_t)(unsigned)rl_i((int)rowl, l0), rows_b);
       const int span = rl_i(spanl, l0);
-      const V T0 = vload<ZV>(pb), T1 = vload<ZV>(pb + step_j);
+      const V T0 = bload<ZV>(rs, zb, 0), T1 = bload<ZV>(rs, zb, row_b);
       V T2 = vzero<ZV>(), T3 = vzero<ZV>();
-      if (span >= 1) T2 = vload<ZV>(pb + 2 * step_j);
-      if (span >= 2) T3 = vload<ZV>(pb + 3 * step_j);
+      if (span >= 1) T2 = bload<ZV>(rs, zb, 2 * row_b);
+      if (span >= 2) T3 = bload<ZV>(rs, zb, 3 * row_b);
       // one branch per pixel and row offset (the offsets are wave-uniform);
       // the empty asm markers keep the compiler from turning the branches
       // into register selects

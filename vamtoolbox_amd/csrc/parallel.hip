@@ -417,10 +417,11 @@ extern "C" int vamx_par_create(const vamx_par_geom* geom, vamx_plan** out_plan) 
   if (geom->mu < 0.0) return set_error(VAMX_ERR_ARG, "vamx_par_create: mu must be >= 0");
   if (geom->occlusion && geom->incl_deg != 0.0)
     return set_error(VAMX_ERR_ARG, "vamx_par_create: occlusion shadowing needs zero inclination");
-  // the forward kernels address a cell's two volume rows through 32-bit
-  // buffer offsets
-  if (4.0 * geom->nz * ((double)geom->n + 2) >= 2147483648.0)
-    return set_error(VAMX_ERR_ARG, "vamx_par_create: n * nz too large for 32-bit row offsets");
+  // the wavefront kernels address a cell's two volume rows, and a voxel
+  // quad's four sinogram rows, through 32-bit buffer offsets
+  if (4.0 * geom->nz * ((double)geom->n + 2) >= 2147483648.0 ||
+      4.0 * geom->nz * (3.0 * geom->n_angles + 1) >= 2147483648.0)
+    return set_error(VAMX_ERR_ARG, "vamx_par_create: n * nz or n_angles * nz too large for 32-bit row offsets");
   // the forward kernels pack a cell index below 2^28 with its parity bits
   if (geom->n >= 16384) return set_error(VAMX_ERR_ARG, "vamx_par_create: n must be < 16384");
   // the back projection's sinogram row offsets are 32-bit element counts
