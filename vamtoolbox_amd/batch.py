@@ -93,7 +93,8 @@ class ParallelBatch:
 
     # -- operators -----------------------------------------------------------
     def forward(self, vol_flat: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
-        """Sinograms of every problem (one persistent launch)."""
+        """Sinograms of every problem (one persistent launch; problems that fill
+        the GPU alone get their own forward launch)."""
         self._check(vol_flat, self.n_vol, "vol_flat")
         out = self.empty_sino() if out is None else out
         self._check(out, self.n_sino, "out")

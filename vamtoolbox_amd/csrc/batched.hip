@@ -1,5 +1,7 @@
 // Batched parallel-beam projection of many independent problems of uneven
-// size in one persistent launch (gfx950).
+// size in one persistent launch (gfx950); in the forward direction, problems
+// large enough to fill the GPU on their own get single-problem launches
+// instead (the same kernels and results as vamx_par_forward).
 //
 // Each problem is a parallel plan (vamx_par_create) with its own volume and
 // sinogram.  A problem's work is cut into tiles of one 256-thread block:
@@ -206,14 +208,42 @@ int run_batched(vamx_plan* const* plans, int n, const float* const* ins, float* 
   if (!plans || n < 1 || !ins || !outs) return set_error(VAMX_ERR_ARG, "batched: empty or null batch");
   int dev = 0;
   VAMX_HIP(hipGetDevice(&dev));
-  std::vector<BatchItem> items((size_t)n);
-  std::vector<int> order((size_t)n);
+  int cus = 0;
+  VAMX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   for (int i = 0; i < n; ++i) {
     vamx_plan* p = plans[i];
     if (!check_plan(p, PLAN_PARALLEL)) return set_error(VAMX_ERR_STATE, "batched: not a parallel plan");
     if (p->device != dev) return set_error(VAMX_ERR_ARG, "batched: plans must live on the current device");
     if (!ins[i] || !outs[i]) return set_error(VAMX_ERR_ARG, "batched: null buffer");
-    BatchItem& it = items[(size_t)i];
+  }
+  // Forward: a problem whose wavefront work alone fills the GPU (16 waves per
+  // CU) runs as its own single-problem launch; the persistent kernel, whose
+  // registers cover every z width and the per-lane fallback, reaches a lower
+  // occupancy than the single-problem forward kernels.  The rest share the
+  // persistent launch.
+  std::vector<int> shared;
+  for (int i = 0; i < n; ++i) {
+    vamx_plan* p = plans[i];
+    if (forward && !p->par.incl) {
+      const int wz = par_wave_zv(p, ins[i], outs[i], p->par.fp_zv);
+      const long long units =
+          p->par.fp_pair && p->par.pair_ok ? fp_pair_tiled(p->par) : fp_tiled_rays(p->par);
+      if (wz && units * wave_zchunks(p->par.nz, wz) >= 16ll * cus) {
+        int rc = par_forward_launch(p, ins[i], outs[i], s, !(p->flags & VAMX_PAR_NO_WARP_CLIP));
+        if (rc) return rc;
+        continue;
+      }
+    }
+    shared.push_back(i);
+  }
+  if (shared.empty()) return VAMX_OK;
+  const int m = (int)shared.size();
+  std::vector<BatchItem> items((size_t)m);
+  std::vector<int> order((size_t)m);
+  for (int si = 0; si < m; ++si) {
+    const int i = shared[(size_t)si];
+    vamx_plan* p = plans[i];
+    BatchItem& it = items[(size_t)si];
     it.g = p->par;
     it.in = ins[i];
     it.out = outs[i];
@@ -245,9 +275,9 @@ int run_batched(vamx_plan* const* plans, int n, const float* const* ins, float* 
     return (double)it.g.n * it.g.n * it.g.na * it.g.nz;
   };
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
-  std::vector<BatchItem> sorted((size_t)n);
+  std::vector<BatchItem> sorted((size_t)m);
   long long tiles = 0;
-  for (int r = 0; r < n; ++r) {
+  for (int r = 0; r < m; ++r) {
     sorted[(size_t)r] = items[(size_t)order[(size_t)r]];
     sorted[(size_t)r].tile_begin = tiles;
     tiles += (sorted[(size_t)r].items + (sorted[(size_t)r].wave ? 3 : 255)) / (sorted[(size_t)r].wave ? 4 : 256);
@@ -261,7 +291,7 @@ int run_batched(vamx_plan* const* plans, int n, const float* const* ins, float* 
     VAMX_HIP(hipMalloc((void**)&sc.d_counter, sizeof(unsigned long long)));
     sc.device = dev;
   }
-  const size_t bytes = (size_t)n * sizeof(BatchItem);
+  const size_t bytes = (size_t)m * sizeof(BatchItem);
   const char* raw = reinterpret_cast<const char*>(sorted.data());
   BatchItem* d_items = nullptr;
   for (size_t t = 0; t < sc.tables.size(); ++t) {
@@ -293,15 +323,13 @@ int run_batched(vamx_plan* const* plans, int n, const float* const* ins, float* 
     d_items = sc.tables.back().d_items;
   }
   hipLaunchKernelGGL(batch_reset, dim3(1), dim3(1), 0, s, sc.d_counter);
-  int cus = 0;
-  VAMX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   long long grid = (long long)std::max(cus, 1) * 8;
   if (grid > tiles) grid = tiles;
   if (forward)
-    hipLaunchKernelGGL(batch_kernel<true>, dim3((unsigned)grid), dim3(256), 0, s, d_items, n, tiles,
+    hipLaunchKernelGGL(batch_kernel<true>, dim3((unsigned)grid), dim3(256), 0, s, d_items, m, tiles,
                        sc.d_counter);
   else
-    hipLaunchKernelGGL(batch_kernel<false>, dim3((unsigned)grid), dim3(256), 0, s, d_items, n, tiles,
+    hipLaunchKernelGGL(batch_kernel<false>, dim3((unsigned)grid), dim3(256), 0, s, d_items, m, tiles,
                        sc.d_counter);
   VAMX_HIP(hipGetLastError());
   return VAMX_OK;

@@ -1014,6 +1014,7 @@ __device__ __forceinline__ void bp_quad_masks(const ParDev& g, int r0, int c0, i
 
 // host helpers implemented in parallel.hip
 int par_slice_minmax(vamx_plan* p, const float* vol, hipStream_t s);
+int par_forward_launch(vamx_plan* p, const float* vol, float* sino, hipStream_t s, bool warp_clip);
 bool par_use_vec4(const vamx_plan* p, const void* a, const void* b);
 int par_wave_zv(const vamx_plan* p, const void* a, const void* b, int requested);
 int wave_zv_choice(int nz, uintptr_t align_bits, int requested);
