@@ -528,7 +528,7 @@ def run_osmo(args):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     it_s = (n_iter - 1) / max(float(opts.iter_times[-1] - opts.iter_times[0]), 1e-9)
-    log("OSMO %d iterations: %.2f s total (with host VER each iteration), %.1f iters/s; VER %.4f -> %.4f" % (
+    log("OSMO %d iterations: %.2f s total (with the VER of each iteration), %.1f iters/s; VER %.4f -> %.4f" % (
         n_iter, dt, it_s, err[0], err[-1]))
     achieved = fp_bytes / (fp_ms * 1e-3) / 1e9
     _line("OSMO iters/s at 256^3 x 360 angles", it_s, "iters/s", 1e3 / it_s, args,

@@ -130,3 +130,19 @@ def test_grid_cells_reproduce_interpn():
     got = np.where(inx & ina, top + wx * (bot - top), 0.0)
     got[nanx | nana] = np.nan
     np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12)
+
+
+def test_calc_ver_tensor_path_matches_numpy():
+    """calcVER on tensors (masked reductions) equals the NumPy path, also per
+    slice (metrics.py:47-87)."""
+    import torch
+    from vamtoolbox_amd import geometry, metrics
+    rng = np.random.default_rng(8)
+    t = (rng.uniform(size=(24, 24, 6)) > 0.6).astype(float)
+    tg = geometry.TargetGeometry(target=t)
+    recon = rng.uniform(size=t.shape)
+    a = metrics.calcVER(tg, recon)
+    b = metrics.calcVER(tg, torch.tensor(recon))
+    assert a == b
+    np.testing.assert_array_equal(metrics.calcVER(tg, recon, layerwise=True),
+                                  metrics.calcVER(tg, torch.tensor(recon), layerwise=True))

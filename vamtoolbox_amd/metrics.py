@@ -14,21 +14,29 @@ def calcVER(target, recon, layerwise=False):
     """Volumetric error rate: #(void voxels >= min gel dose) / (#gel + #void)."""
     gel_inds, void_inds = target.gel_inds, target.void_inds
     if torch is not None and isinstance(recon, torch.Tensor):
+        # masked reductions on the device (no compaction, one host sync for
+        # the result); the gel / void masks and their count are cached on the
+        # target
         g = getattr(target, "_gel_t", None)
         if g is None or g.device != recon.device:
-            target._gel_t = torch.as_tensor(gel_inds, device=recon.device)
-            target._void_t = torch.as_tensor(void_inds, device=recon.device)
-        gt, vt = target._gel_t, target._void_t
-        num = int(gel_inds.sum() + void_inds.sum())
+            target._gel_t = torch.as_tensor(np.asarray(gel_inds), device=recon.device)
+            target._void_t = torch.as_tensor(np.asarray(void_inds), device=recon.device)
+            target._num_gv = int(np.sum(gel_inds) + np.sum(void_inds))
+            target._n_gel = int(np.sum(gel_inds))
+        gt, vt, num = target._gel_t, target._void_t, target._num_gv
+        if target._n_gel == 0:
+            raise ValueError("zero-size array to reduction operation minimum which has no identity")
+        inf = torch.tensor(float("inf"), dtype=recon.dtype, device=recon.device)
         if not layerwise:
-            min_gel = torch.amin(recon[gt])
-            return float(torch.sum(recon[vt] >= min_gel).item()) / num
-        out = np.zeros(recon.shape[2])
-        for z in range(recon.shape[2]):
-            sl = recon[:, :, z]
-            min_gel = torch.amin(sl[gt[:, :, z] if gt.ndim == 3 else gt])
-            out[z] = float(torch.sum(sl[vt[:, :, z] if vt.ndim == 3 else vt] >= min_gel).item()) / num
-        return out
+            min_gel = torch.amin(torch.where(gt, recon, inf))
+            return float(torch.count_nonzero(vt & (recon >= min_gel)).item()) / num
+        gt3 = gt if gt.ndim == 3 else gt[..., None]
+        vt3 = vt if vt.ndim == 3 else vt[..., None]
+        min_gel = torch.amin(torch.where(gt3, recon, inf), dim=(0, 1))          # per slice
+        if bool(torch.isinf(min_gel).any()):
+            raise ValueError("zero-size array to reduction operation minimum which has no identity")
+        counts = torch.count_nonzero(vt3 & (recon >= min_gel), dim=(0, 1))
+        return counts.double().cpu().numpy() / num
     num_gel_void = np.sum(gel_inds[:]) + np.sum(void_inds[:])
     if not layerwise:
         min_gel_dose = np.min(recon[gel_inds])
