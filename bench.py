@@ -209,8 +209,8 @@ def main():
                          "raytrace: curved-ray tracer at 192^3 x 360 angles (RK4) and its BCLP iteration")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: the light-propagation path runs on a single MI355X; "
-              "multi-GPU runs are not part of this build (see DESIGN.md for how it would shard)",
+        print("bench.py measures one GPU only: the light-propagation path runs on a single MI355X, "
+              "and --gpus N with N > 1 is not supported",
               file=sys.stderr)
         sys.exit(2)
     {"bclp": run_bclp, "osmo": run_osmo, "batched": run_batched, "raytrace": run_raytrace}[args.workload](args)
