@@ -210,6 +210,7 @@ PAIR_CASES = [
     (41, 64, np.arange(0.0, 360.0, 4.0), 2.5, 0.0, "signed"),       # waves project the rays separately
     (64, 40, np.arange(0.0, 360.0, 3.0), -3.0, 0.01, "signed"),     # partial z chunk
     (33, 256, np.concatenate([np.arange(0.0, 360.0, 10.0), [5.0, 95.0]]), 4.0, 0.01, "signed"),  # + single angles
+    (32, 64, np.array([0.0, 0.0, 180.0, 45.0, 225.0, 225.0, 90.0, -90.0]), 1.5, 0.01, "signed"),  # duplicates, unsorted
 ]
 
 
