@@ -153,7 +153,6 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchItem* __restrict_
       if (items[mid].tile_begin <= tile) lo = mid; else hi = mid - 1;
     }
     const BatchItem& it = items[lo];
-    const ParDev& g = it.g;
     // No `continue` below: a per-thread continue makes the loop header (and
     // its barriers) run for a subset of lanes, so waves of the block would
     // disagree on the barrier count.  Every thread reaches the loop end.

@@ -327,7 +327,7 @@ __global__ __launch_bounds__(256) void rt_gmax_part(const float* __restrict__ gi
   sh[threadIdx.x] = m;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) sh[threadIdx.x] = fmaxf(sh[threadIdx.x], sh[threadIdx.x + s]);
+    if ((int)threadIdx.x < s) sh[threadIdx.x] = fmaxf(sh[threadIdx.x], sh[threadIdx.x + s]);
     __syncthreads();
   }
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
@@ -343,7 +343,7 @@ __global__ __launch_bounds__(256) void rt_gmax_final(const float* __restrict__ p
   sh[threadIdx.x] = m;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) sh[threadIdx.x] = fmaxf(sh[threadIdx.x], sh[threadIdx.x + s]);
+    if ((int)threadIdx.x < s) sh[threadIdx.x] = fmaxf(sh[threadIdx.x], sh[threadIdx.x + s]);
     __syncthreads();
   }
   if (threadIdx.x == 0) {
