@@ -217,7 +217,7 @@ __global__ __launch_bounds__(256) void bclp_fp_update_wave(ParDev g, const float
 // C, paired: the opposite ray (k2, i2) of a pair is projected from the same
 // taps and updated too
 template <int ZV, bool REUSE>
-__global__ __launch_bounds__(256) void bclp_fp_update_pair(ParDev g, const float* __restrict__ operand,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void bclp_fp_update_pair(ParDev g, const float* __restrict__ operand,
                                                            float* __restrict__ gsino,
                                                            const float* __restrict__ minmax, int clip,
                                                            const double* __restrict__ scale_p, float lr, float glb,

@@ -173,9 +173,10 @@ __global__ __launch_bounds__(256) void par_fp_wave_kernel(ParDev g, const float*
 }
 
 // paired forward projection: one wavefront = ray (k, i) and, when it exists,
-// the opposite ray (k2, i2) x 64*ZV z, from one pass over the taps
+// the opposite ray (k2, i2) x 64*ZV z, from one pass over the taps.  Held to
+// 64 VGPRs (8 waves per SIMD; a 12-byte spill) -- faster than 71 VGPRs at 7.
 template <int ZV, bool REUSE>
-__global__ __launch_bounds__(256) void par_fp_pair_kernel(ParDev g, const float* __restrict__ vol,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void par_fp_pair_kernel(ParDev g, const float* __restrict__ vol,
                                                           float* __restrict__ sino, const float* __restrict__ minmax,
                                                           int clip) {
   using V = typename VecT<ZV>::T;
