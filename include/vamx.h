@@ -102,6 +102,9 @@ typedef struct vamx_par_tuning {
   int32_t fp_pair;          /* forward: 1 = project angles a and a + 180 deg in one pass over the
                                taps when 2 * det_offset is an integer and there is no inclination
                                or occlusion (default 1); fp_tile_angles then counts angle pairs */
+  int32_t z_split;          /* both: 1 = where a lane would hold one z value (nz odd or buffers
+                               not 8-byte aligned) and the 64-z chunk count is even, lanes hold
+                               z and z + 64 instead: half the wavefronts (default 1) */
 } vamx_par_tuning;
 int vamx_par_set_tuning(vamx_plan* plan, const vamx_par_tuning* tuning);
 int vamx_par_get_tuning(vamx_plan* plan, vamx_par_tuning* tuning);

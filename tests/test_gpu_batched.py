@@ -19,10 +19,12 @@ def test_batched_equals_individual(gpu, span):
     from vamtoolbox_amd import _lib
     from vamtoolbox_amd.parallel import ParallelPlan
     rng = np.random.default_rng(11)
-    # thin z (per-lane kernels) and z ranges the wavefront kernels take (full and partial chunks)
+    # thin z (per-lane kernels) and z ranges the wavefront kernels take (full and partial chunks,
+    # odd nz with lanes holding z and z + 64)
     # (the last problem fills the GPU alone: its forward runs as its own launch)
     specs = [(32, 8, 30, 0.0), (48, 5, 45, 0.01), (24, 12, 90, 0.0), (40, 3, 20, 0.02), (16, 1, 60, 0.0),
-             (32, 64, 40, 0.01), (24, 100, 30, 0.0), (21, 40, 25, 0.02), (18, 256, 12, 0.0), (128, 64, 360, 0.01)]
+             (32, 64, 40, 0.01), (24, 100, 30, 0.0), (21, 40, 25, 0.02), (18, 256, 12, 0.0), (26, 77, 36, 0.01),
+             (19, 201, 20, 0.0), (128, 64, 360, 0.01)]
     plans, vols, sinos, ref_s, bps, ref_b = [], [], [], [], [], []
     for n, nz, na, mu in specs:
         ang = np.linspace(0, span, na, endpoint=False)

@@ -200,7 +200,8 @@ def test_tuning_settings_agree(gpu):
     rep = plan.autotune(tv, ts, reps=1)
     assert set(rep["choice"]) == {"fp_tile_angles", "fp_tile_rays", "fp_zv", "fp_reuse", "fp_pair", "bp_tile",
                                   "bp_zv"}
-    assert plan.get_tuning() == rep["choice"]
+    tuned = plan.get_tuning()
+    assert {k: tuned[k] for k in rep["choice"]} == rep["choice"] and tuned["z_split"] == 1
 
 
 PAIR_CASES = [
@@ -258,6 +259,65 @@ def test_pairing_matches_unpaired_at_batch_shapes(gpu, n, nz, na, mu):
     err = np.abs(a - b).max() / scale
     print("n %d nz %d na %d: max |paired - unpaired| / max %.2e, rel L2 %.2e" % (n, nz, na, err, rel_l2(a, b)))
     assert err <= 2e-6 and rel_l2(a, b) <= 1e-6
+
+
+# shapes where a lane holds one z (odd nz, or buffers 4 bytes off 8-byte
+# alignment) and the 64-z chunk count is even: lanes then hold z and z + 64
+SPLIT_CASES = [
+    # n, nz, angles, det_offset, mu, buffers offset by one float
+    (30, 77, np.arange(0.0, 360.0, 8.0), 0.0, 0.02, False),
+    (23, 127, np.arange(0.0, 180.0, 6.0), 1.5, 0.0, False),
+    (26, 65, np.arange(2.0, 360.0, 12.0), -2.0, 0.01, False),
+    (20, 201, np.arange(0.0, 360.0, 15.0), 0.5, 0.0, False),         # 4 chunks of 64 -> 2 of 128
+    (24, 100, np.arange(0.0, 360.0, 10.0), 1.0, 0.03, True),         # even nz, misaligned buffers
+]
+
+
+@pytest.mark.parametrize("case", range(len(SPLIT_CASES)))
+@pytest.mark.parametrize("positive", [False, True])
+def test_split_z_layout(gpu, case, positive):
+    """Lanes holding z and z + 64 give the oracle's result and, z by z, the
+    same operations as one z per lane: paired and unpaired forward and the
+    back projection are bitwise equal with the split layout on and off."""
+    import torch
+    import oracle.d1 as d1
+    from vamtoolbox_amd.parallel import ParallelPlan
+    n, nz, ang, off, mu, shifted = SPLIT_CASES[case]
+    print("case", case, n, nz, len(ang), off, mu, shifted, "positive" if positive else "signed", flush=True)
+    rng = np.random.default_rng(500 + case)
+    vol = rng.uniform(0.2, 1.0, (n, n, nz)) if positive else rng.uniform(-1, 1, (n, n, nz))
+    sino = rng.uniform(0, 1, (n, len(ang), nz))
+
+    def place(a):
+        t = torch.tensor(a, dtype=torch.float32, device=gpu)
+        if not shifted:
+            return t
+        buf = torch.zeros(t.numel() + 1, dtype=torch.float32, device=gpu)
+        buf[1:] = t.reshape(-1)
+        return buf[1:].view(t.shape)
+
+    def empty(shape):
+        buf = torch.zeros(int(np.prod(shape)) + 1, dtype=torch.float32, device=gpu)
+        return (buf[1:] if shifted else buf[:-1]).view(shape)
+
+    tv, ts = place(vol), place(sino)
+    assert (tv.data_ptr() % 8 != 0) == shifted
+    plan = ParallelPlan(n, nz, ang, det_offset=off, mu=mu, device=gpu)
+    assert plan.get_tuning()["z_split"] == 1
+    res = {}
+    for split in (1, 0):
+        for pair in (1, 0):
+            plan.set_tuning(z_split=split, fp_pair=pair)
+            res[("f", split, pair)] = plan.forward(tv, out=empty(plan.sino_shape)).cpu().numpy()
+        res[("b", split)] = plan.backward(ts, out=empty(plan.vol_shape)).cpu().numpy()
+    ref_f, ref_b = d1.forward(vol, ang, det_offset=off, mu=mu), d1.backward(sino, ang, det_offset=off, mu=mu)
+    e1, e2 = rel_l2(res[("f", 1, 1)], ref_f), rel_l2(res[("b", 1)], ref_b)
+    print("rel L2 forward %.2e backward %.2e" % (e1, e2))
+    assert e1 <= TOL and e2 <= TOL
+    assert rel_l2(res[("f", 1, 0)], ref_f) <= TOL
+    assert np.array_equal(res[("f", 1, 1)], res[("f", 0, 1)])
+    assert np.array_equal(res[("f", 1, 0)], res[("f", 0, 0)])
+    assert np.array_equal(res[("b", 1)], res[("b", 0)])
 
 
 # --------------------------------------------------------------------------

@@ -46,6 +46,7 @@ class ParTuning(ctypes.Structure):
         ("bp_tile", c_int32),
         ("bp_zv", c_int32),
         ("fp_pair", c_int32),
+        ("z_split", c_int32),
     ]
 
 

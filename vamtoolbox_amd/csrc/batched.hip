@@ -7,7 +7,8 @@
 // sinogram.  A problem's work is cut into tiles of one 256-thread block:
 // four wavefronts of the wavefront-uniform kernels (one ray -- or a ray and
 // its opposite ray, when the plan pairs opposite angles -- or one 2x2 voxel
-// quad, and 64 x ZV z values per wavefront, in the plan's L2-aware order) or,
+// quad, and 64 x ZV z values per wavefront -- with ZV = 2 possibly split into
+// z and z + 64 -- in the plan's L2-aware order) or,
 // for shapes those do not take (inclination, very thin z), 256 items of the
 // per-lane kernels.  Tiles are ordered largest problem first and handed out
 // by a device-side counter, so the resident blocks of the single launch drain
@@ -33,68 +34,69 @@ struct BatchItem {
   int clip;              // forward: warp clip; backward: circle clip
   int wave;              // 1: wavefront-uniform kernels
   int pair;              // forward, wave mode: opposite-angle pairs (the plan's paired work order)
+  int split;             // wave mode, zv 2: lanes hold z and z + 64
 };
 
 __global__ void batch_reset(unsigned long long* counter) { *counter = 0ull; }
 
-template <int ZV>
+template <int ZV, bool SPLIT>
 __device__ __forceinline__ void batch_fp_wave(const BatchItem& it, long long w, int lane) {
   const ParDev& g = it.g;
-  int k, i, z;
-  if (!fp_wave_item<ZV>(w, g, lane, k, i, z)) return;
-  const int zl = z < g.nz - ZV ? z : g.nz - ZV;
-  const bool clip_on = it.clip && wave_clip_active<ZV>(it.minmax, g.nz, zl);
-  const typename VecT<ZV>::T acc = clip_on ? fp_ray_wave<ZV, true, true>(g, it.in, it.minmax, k, i, zl, lane)
-                                           : fp_ray_wave<ZV, false, true>(g, it.in, it.minmax, k, i, zl, lane);
-  if (z < g.nz) vstore<ZV>(it.out + ((size_t)k * g.na + i) * g.nz + z, acc);
+  int k, i, z, zl, zl2;
+  if (!fp_wave_item<ZV, SPLIT>(w, g, lane, k, i, z)) return;
+  wave_zload<ZV, SPLIT>(z, g.nz, zl, zl2);
+  const bool clip_on = it.clip && wave_clip_active<ZV, SPLIT>(it.minmax, g.nz, zl, zl2);
+  const typename VecT<ZV>::T acc =
+      clip_on ? fp_ray_wave<ZV, true, true, false, SPLIT>(g, it.in, it.minmax, k, i, zl, lane, nullptr, zl2)
+              : fp_ray_wave<ZV, false, true, false, SPLIT>(g, it.in, it.minmax, k, i, zl, lane, nullptr, zl2);
+  wave_zstore<ZV, SPLIT>(it.out + ((size_t)k * g.na + i) * g.nz, z, g.nz, acc);
 }
 
 // paired forward: ray (k, i) and its opposite ray from one pass over the
 // taps, as the single-problem paired kernel (clipped waves project the two
 // rays one after the other)
-template <int ZV>
+template <int ZV, bool SPLIT>
 __device__ __forceinline__ void batch_fp_pair(const BatchItem& it, long long w, int lane) {
   using V = typename VecT<ZV>::T;
   const ParDev& g = it.g;
-  const PairItem pi = fp_pair_item<ZV>(w, g, lane);
+  const PairItem pi = fp_pair_item<ZV, SPLIT>(w, g, lane);
   if (!pi.ok) return;
   const int z = pi.z;
-  const int zl = z < g.nz - ZV ? z : g.nz - ZV;
-  const bool clip_on = it.clip && wave_clip_active<ZV>(it.minmax, g.nz, zl);
+  int zl, zl2;
+  wave_zload<ZV, SPLIT>(z, g.nz, zl, zl2);
+  const bool clip_on = it.clip && wave_clip_active<ZV, SPLIT>(it.minmax, g.nz, zl, zl2);
   if (clip_on) {
     for (int h = 0; h < (pi.k2 >= 0 ? 2 : 1); ++h) {
       const int kh = h ? pi.k2 : pi.k, ih = h ? pi.i2 : pi.i;
-      const V acc = fp_ray_wave<ZV, true, true>(g, it.in, it.minmax, kh, ih, zl, lane);
-      if (z < g.nz) vstore<ZV>(it.out + ((size_t)kh * g.na + ih) * g.nz + z, acc);
+      const V acc = fp_ray_wave<ZV, true, true, false, SPLIT>(g, it.in, it.minmax, kh, ih, zl, lane, nullptr, zl2);
+      wave_zstore<ZV, SPLIT>(it.out + ((size_t)kh * g.na + ih) * g.nz, z, g.nz, acc);
     }
     return;
   }
   V acc, acc2 = vzero<ZV>();
   if (pi.k2 >= 0)
-    acc = fp_ray_wave<ZV, false, true, true>(g, it.in, it.minmax, pi.k, pi.i, zl, lane, &acc2);
+    acc = fp_ray_wave<ZV, false, true, true, SPLIT>(g, it.in, it.minmax, pi.k, pi.i, zl, lane, &acc2, zl2);
   else
-    acc = fp_ray_wave<ZV, false, true>(g, it.in, it.minmax, pi.k, pi.i, zl, lane);
-  if (z >= g.nz) return;
-  vstore<ZV>(it.out + ((size_t)pi.k * g.na + pi.i) * g.nz + z, acc);
-  if (pi.k2 >= 0) vstore<ZV>(it.out + ((size_t)pi.k2 * g.na + pi.i2) * g.nz + z, acc2);
+    acc = fp_ray_wave<ZV, false, true, false, SPLIT>(g, it.in, it.minmax, pi.k, pi.i, zl, lane, nullptr, zl2);
+  wave_zstore<ZV, SPLIT>(it.out + ((size_t)pi.k * g.na + pi.i) * g.nz, z, g.nz, acc);
+  if (pi.k2 >= 0) wave_zstore<ZV, SPLIT>(it.out + ((size_t)pi.k2 * g.na + pi.i2) * g.nz, z, g.nz, acc2);
 }
 
-template <int ZV>
+template <int ZV, bool SPLIT>
 __device__ __forceinline__ void batch_bp_wave(const BatchItem& it, long long w, int lane) {
   const ParDev& g = it.g;
-  int r0, c0, z;
-  if (!bp_wave_quad<ZV>(w, g, lane, r0, c0, z)) return;
-  const int zl = z < g.nz - ZV ? z : g.nz - ZV;
+  int r0, c0, z, zl, zl2;
+  if (!bp_wave_quad<ZV, SPLIT>(w, g, lane, r0, c0, z)) return;
+  wave_zload<ZV, SPLIT>(z, g.nz, zl, zl2);
   unsigned inrange, active;
   bp_quad_masks(g, r0, c0, it.clip, inrange, active);
   typename VecT<ZV>::T acc[4];
-  bp_quad_wave<ZV>(g, it.in, r0, c0, active, zl, lane, acc);
-  if (z >= g.nz) return;
+  bp_quad_wave<ZV, SPLIT>(g, it.in, r0, c0, active, zl, lane, acc, zl2);
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     if (!((inrange >> q) & 1u)) continue;
-    float* outp = it.out + ((size_t)(r0 + (q >> 1)) * g.n + c0 + (q & 1)) * g.nz + z;
-    vstore<ZV>(outp, ((active >> q) & 1u) ? acc[q] : vzero<ZV>());
+    float* outp = it.out + ((size_t)(r0 + (q >> 1)) * g.n + c0 + (q & 1)) * g.nz;
+    wave_zstore<ZV, SPLIT>(outp, z, g.nz, ((active >> q) & 1u) ? acc[q] : vzero<ZV>());
   }
 }
 
@@ -160,17 +162,20 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchItem* __restrict_
       const long long w = (tile - it.tile_begin) * 4 + slot;
       if (w < it.items) {
         if (FORWARD && it.pair) {
-          if (it.zv == 4) batch_fp_pair<4>(it, w, lane);
-          else if (it.zv == 2) batch_fp_pair<2>(it, w, lane);
-          else batch_fp_pair<1>(it, w, lane);
+          if (it.split) batch_fp_pair<2, true>(it, w, lane);
+          else if (it.zv == 4) batch_fp_pair<4, false>(it, w, lane);
+          else if (it.zv == 2) batch_fp_pair<2, false>(it, w, lane);
+          else batch_fp_pair<1, false>(it, w, lane);
         } else if (FORWARD) {
-          if (it.zv == 4) batch_fp_wave<4>(it, w, lane);
-          else if (it.zv == 2) batch_fp_wave<2>(it, w, lane);
-          else batch_fp_wave<1>(it, w, lane);
+          if (it.split) batch_fp_wave<2, true>(it, w, lane);
+          else if (it.zv == 4) batch_fp_wave<4, false>(it, w, lane);
+          else if (it.zv == 2) batch_fp_wave<2, false>(it, w, lane);
+          else batch_fp_wave<1, false>(it, w, lane);
         } else {
-          if (it.zv == 4) batch_bp_wave<4>(it, w, lane);
-          else if (it.zv == 2) batch_bp_wave<2>(it, w, lane);
-          else batch_bp_wave<1>(it, w, lane);
+          if (it.split) batch_bp_wave<2, true>(it, w, lane);
+          else if (it.zv == 4) batch_bp_wave<4, false>(it, w, lane);
+          else if (it.zv == 2) batch_bp_wave<2, false>(it, w, lane);
+          else batch_bp_wave<1, false>(it, w, lane);
         }
       }
     } else {
@@ -224,7 +229,8 @@ int run_batched(vamx_plan* const* plans, int n, const float* const* ins, float* 
   for (int i = 0; i < n; ++i) {
     vamx_plan* p = plans[i];
     if (forward && !p->par.incl) {
-      const int wz = par_wave_zv(p, ins[i], outs[i], p->par.fp_zv);
+      int wz = par_wave_zv(p, ins[i], outs[i], p->par.fp_zv);
+      if (par_wave_split(p, wz)) wz = 2;
       const long long units =
           p->par.fp_pair && p->par.pair_ok ? fp_pair_tiled(p->par) : fp_tiled_rays(p->par);
       if (wz && units * wave_zchunks(p->par.nz, wz) >= 16ll * cus) {
@@ -248,7 +254,9 @@ int run_batched(vamx_plan* const* plans, int n, const float* const* ins, float* 
     it.out = outs[i];
     it.minmax = p->d_minmax;
     it.clip = forward ? !(p->flags & VAMX_PAR_NO_WARP_CLIP) : !(p->flags & VAMX_PAR_NO_CIRCLE_CLIP);
-    const int wz = par_wave_zv(p, ins[i], outs[i], forward ? it.g.fp_zv : it.g.bp_zv);
+    int wz = par_wave_zv(p, ins[i], outs[i], forward ? it.g.fp_zv : it.g.bp_zv);
+    it.split = par_wave_split(p, wz) ? 1 : 0;
+    if (it.split) wz = 2;
     it.pair = 0;
     if (wz) {
       it.wave = 1;

@@ -60,6 +60,22 @@ __device__ __forceinline__ float4 bload<4>(__amdgpu_buffer_rsrc_t r, uint32_t vo
   return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), __uint_as_float(v[3]));
 }
 
+// Lane z layouts of the wavefront kernels: contiguous (a lane owns ZV
+// consecutive z) or split (SPLIT, ZV = 2: a lane owns z and z + 64, so one
+// wavefront covers 128 z with dword loads when nz is odd and no wider load
+// is aligned).  zb / zb2 are the byte offsets of the lane's (clamped) z.
+template <int ZV, bool SPLIT>
+__device__ __forceinline__ typename VecT<ZV>::T tload(__amdgpu_buffer_rsrc_t r, uint32_t zb, uint32_t zb2,
+                                                      uint32_t soff) {
+  if constexpr (SPLIT) {
+    static_assert(ZV == 2, "the split layout holds two z values per lane");
+    return make_float2(__uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)zb, (int)soff, 0)),
+                       __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)zb2, (int)soff, 0)));
+  } else {
+    return bload<ZV>(r, zb, soff);
+  }
+}
+
 template <int ZV> __device__ __forceinline__ void vstore(float* p, typename VecT<ZV>::T v);
 template <> __device__ __forceinline__ void vstore<1>(float* p, float v) { *p = v; }
 template <> __device__ __forceinline__ void vstore<2>(float* p, float2 v) { *reinterpret_cast<float2*>(p) = v; }
@@ -552,9 +568,38 @@ __device__ __forceinline__ float rl_f(float v, int lane) {
 // The warp clip of a slice is a no-op when 0 lies in [min, max] of the slice:
 // bilinear samples are convex combinations of the slice's values and the zero
 // padding.  Wave-uniform test over the wave's 64*ZV slices.
-template <int ZV>
-__device__ __forceinline__ bool wave_clip_active(const float* __restrict__ minmax, int nz, int z) {
-  const typename VecT<ZV>::T lo = vload<ZV>(minmax + z), hi = vload<ZV>(minmax + nz + z);
+// z of the loads of a lane whose first z is z: lanes past nz load a copy of
+// the last valid z and store nothing
+template <int ZV, bool SPLIT>
+__device__ __forceinline__ void wave_zload(int z, int nz, int& zl, int& zl2) {
+  if constexpr (SPLIT) {
+    zl = z < nz - 1 ? z : nz - 1;
+    zl2 = z + 64 < nz - 1 ? z + 64 : nz - 1;
+  } else {
+    zl = z < nz - ZV ? z : nz - ZV;
+    zl2 = 0;
+  }
+}
+// store of a lane's z values into the z column at col
+template <int ZV, bool SPLIT>
+__device__ __forceinline__ void wave_zstore(float* col, int z, int nz, typename VecT<ZV>::T v) {
+  if constexpr (SPLIT) {
+    if (z < nz) col[z] = v.x;
+    if (z + 64 < nz) col[z + 64] = v.y;
+  } else {
+    if (z < nz) vstore<ZV>(col + z, v);
+  }
+}
+
+// the slice range of the lane's z values (contiguous, or z and z2 split)
+template <int ZV, bool SPLIT = false>
+__device__ __forceinline__ typename VecT<ZV>::T zvals(const float* __restrict__ a, int z, int z2) {
+  if constexpr (SPLIT) return make_float2(a[z], a[z2]);
+  else return vload<ZV>(a + z);
+}
+template <int ZV, bool SPLIT = false>
+__device__ __forceinline__ bool wave_clip_active(const float* __restrict__ minmax, int nz, int z, int z2 = 0) {
+  const typename VecT<ZV>::T lo = zvals<ZV, SPLIT>(minmax, z, z2), hi = zvals<ZV, SPLIT>(minmax + nz, z, z2);
   bool need = false;
 #pragma unroll
   for (int q = 0; q < ZV; ++q) need |= !(vget(lo, q) <= 0.0f && 0.0f <= vget(hi, q));
@@ -596,10 +641,11 @@ __device__ __forceinline__ void axis_slots(double coord, int n, int& b, double& 
 // rounding error inside the -1 or n border clips up to the slice minimum
 // while one on the border stays 0, so PAIR excludes CLIP and clipped waves
 // project the two rays separately.
-template <int ZV, bool CLIP, bool REUSE, bool PAIR = false>
+template <int ZV, bool CLIP, bool REUSE, bool PAIR = false, bool SPLIT = false>
 __device__ __forceinline__ typename VecT<ZV>::T fp_ray_wave(const ParDev& g, const float* __restrict__ vol,
                                                              const float* __restrict__ minmax, int k, int i, int z,
-                                                             int lane, typename VecT<ZV>::T* acc2 = nullptr) {
+                                                             int lane, typename VecT<ZV>::T* acc2 = nullptr,
+                                                             int z2 = 0) {
   using V = typename VecT<ZV>::T;
   static_assert(!(PAIR && CLIP), "the clipped sample is discontinuous at the border: no pairing");
   const int n = g.n;
@@ -610,15 +656,15 @@ __device__ __forceinline__ typename VecT<ZV>::T fp_ray_wave(const ParDev& g, con
   const double t = (double)(k - g.c) - g.det_offset;
   V lo = vzero<ZV>(), hi = vzero<ZV>();
   if constexpr (CLIP) {
-    lo = vload<ZV>(minmax + z);
-    hi = vload<ZV>(minmax + g.nz + z);
+    lo = zvals<ZV, SPLIT>(minmax, z, z2);
+    hi = zvals<ZV, SPLIT>(minmax + g.nz, z, z2);
   }
   const size_t nzs = (size_t)g.nz;
   const size_t step_c = nzs, step_r = (size_t)n * nzs;
   // taps are buffer loads from a per-sample descriptor at the cell's corner
   // voxel column: lane byte offset 4z, scalar byte offsets of the other
   // three corners; the descriptor spans the two rows of the cell
-  const uint32_t zb = 4u * (uint32_t)z;
+  const uint32_t zb = 4u * (uint32_t)z, zb2 = 4u * (uint32_t)z2;
   const uint32_t oc = (uint32_t)(4 * step_c), orr = (uint32_t)(4 * step_r), orc = oc + orr;
   const uint32_t span = orc + oc;
   V acc = vzero<ZV>(), accp = vzero<ZV>();
@@ -682,46 +728,46 @@ __device__ __forceinline__ typename VecT<ZV>::T fp_ray_wave(const ParDev& g, con
 #define VAMX_TAP_ARMS(C00, C01, C10, C11, P)                                       \
   if (d == 1) {             /* one column right: the right column is new */      \
     asm volatile("; vamx fp " #P " d1");                                        \
-    C01 = bload<ZV>(rs, zb, oc);                                                 \
-    C11 = bload<ZV>(rs, zb, orc);                                                \
+    C01 = tload<ZV, SPLIT>(rs, zb, zb2, oc);                                                 \
+    C11 = tload<ZV, SPLIT>(rs, zb, zb2, orc);                                                \
   } else if (d == n) {      /* one row down: the bottom row is new */           \
     asm volatile("; vamx fp " #P " dn");                                        \
-    C10 = bload<ZV>(rs, zb, orr);                                                \
-    C11 = bload<ZV>(rs, zb, orc);                                                \
+    C10 = tload<ZV, SPLIT>(rs, zb, zb2, orr);                                                \
+    C11 = tload<ZV, SPLIT>(rs, zb, zb2, orc);                                                \
   } else if (d == -1) {                                                         \
     asm volatile("; vamx fp " #P " dm1");                                       \
-    C00 = bload<ZV>(rs, zb, 0);                                                  \
-    C10 = bload<ZV>(rs, zb, orr);                                                \
+    C00 = tload<ZV, SPLIT>(rs, zb, zb2, 0);                                                  \
+    C10 = tload<ZV, SPLIT>(rs, zb, zb2, orr);                                                \
   } else if (d == -n) {                                                         \
     asm volatile("; vamx fp " #P " dmn");                                       \
-    C00 = bload<ZV>(rs, zb, 0);                                                  \
-    C01 = bload<ZV>(rs, zb, oc);                                                 \
+    C00 = tload<ZV, SPLIT>(rs, zb, zb2, 0);                                                  \
+    C01 = tload<ZV, SPLIT>(rs, zb, zb2, oc);                                                 \
   } else if (d == n + 1) {  /* diagonal steps keep one tap */                   \
     asm volatile("; vamx fp " #P " dnp1");                                      \
-    C01 = bload<ZV>(rs, zb, oc);                                                 \
-    C10 = bload<ZV>(rs, zb, orr);                                                \
-    C11 = bload<ZV>(rs, zb, orc);                                                \
+    C01 = tload<ZV, SPLIT>(rs, zb, zb2, oc);                                                 \
+    C10 = tload<ZV, SPLIT>(rs, zb, zb2, orr);                                                \
+    C11 = tload<ZV, SPLIT>(rs, zb, zb2, orc);                                                \
   } else if (d == n - 1) {                                                      \
     asm volatile("; vamx fp " #P " dnm1");                                      \
-    C00 = bload<ZV>(rs, zb, 0);                                                  \
-    C10 = bload<ZV>(rs, zb, orr);                                                \
-    C11 = bload<ZV>(rs, zb, orc);                                                \
+    C00 = tload<ZV, SPLIT>(rs, zb, zb2, 0);                                                  \
+    C10 = tload<ZV, SPLIT>(rs, zb, zb2, orr);                                                \
+    C11 = tload<ZV, SPLIT>(rs, zb, zb2, orc);                                                \
   } else if (d == -n + 1) {                                                     \
     asm volatile("; vamx fp " #P " dmnp1");                                     \
-    C00 = bload<ZV>(rs, zb, 0);                                                  \
-    C01 = bload<ZV>(rs, zb, oc);                                                 \
-    C11 = bload<ZV>(rs, zb, orc);                                                \
+    C00 = tload<ZV, SPLIT>(rs, zb, zb2, 0);                                                  \
+    C01 = tload<ZV, SPLIT>(rs, zb, zb2, oc);                                                 \
+    C11 = tload<ZV, SPLIT>(rs, zb, zb2, orc);                                                \
   } else if (d == -n - 1) {                                                     \
     asm volatile("; vamx fp " #P " dmnm1");                                     \
-    C00 = bload<ZV>(rs, zb, 0);                                                  \
-    C01 = bload<ZV>(rs, zb, oc);                                                 \
-    C10 = bload<ZV>(rs, zb, orr);                                                \
+    C00 = tload<ZV, SPLIT>(rs, zb, zb2, 0);                                                  \
+    C01 = tload<ZV, SPLIT>(rs, zb, zb2, oc);                                                 \
+    C10 = tload<ZV, SPLIT>(rs, zb, zb2, orr);                                                \
   } else if (d != 0) {                                                          \
     asm volatile("; vamx fp " #P " full");                                      \
-    C00 = bload<ZV>(rs, zb, 0);                                                  \
-    C01 = bload<ZV>(rs, zb, oc);                                                 \
-    C10 = bload<ZV>(rs, zb, orr);                                                \
-    C11 = bload<ZV>(rs, zb, orc);                                                \
+    C00 = tload<ZV, SPLIT>(rs, zb, zb2, 0);                                                  \
+    C01 = tload<ZV, SPLIT>(rs, zb, zb2, oc);                                                 \
+    C10 = tload<ZV, SPLIT>(rs, zb, zb2, orr);                                                \
+    C11 = tload<ZV, SPLIT>(rs, zb, zb2, orc);                                                \
   }
       const int d = REUSE ? b - prev : 2 * n + 2;   // without reuse every sample reloads
       prev = b;
@@ -761,16 +807,18 @@ __device__ __forceinline__ typename VecT<ZV>::T fp_ray_wave(const ParDev& g, con
 // fma(w1, b[j+1], fma(w0, b[j], acc)) in angle order -- the same operations,
 // in the same order, as one column at a time.  Requires n >= 2 and all 64
 // lanes of the wavefront (cross-lane swizzles).
-template <int ZV>
+template <int ZV, bool SPLIT = false>
 __device__ __forceinline__ void bp_quad_wave(const ParDev& g, const float* __restrict__ sino, int r0, int c0,
-                                             unsigned active, int z, int lane, typename VecT<ZV>::T acc[4]) {
+                                             unsigned active, int z, int lane, typename VecT<ZV>::T acc[4],
+                                             int z2 = 0) {
   using V = typename VecT<ZV>::T;
   const int p = lane & 3, a = lane >> 2;
   const bool pin = (active >> p) & 1u;
   const double x = (double)(c0 + (p & 1) - g.c), y = (double)(r0 + (p >> 1) - g.c);
   const size_t nzs = (size_t)g.nz;
   const size_t step_j = (size_t)g.na * nzs;
-  const uint32_t zb = 4u * (uint32_t)z, row_b = (uint32_t)(4 * step_j), rows_b = 3 * row_b + 4u * (uint32_t)g.nz;
+  const uint32_t zb = 4u * (uint32_t)z, zb2 = 4u * (uint32_t)z2, row_b = (uint32_t)(4 * step_j);
+  const uint32_t rows_b = 3 * row_b + 4u * (uint32_t)g.nz;
 #pragma unroll
   for (int q = 0; q < 4; ++q) acc[q] = vzero<ZV>();
   for (int i0 = 0; i0 < g.na; i0 += 16) {
@@ -826,10 +874,10 @@ __device__ __forceinline__ void bp_quad_wave(const ParDev& g, const float* __res
       // scalar row offsets
       const __amdgpu_buffer_rsrc_t rs = buf_rsrc(sino + (size_t)(unsigned)rl_i((int)rowl, l0), rows_b);
       const int span = rl_i(spanl, l0);
-      const V T0 = bload<ZV>(rs, zb, 0), T1 = bload<ZV>(rs, zb, row_b);
+      const V T0 = tload<ZV, SPLIT>(rs, zb, zb2, 0), T1 = tload<ZV, SPLIT>(rs, zb, zb2, row_b);
       V T2 = vzero<ZV>(), T3 = vzero<ZV>();
-      if (span >= 1) T2 = bload<ZV>(rs, zb, 2 * row_b);
-      if (span >= 2) T3 = bload<ZV>(rs, zb, 3 * row_b);
+      if (span >= 1) T2 = tload<ZV, SPLIT>(rs, zb, zb2, 2 * row_b);
+      if (span >= 2) T3 = tload<ZV, SPLIT>(rs, zb, zb2, 3 * row_b);
       // one branch per pixel and row offset (the offsets are wave-uniform);
       // the empty asm markers keep the compiler from turning the branches
       // into register selects
@@ -925,14 +973,14 @@ __device__ __forceinline__ long long wave_index(unsigned lb) {
 // partial (lanes past nz load from a clamped z and store nothing)
 __host__ __device__ inline int wave_zchunks(int nz, int zv) { return (nz + 64 * zv - 1) / (64 * zv); }
 
-template <int ZV>
+template <int ZV, bool SPLIT = false>
 __device__ __forceinline__ bool fp_wave_item(long long wave, const ParDev& g, int lane, int& k, int& i, int& z) {
   const long long rays = fp_tiled_rays(g);
   const int nch = wave_zchunks(g.nz, ZV);
   if (wave >= rays * nch) return false;
   const int zc = (int)(wave / rays);
   if (!fp_tile_ray(wave % rays, g, k, i)) return false;
-  z = zc * 64 * ZV + lane * ZV;
+  z = SPLIT ? zc * 64 * ZV + lane : zc * 64 * ZV + lane * ZV;
   return true;
 }
 
@@ -955,7 +1003,7 @@ struct PairItem {
   int k, i, k2, i2, z;
   bool ok;
 };
-template <int ZV>
+template <int ZV, bool SPLIT = false>
 __host__ __device__ __forceinline__ PairItem fp_pair_item(long long wave, const ParDev& g, int lane) {
   PairItem it{0, 0, -1, -1, 0, false};
   const long long tiled = fp_pair_tiled(g);
@@ -979,20 +1027,20 @@ __host__ __device__ __forceinline__ PairItem fp_pair_item(long long wave, const 
   it.i = e < g.n ? u1 : u2;
   it.k2 = has2 ? kk : -1;
   it.i2 = has2 ? u2 : -1;
-  it.z = zc * 64 * ZV + lane * ZV;
+  it.z = SPLIT ? zc * 64 * ZV + lane : zc * 64 * ZV + lane * ZV;   // split: z and z + 64
   it.ok = true;
   return it;
 }
 
 // wave -> (voxel quad, z) of the back-projection kernels; false for padding
-template <int ZV>
+template <int ZV, bool SPLIT = false>
 __device__ __forceinline__ bool bp_wave_quad(long long wave, const ParDev& g, int lane, int& r0, int& c0, int& z) {
   const long long quads = bp_tiled_quads(g);
   const int nch = wave_zchunks(g.nz, ZV);
   if (wave >= quads * nch) return false;
   const int zc = (int)(wave / quads);
   if (!bp_tile_quad(wave % quads, g, r0, c0)) return false;
-  z = zc * 64 * ZV + lane * ZV;
+  z = SPLIT ? zc * 64 * ZV + lane : zc * 64 * ZV + lane * ZV;
   return true;
 }
 
@@ -1017,6 +1065,7 @@ int par_slice_minmax(vamx_plan* p, const float* vol, hipStream_t s);
 int par_forward_launch(vamx_plan* p, const float* vol, float* sino, hipStream_t s, bool warp_clip);
 bool par_use_vec4(const vamx_plan* p, const void* a, const void* b);
 int par_wave_zv(const vamx_plan* p, const void* a, const void* b, int requested);
+bool par_wave_split(const vamx_plan* p, int zv);
 int wave_zv_choice(int nz, uintptr_t align_bits, int requested);
 void par_pair_units(const std::vector<double>& cs, int na, int n, int c, double D, std::vector<int>& units,
                     int& npaired_rays, int& first2);

@@ -157,77 +157,78 @@ __global__ __launch_bounds__(256) void par_fp_kernel(ParDev g, const float* __re
 }
 
 // wavefront-uniform forward projection: one wavefront = one ray x 64*ZV z
-template <int ZV, bool REUSE>
+// (SPLIT: ZV = 2 with lanes owning z and z + 64)
+template <int ZV, bool REUSE, bool SPLIT>
 __global__ __launch_bounds__(256) void par_fp_wave_kernel(ParDev g, const float* __restrict__ vol,
                                                           float* __restrict__ sino, const float* __restrict__ minmax,
                                                           int clip) {
   const int lane = threadIdx.x & 63;
   const long long wave = wave_index(xcd_remap(blockIdx.x, gridDim.x));
-  int k, i, z;
-  if (!fp_wave_item<ZV>(wave, g, lane, k, i, z)) return;
-  const int zl = z < g.nz - ZV ? z : g.nz - ZV;   // lanes past nz compute a copy and store nothing
-  const bool clip_on = clip && wave_clip_active<ZV>(minmax, g.nz, zl);
-  typename VecT<ZV>::T acc = clip_on ? fp_ray_wave<ZV, true, REUSE>(g, vol, minmax, k, i, zl, lane)
-                                     : fp_ray_wave<ZV, false, REUSE>(g, vol, minmax, k, i, zl, lane);
-  if (z < g.nz) vstore<ZV>(sino + ((size_t)k * g.na + i) * g.nz + z, acc);
+  int k, i, z, zl, zl2;
+  if (!fp_wave_item<ZV, SPLIT>(wave, g, lane, k, i, z)) return;
+  wave_zload<ZV, SPLIT>(z, g.nz, zl, zl2);
+  const bool clip_on = clip && wave_clip_active<ZV, SPLIT>(minmax, g.nz, zl, zl2);
+  typename VecT<ZV>::T acc =
+      clip_on ? fp_ray_wave<ZV, true, REUSE, false, SPLIT>(g, vol, minmax, k, i, zl, lane, nullptr, zl2)
+              : fp_ray_wave<ZV, false, REUSE, false, SPLIT>(g, vol, minmax, k, i, zl, lane, nullptr, zl2);
+  wave_zstore<ZV, SPLIT>(sino + ((size_t)k * g.na + i) * g.nz, z, g.nz, acc);
 }
 
 // paired forward projection: one wavefront = ray (k, i) and, when it exists,
 // the opposite ray (k2, i2) x 64*ZV z, from one pass over the taps.  Held to
 // 64 VGPRs (8 waves per SIMD; a 12-byte spill) -- faster than 71 VGPRs at 7.
-template <int ZV, bool REUSE>
+template <int ZV, bool REUSE, bool SPLIT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void par_fp_pair_kernel(ParDev g, const float* __restrict__ vol,
                                                           float* __restrict__ sino, const float* __restrict__ minmax,
                                                           int clip) {
   using V = typename VecT<ZV>::T;
   const int lane = threadIdx.x & 63;
   const long long wave = wave_index(xcd_remap(blockIdx.x, gridDim.x));
-  const PairItem it = fp_pair_item<ZV>(wave, g, lane);
+  const PairItem it = fp_pair_item<ZV, SPLIT>(wave, g, lane);
   if (!it.ok) return;
   const int k = it.k, i = it.i, k2 = it.k2, i2 = it.i2, z = it.z;
-  const int zl = z < g.nz - ZV ? z : g.nz - ZV;
-  const bool clip_on = clip && wave_clip_active<ZV>(minmax, g.nz, zl);
+  int zl, zl2;
+  wave_zload<ZV, SPLIT>(z, g.nz, zl, zl2);
+  const bool clip_on = clip && wave_clip_active<ZV, SPLIT>(minmax, g.nz, zl, zl2);
   if (clip_on) {
     // the clipped sample is not continuous in its position: the two rays are
     // projected one after the other (one call site keeps the register count
     // of the unpaired kernel)
     for (int h = 0; h < (k2 >= 0 ? 2 : 1); ++h) {
       const int kh = h ? k2 : k, ih = h ? i2 : i;
-      const V acc = fp_ray_wave<ZV, true, REUSE>(g, vol, minmax, kh, ih, zl, lane);
-      if (z < g.nz) vstore<ZV>(sino + ((size_t)kh * g.na + ih) * g.nz + z, acc);
+      const V acc = fp_ray_wave<ZV, true, REUSE, false, SPLIT>(g, vol, minmax, kh, ih, zl, lane, nullptr, zl2);
+      wave_zstore<ZV, SPLIT>(sino + ((size_t)kh * g.na + ih) * g.nz, z, g.nz, acc);
     }
     return;
   }
   V acc, acc2 = vzero<ZV>();
   if (k2 >= 0)
-    acc = fp_ray_wave<ZV, false, REUSE, true>(g, vol, minmax, k, i, zl, lane, &acc2);
+    acc = fp_ray_wave<ZV, false, REUSE, true, SPLIT>(g, vol, minmax, k, i, zl, lane, &acc2, zl2);
   else
-    acc = fp_ray_wave<ZV, false, REUSE>(g, vol, minmax, k, i, zl, lane);
-  if (z >= g.nz) return;
-  vstore<ZV>(sino + ((size_t)k * g.na + i) * g.nz + z, acc);
-  if (k2 >= 0) vstore<ZV>(sino + ((size_t)k2 * g.na + i2) * g.nz + z, acc2);
+    acc = fp_ray_wave<ZV, false, REUSE, false, SPLIT>(g, vol, minmax, k, i, zl, lane, nullptr, zl2);
+  wave_zstore<ZV, SPLIT>(sino + ((size_t)k * g.na + i) * g.nz, z, g.nz, acc);
+  if (k2 >= 0) wave_zstore<ZV, SPLIT>(sino + ((size_t)k2 * g.na + i2) * g.nz, z, g.nz, acc2);
 }
 
 // wavefront-uniform back projection: one wavefront = one 2x2 quad of voxel
 // columns x 64*ZV z
-template <int ZV>
+template <int ZV, bool SPLIT>
 __global__ __launch_bounds__(256) void par_bp_wave_kernel(ParDev g, const float* __restrict__ sino,
                                                           float* __restrict__ vol, int clipc) {
   const int lane = threadIdx.x & 63;
   const long long wave = wave_index(xcd_remap(blockIdx.x, gridDim.x));
-  int r0, c0, z;
-  if (!bp_wave_quad<ZV>(wave, g, lane, r0, c0, z)) return;
-  const int zl = z < g.nz - ZV ? z : g.nz - ZV;
+  int r0, c0, z, zl, zl2;
+  if (!bp_wave_quad<ZV, SPLIT>(wave, g, lane, r0, c0, z)) return;
+  wave_zload<ZV, SPLIT>(z, g.nz, zl, zl2);
   unsigned inrange, active;
   bp_quad_masks(g, r0, c0, clipc, inrange, active);
   typename VecT<ZV>::T acc[4];
-  bp_quad_wave<ZV>(g, sino, r0, c0, active, zl, lane, acc);
-  if (z >= g.nz) return;
+  bp_quad_wave<ZV, SPLIT>(g, sino, r0, c0, active, zl, lane, acc, zl2);
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     if (!((inrange >> q) & 1u)) continue;
-    float* outp = vol + ((size_t)(r0 + (q >> 1)) * g.n + c0 + (q & 1)) * g.nz + z;
-    vstore<ZV>(outp, ((active >> q) & 1u) ? acc[q] : vzero<ZV>());
+    float* outp = vol + ((size_t)(r0 + (q >> 1)) * g.n + c0 + (q & 1)) * g.nz;
+    wave_zstore<ZV, SPLIT>(outp, z, g.nz, ((active >> q) & 1u) ? acc[q] : vzero<ZV>());
   }
 }
 
@@ -343,6 +344,15 @@ int vamx::wave_zv_choice(int nz, uintptr_t align_bits, int requested) {
     }
   }
   return best_util >= 0.5 ? best : 0;
+}
+
+bool vamx::par_wave_split(const vamx_plan* p, int zv) {
+  // the split z layout (ZV = 2, lanes own z and z + 64) where the contiguous
+  // choice is one z per lane: half the wavefronts at the same lane
+  // utilisation when the 64-z chunk count is even
+  if (zv != 1 || !p->par.z_split) return false;
+  const int ch = wave_zchunks(p->par.nz, 1);
+  return ch % 2 == 0;
 }
 
 bool vamx::par_use_vec4(const vamx_plan* p, const void* a, const void* b) {
@@ -477,6 +487,7 @@ extern "C" int vamx_par_create(const vamx_par_geom* geom, vamx_plan** out_plan) 
   g.pair_u = n - npaired_rays;
   g.pair_first2 = first2;
   g.fp_pair = 1;
+  g.z_split = 1;
   // defaults: ParallelPlan.autotune's choice at 256^3 x 360 angles on MI355X
   // (8 angle pairs x 8 rays, paired)
   g.fp_ta = 8;
@@ -561,20 +572,25 @@ int par_forward_launch(vamx_plan* p, const float* vol, float* sino, hipStream_t 
     int rc = par_slice_minmax(p, vol, s);
     if (rc) return rc;
   }
-  const int wz = par_wave_zv(p, vol, sino, g.fp_zv);
+  int wz = par_wave_zv(p, vol, sino, g.fp_zv);
+  const bool split = par_wave_split(p, wz);
+  if (split) wz = 2;
   if (wz && g.fp_pair && g.pair_ok) {
     const long long threads = fp_pair_tiled(g) * wave_zchunks(g.nz, wz) * 64;
     const dim3 grid(blocks_for(threads, 256)), block(256);
     const int clip = warp_clip ? 1 : 0;
-#define VAMX_FPP(Z, R) hipLaunchKernelGGL((par_fp_pair_kernel<Z, R>), grid, block, 0, s, g, vol, sino, p->d_minmax, clip)
+#define VAMX_FPP(Z, R, S) \
+  hipLaunchKernelGGL((par_fp_pair_kernel<Z, R, S>), grid, block, 0, s, g, vol, sino, p->d_minmax, clip)
     if (g.fp_reuse) {
-      if (wz == 4) VAMX_FPP(4, true);
-      else if (wz == 2) VAMX_FPP(2, true);
-      else VAMX_FPP(1, true);
+      if (split) VAMX_FPP(2, true, true);
+      else if (wz == 4) VAMX_FPP(4, true, false);
+      else if (wz == 2) VAMX_FPP(2, true, false);
+      else VAMX_FPP(1, true, false);
     } else {
-      if (wz == 4) VAMX_FPP(4, false);
-      else if (wz == 2) VAMX_FPP(2, false);
-      else VAMX_FPP(1, false);
+      if (split) VAMX_FPP(2, false, true);
+      else if (wz == 4) VAMX_FPP(4, false, false);
+      else if (wz == 2) VAMX_FPP(2, false, false);
+      else VAMX_FPP(1, false, false);
     }
 #undef VAMX_FPP
     VAMX_HIP(hipGetLastError());
@@ -584,15 +600,18 @@ int par_forward_launch(vamx_plan* p, const float* vol, float* sino, hipStream_t 
     const long long threads = fp_tiled_rays(g) * wave_zchunks(g.nz, wz) * 64;
     const dim3 grid(blocks_for(threads, 256)), block(256);
     const int clip = warp_clip ? 1 : 0;
-#define VAMX_FPW(Z, R) hipLaunchKernelGGL((par_fp_wave_kernel<Z, R>), grid, block, 0, s, g, vol, sino, p->d_minmax, clip)
+#define VAMX_FPW(Z, R, S) \
+  hipLaunchKernelGGL((par_fp_wave_kernel<Z, R, S>), grid, block, 0, s, g, vol, sino, p->d_minmax, clip)
     if (g.fp_reuse) {
-      if (wz == 4) VAMX_FPW(4, true);
-      else if (wz == 2) VAMX_FPW(2, true);
-      else VAMX_FPW(1, true);
+      if (split) VAMX_FPW(2, true, true);
+      else if (wz == 4) VAMX_FPW(4, true, false);
+      else if (wz == 2) VAMX_FPW(2, true, false);
+      else VAMX_FPW(1, true, false);
     } else {
-      if (wz == 4) VAMX_FPW(4, false);
-      else if (wz == 2) VAMX_FPW(2, false);
-      else VAMX_FPW(1, false);
+      if (split) VAMX_FPW(2, false, true);
+      else if (wz == 4) VAMX_FPW(4, false, false);
+      else if (wz == 2) VAMX_FPW(2, false, false);
+      else VAMX_FPW(1, false, false);
     }
 #undef VAMX_FPW
     VAMX_HIP(hipGetLastError());
@@ -626,17 +645,21 @@ int par_backward_launch(vamx_plan* p, const float* sino, float* vol, hipStream_t
     VAMX_HIP(hipGetLastError());
     return VAMX_OK;
   }
-  const int wz = par_wave_zv(p, sino, vol, g.bp_zv);
+  int wz = par_wave_zv(p, sino, vol, g.bp_zv);
+  const bool split = par_wave_split(p, wz);
+  if (split) wz = 2;
   if (wz) {
     const long long threads = bp_tiled_quads(g) * wave_zchunks(g.nz, wz) * 64;
     const dim3 grid(blocks_for(threads, 256)), block(256);
     const int cc = clipc ? 1 : 0;
-    if (wz == 4)
-      hipLaunchKernelGGL(par_bp_wave_kernel<4>, grid, block, 0, s, g, sino, vol, cc);
+    if (split)
+      hipLaunchKernelGGL((par_bp_wave_kernel<2, true>), grid, block, 0, s, g, sino, vol, cc);
+    else if (wz == 4)
+      hipLaunchKernelGGL((par_bp_wave_kernel<4, false>), grid, block, 0, s, g, sino, vol, cc);
     else if (wz == 2)
-      hipLaunchKernelGGL(par_bp_wave_kernel<2>, grid, block, 0, s, g, sino, vol, cc);
+      hipLaunchKernelGGL((par_bp_wave_kernel<2, false>), grid, block, 0, s, g, sino, vol, cc);
     else
-      hipLaunchKernelGGL(par_bp_wave_kernel<1>, grid, block, 0, s, g, sino, vol, cc);
+      hipLaunchKernelGGL((par_bp_wave_kernel<1, false>), grid, block, 0, s, g, sino, vol, cc);
     VAMX_HIP(hipGetLastError());
     return VAMX_OK;
   }
@@ -721,9 +744,9 @@ extern "C" int vamx_par_set_tuning(vamx_plan* plan, const vamx_par_tuning* t) {
   auto pow2 = [](int v) { return v == 1 || v == 2 || v == 4; };
   if (t->fp_tile_angles < 1 || t->fp_tile_rays < 1 || t->bp_tile < 1 || !pow2(t->fp_zv) || !pow2(t->bp_zv) ||
       t->fp_tile_angles * t->fp_tile_rays > (1 << 20) || t->bp_tile > 1024 || t->fp_reuse < 0 || t->fp_reuse > 1 ||
-      t->fp_pair < 0 || t->fp_pair > 1)
-    return set_error(VAMX_ERR_ARG,
-                     "vamx_par_set_tuning: tiles must be >= 1, z widths 1, 2 or 4, fp_reuse and fp_pair 0 or 1");
+      t->fp_pair < 0 || t->fp_pair > 1 || t->z_split < 0 || t->z_split > 1)
+    return set_error(VAMX_ERR_ARG, "vamx_par_set_tuning: tiles must be >= 1, z widths 1, 2 or 4, fp_reuse, "
+                                   "fp_pair and z_split 0 or 1");
   plan->par.fp_ta = t->fp_tile_angles;
   plan->par.fp_tk = t->fp_tile_rays;
   plan->par.fp_zv = t->fp_zv;
@@ -731,6 +754,7 @@ extern "C" int vamx_par_set_tuning(vamx_plan* plan, const vamx_par_tuning* t) {
   plan->par.bp_tile = t->bp_tile;
   plan->par.bp_zv = t->bp_zv;
   plan->par.fp_pair = t->fp_pair;
+  plan->par.z_split = t->z_split;
   return VAMX_OK;
 }
 
@@ -744,5 +768,6 @@ extern "C" int vamx_par_get_tuning(vamx_plan* plan, vamx_par_tuning* t) {
   t->bp_tile = plan->par.bp_tile;
   t->bp_zv = plan->par.bp_zv;
   t->fp_pair = plan->par.fp_pair;
+  t->z_split = plan->par.z_split;
   return VAMX_OK;
 }

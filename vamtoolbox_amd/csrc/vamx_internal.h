@@ -34,6 +34,7 @@ struct ParDev {
   int fp_ta, fp_tk, fp_zv;   // forward: tile of fp_ta angles x fp_tk rays, z values per lane
   int fp_reuse;              // forward: keep the previous sample's taps in registers
   int bp_tile, bp_zv;        // back projection: bp_tile x bp_tile voxel columns, z values per lane
+  int z_split;               // allow lanes holding z and z + 64 where one z per lane is the choice
   // Opposite-angle pairing of the forward projection.  Ray k at angle a and
   // ray 2c + D - k at angle a + 180 deg (D = 2 det_offset, an integer) sample
   // the same points in reverse order, so one pass over the taps serves both.
