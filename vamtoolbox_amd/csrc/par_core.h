@@ -805,8 +805,12 @@ __device__ __forceinline__ typename VecT<ZV>::T fp_ray_wave(const ParDev& g, con
 // the float64 geometry of 16 angles x 4 pixels; the per-angle loop then
 // loads the rows [bmin, bmax + 1] once and every pixel accumulates
 // fma(w1, b[j+1], fma(w0, b[j], acc)) in angle order -- the same operations,
-// in the same order, as one column at a time.  Requires n >= 2 and all 64
-// lanes of the wavefront (cross-lane swizzles).
+// in the same order, as one column at a time.  The 16 angles' weights pass
+// through a per-wavefront LDS buffer and come back as one broadcast 16-byte
+// read per row weight, and each angle's row offsets, span and first row are
+// packed into one value: the per-angle loop moves one value from a lane.
+// Requires n >= 2, all 64 lanes of the wavefront (cross-lane swizzles) and
+// blocks of at most 4 wavefronts (the LDS buffer's size).
 template <int ZV, bool SPLIT = false>
 __device__ __forceinline__ void bp_quad_wave(const ParDev& g, const float* __restrict__ sino, int r0, int c0,
                                              unsigned active, int z, int lane, typename VecT<ZV>::T acc[4],
@@ -819,6 +823,8 @@ __device__ __forceinline__ void bp_quad_wave(const ParDev& g, const float* __res
   const size_t step_j = (size_t)g.na * nzs;
   const uint32_t zb = 4u * (uint32_t)z, zb2 = 4u * (uint32_t)z2, row_b = (uint32_t)(4 * step_j);
   const uint32_t rows_b = 3 * row_b + 4u * (uint32_t)g.nz;
+  __shared__ float4 bw_all[4][2][16];   // per wavefront of the block: weights of the lower / upper row
+  float4 (&bw)[2][16] = bw_all[__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) & 3];
 #pragma unroll
   for (int q = 0; q < 4; ++q) acc[q] = vzero<ZV>();
   for (int i0 = 0; i0 < g.na; i0 += 16) {
@@ -855,25 +861,29 @@ __device__ __forceinline__ void bp_quad_wave(const ParDev& g, const float* __res
     lo = min(lo, __shfl_xor(lo, 2));
     hi = max(hi, __shfl_xor(hi, 1));
     hi = max(hi, __shfl_xor(hi, 2));
-    const int offl = jr - lo, spanl = hi - lo;
-    const unsigned rowl = ((unsigned)lo * (unsigned)g.na + (unsigned)i) * (unsigned)g.nz;
+    // per angle: bits 0-7 the pixels' row offsets from bmin, 8-9 the span
+    // bmax - bmin, 10.. bmin (n < 16384)
+    int bits = valid ? (jr - lo) << (2 * p) : 0;
+    bits |= __shfl_xor(bits, 1);
+    bits |= __shfl_xor(bits, 2);
+    const int pk = hi >= 0 ? bits | ((hi - lo) << 8) | (lo << 10) : 0;
+    __builtin_amdgcn_wave_barrier();
+    reinterpret_cast<float*>(&bw[0][a])[p] = w0;
+    reinterpret_cast<float*>(&bw[1][a])[p] = w1;
+    __builtin_amdgcn_wave_barrier();
     while (am) {
       const int l0 = __builtin_ctzll(am);   // lane of pixel 0 of this angle
       am &= am - 1;
       const unsigned pm = (unsigned)(m >> l0) & 0xFu;
-      int jq[4];
-      float u0[4], u1[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        jq[q] = rl_i(offl, l0 + q);
-        u0[q] = rl_f(w0, l0 + q);
-        u1[q] = rl_f(w1, l0 + q);
-      }
+      const int pkv = rl_i(pk, l0);
+      const float4 U0 = bw[0][l0 >> 2], U1 = bw[1][l0 >> 2];
+      const float u0[4] = {U0.x, U0.y, U0.z, U0.w}, u1[4] = {U1.x, U1.y, U1.z, U1.w};
       // rows bmin .. bmax + 1 <= n - 1 (jr <= n - 2); span <= 2 by the bound above
       // buffer loads from a descriptor at row bmin: lane byte offset 4z,
       // scalar row offsets
-      const __amdgpu_buffer_rsrc_t rs = buf_rsrc(sino + (size_t)(unsigned)rl_i((int)rowl, l0), rows_b);
-      const int span = rl_i(spanl, l0);
+      const unsigned rowl = ((unsigned)(pkv >> 10) * (unsigned)g.na + (unsigned)(i0 + (l0 >> 2))) * (unsigned)g.nz;
+      const __amdgpu_buffer_rsrc_t rs = buf_rsrc(sino + (size_t)rowl, rows_b);
+      const int span = (pkv >> 8) & 3;
       const V T0 = tload<ZV, SPLIT>(rs, zb, zb2, 0), T1 = tload<ZV, SPLIT>(rs, zb, zb2, row_b);
       V T2 = vzero<ZV>(), T3 = vzero<ZV>();
       if (span >= 1) T2 = tload<ZV, SPLIT>(rs, zb, zb2, 2 * row_b);
@@ -883,7 +893,7 @@ __device__ __forceinline__ void bp_quad_wave(const ParDev& g, const float* __res
       // into register selects
 #define VAMX_BP_PIXEL(Q)                                                        \
   if ((pm >> (Q)) & 1u) {                                                      \
-    const int off = jq[Q];                                                     \
+    const int off = (pkv >> (2 * (Q))) & 3;                                    \
     if (off == 0) {                                                            \
       asm volatile("; vamx bp " #Q " 0");                                      \
       acc[Q] = vfma(u1[Q], T1, vfma(u0[Q], T0, acc[Q]));                       \
