@@ -623,7 +623,9 @@ __device__ __forceinline__ void axis_slots(double coord, int n, int& b, double& 
 // the previous sample's four taps in registers: between consecutive samples
 // the floor corner b moves by at most one pixel per axis, and when it moves
 // along one axis only, two of the four taps are the previous sample's.  b is
-// wave-uniform, so the case test is a scalar branch.
+// wave-uniform, so the case test is a scalar branch.  Lanes evaluate the
+// geometry of 64 samples at once; the per-sample loop takes the cell from
+// the sample's lane and its weights from the LDS buffer (fp_lds_slot).
 //
 // PAIR also projects the opposite ray, k2 = 2c + D - k at angle i + 180 deg,
 // into *acc2: its sample r2 lies on sample r = 2c - r2 of ray k (rotating by
@@ -641,6 +643,25 @@ __device__ __forceinline__ void axis_slots(double coord, int n, int& b, double& 
 // rounding error inside the -1 or n border clips up to the slice minimum
 // while one on the border stays 0, so PAIR excludes CLIP and clipped waves
 // project the two rays separately.
+// Per-wavefront LDS buffers through which the wavefront cores hand each
+// lane's sample weights to the whole wavefront: a broadcast LDS read per
+// sample replaces one cross-lane move per weight.  One buffer per kernel
+// (these functions are not templates), for blocks of at most 4 wavefronts.
+struct FpSlot {
+  float4* w;   // [64] the four bilinear tap weights of each lane's sample
+  float2* a;   // [64] its absorption weights (ray, partner ray)
+};
+__device__ __forceinline__ FpSlot fp_lds_slot() {
+  __shared__ float4 w[4][64];
+  __shared__ float2 a[4][64];
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) & 3;
+  return {w[wv], a[wv]};
+}
+__device__ __forceinline__ float4* bp_lds_slot() {
+  __shared__ float4 w[4][2][16];   // weights of the lower / upper row, 16 angles x 4 pixels
+  return w[__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) & 3][0];
+}
+
 template <int ZV, bool CLIP, bool REUSE, bool PAIR = false, bool SPLIT = false>
 __device__ __forceinline__ typename VecT<ZV>::T fp_ray_wave(const ParDev& g, const float* __restrict__ vol,
                                                              const float* __restrict__ minmax, int k, int i, int z,
@@ -675,6 +696,7 @@ __device__ __forceinline__ typename VecT<ZV>::T fp_ray_wave(const ParDev& g, con
   // the ones that left (same parity), so the window never shifts between
   // registers; each lane delivers its weights in the same parity order.
   V T00 = vzero<ZV>(), T01 = vzero<ZV>(), T10 = vzero<ZV>(), T11 = vzero<ZV>();
+  const FpSlot sl = fp_lds_slot();
   int prev = -2 * n - 2;   // no taps held
   const int r_end = PAIR ? (2 * g.c + 1 > n ? 2 * g.c + 1 : n) : n;
   const int r2_lo = 2 * g.c - n + 1;   // partner's first sample (PAIR)
@@ -716,12 +738,19 @@ __device__ __forceinline__ typename VecT<ZV>::T fp_ray_wave(const ParDev& g, con
       }
     }
     unsigned long long m = __ballot(valid);
+    // the wave's previous reads of the buffers precede these writes (LDS
+    // operations of a wavefront complete in order)
+    __builtin_amdgcn_wave_barrier();
+    sl.w[lane] = make_float4(w00, w01, w10, w11);
+    sl.a[lane] = make_float2(aw, aw2);
+    __builtin_amdgcn_wave_barrier();
     while (m) {
       const int j = __builtin_ctzll(m);
       m &= m - 1;
       const int bp = rl_i(base, j);
       const int b = bp & 0x0fffffff, par = (bp >> 28) & 3;   // par: row parity << 1 | column parity
-      const float q00 = rl_f(w00, j), q01 = rl_f(w01, j), q10 = rl_f(w10, j), q11 = rl_f(w11, j);
+      const float4 Q = sl.w[j];
+      const float q00 = Q.x, q01 = Q.y, q10 = Q.z, q11 = Q.w;
       const __amdgpu_buffer_rsrc_t rs = buf_rsrc(vol + (size_t)b * nzs, span);
       // C00..C11 name the registers of the cell's corners (0,0) .. (1,1) for
       // this parity; each arm loads the corners the previous cell lacked
@@ -781,11 +810,12 @@ __device__ __forceinline__ typename VecT<ZV>::T fp_ray_wave(const ParDev& g, con
       if constexpr (CLIP) {
         V v = vfma(q11, T11, vfma(q10, T10, vfma(q01, T01, vmul(q00, T00))));
         v = vclamp(v, lo, hi);  // every enumerated sample has an in-range tap of positive weight
-        acc = vfma(rl_f(aw, j), v, acc);
+        acc = vfma(sl.a[j].x, v, acc);
       } else if constexpr (PAIR) {
         const V v = vfma(q11, T11, vfma(q10, T10, vfma(q01, T01, vmul(q00, T00))));
-        acc = vfma(rl_f(aw, j), v, acc);
-        accp = vfma(rl_f(aw2, j), v, accp);
+        const float2 A = sl.a[j];
+        acc = vfma(A.x, v, acc);
+        accp = vfma(A.y, v, accp);
       } else {
         acc = vfma(q11, T11, vfma(q10, T10, vfma(q01, T01, vfma(q00, T00, acc))));
       }
@@ -823,8 +853,8 @@ __device__ __forceinline__ void bp_quad_wave(const ParDev& g, const float* __res
   const size_t step_j = (size_t)g.na * nzs;
   const uint32_t zb = 4u * (uint32_t)z, zb2 = 4u * (uint32_t)z2, row_b = (uint32_t)(4 * step_j);
   const uint32_t rows_b = 3 * row_b + 4u * (uint32_t)g.nz;
-  __shared__ float4 bw_all[4][2][16];   // per wavefront of the block: weights of the lower / upper row
-  float4 (&bw)[2][16] = bw_all[__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) & 3];
+  float4* const bw0 = bp_lds_slot();   // [16] lower-row weights, then [16] upper-row weights
+  float4* const bw1 = bw0 + 16;
 #pragma unroll
   for (int q = 0; q < 4; ++q) acc[q] = vzero<ZV>();
   for (int i0 = 0; i0 < g.na; i0 += 16) {
@@ -868,15 +898,15 @@ __device__ __forceinline__ void bp_quad_wave(const ParDev& g, const float* __res
     bits |= __shfl_xor(bits, 2);
     const int pk = hi >= 0 ? bits | ((hi - lo) << 8) | (lo << 10) : 0;
     __builtin_amdgcn_wave_barrier();
-    reinterpret_cast<float*>(&bw[0][a])[p] = w0;
-    reinterpret_cast<float*>(&bw[1][a])[p] = w1;
+    reinterpret_cast<float*>(&bw0[a])[p] = w0;
+    reinterpret_cast<float*>(&bw1[a])[p] = w1;
     __builtin_amdgcn_wave_barrier();
     while (am) {
       const int l0 = __builtin_ctzll(am);   // lane of pixel 0 of this angle
       am &= am - 1;
       const unsigned pm = (unsigned)(m >> l0) & 0xFu;
       const int pkv = rl_i(pk, l0);
-      const float4 U0 = bw[0][l0 >> 2], U1 = bw[1][l0 >> 2];
+      const float4 U0 = bw0[l0 >> 2], U1 = bw1[l0 >> 2];
       const float u0[4] = {U0.x, U0.y, U0.z, U0.w}, u1[4] = {U1.x, U1.y, U1.z, U1.w};
       // rows bmin .. bmax + 1 <= n - 1 (jr <= n - 2); span <= 2 by the bound above
       // buffer loads from a descriptor at row bmin: lane byte offset 4z,
