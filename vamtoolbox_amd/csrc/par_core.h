@@ -857,70 +857,75 @@ __device__ __forceinline__ void bp_quad_wave(const ParDev& g, const float* __res
   float4* const bw1 = bw0 + 16;
 #pragma unroll
   for (int q = 0; q < 4; ++q) acc[q] = vzero<ZV>();
-  for (int i0 = 0; i0 < g.na; i0 += 16) {
-    const int i = i0 + a;
-    int jr = 0;
-    float w0 = 0.f, w1 = 0.f;
-    bool valid = false;
-    if (pin && i < g.na) {
-      const double co = g.cs[2 * i], sn = g.cs[2 * i + 1];
-      const double t = x * co - y * sn;
-      int j, j1;
-      float w;
-      if (interp_node(g, t, j, j1, w)) {
-        const float aw = (g.mu != 0.0) ? abs_weight(g, t, x * sn + y * co) : 1.0f;
-        valid = aw != 0.0f;
-        if (j == g.n - 1) {        // exactly at the last node: fp[n-1]
-          jr = j - 1;
-          w1 = aw;
-        } else {
-          jr = j;
-          w0 = aw * (1.0f - w);
-          w1 = aw * w;
+  // One loop over the angles (a nested per-batch loop makes the compiler
+  // copy the accumulators between two register sets on every angle); every
+  // 16th angle first evaluates the geometry of the next 16.
+  unsigned long long m = 0ull;   // valid pixels of the batch's angles (bit 4a + p)
+  int pk = 0;
+  for (int ia = 0; ia < g.na; ++ia) {
+    if ((ia & 15) == 0) {
+      const int i = ia + a;
+      int jr = 0;
+      float w0 = 0.f, w1 = 0.f;
+      bool valid = false;
+      if (pin && i < g.na) {
+        const double co = g.cs[2 * i], sn = g.cs[2 * i + 1];
+        const double t = x * co - y * sn;
+        int j, j1;
+        float w;
+        if (interp_node(g, t, j, j1, w)) {
+          const float aw = (g.mu != 0.0) ? abs_weight(g, t, x * sn + y * co) : 1.0f;
+          valid = aw != 0.0f;
+          if (j == g.n - 1) {        // exactly at the last node: fp[n-1]
+            jr = j - 1;
+            w1 = aw;
+          } else {
+            jr = j;
+            w0 = aw * (1.0f - w);
+            w1 = aw * w;
+          }
         }
       }
+      m = __ballot(valid);
+      // the quad's row range over its valid pixels (the four lanes of one angle
+      // are lanes 4a .. 4a+3), each pixel's row offset in it and the element
+      // offset of its first row, (bmin * na + i) * nz -- lane-parallel for 16
+      // angles at once, so the per-angle loop below only broadcasts them
+      int lo = valid ? jr : 0x7fffffff, hi = valid ? jr : -1;
+      lo = min(lo, __shfl_xor(lo, 1));
+      lo = min(lo, __shfl_xor(lo, 2));
+      hi = max(hi, __shfl_xor(hi, 1));
+      hi = max(hi, __shfl_xor(hi, 2));
+      // per angle: bits 0-7 the pixels' row offsets from bmin, 8-9 the span
+      // bmax - bmin, 10.. bmin (n < 16384)
+      int bits = valid ? (jr - lo) << (2 * p) : 0;
+      bits |= __shfl_xor(bits, 1);
+      bits |= __shfl_xor(bits, 2);
+      pk = hi >= 0 ? bits | ((hi - lo) << 8) | (lo << 10) : 0;
+      __builtin_amdgcn_wave_barrier();
+      reinterpret_cast<float*>(&bw0[a])[p] = w0;
+      reinterpret_cast<float*>(&bw1[a])[p] = w1;
+      __builtin_amdgcn_wave_barrier();
     }
-    const unsigned long long m = __ballot(valid);
-    unsigned long long am = (m | (m >> 1) | (m >> 2) | (m >> 3)) & 0x1111111111111111ull;
-    // the quad's row range over its valid pixels (the four lanes of one angle
-    // are lanes 4a .. 4a+3), each pixel's row offset in it and the element
-    // offset of its first row, (bmin * na + i) * nz -- lane-parallel for 16
-    // angles at once, so the per-angle loop below only broadcasts them
-    int lo = valid ? jr : 0x7fffffff, hi = valid ? jr : -1;
-    lo = min(lo, __shfl_xor(lo, 1));
-    lo = min(lo, __shfl_xor(lo, 2));
-    hi = max(hi, __shfl_xor(hi, 1));
-    hi = max(hi, __shfl_xor(hi, 2));
-    // per angle: bits 0-7 the pixels' row offsets from bmin, 8-9 the span
-    // bmax - bmin, 10.. bmin (n < 16384)
-    int bits = valid ? (jr - lo) << (2 * p) : 0;
-    bits |= __shfl_xor(bits, 1);
-    bits |= __shfl_xor(bits, 2);
-    const int pk = hi >= 0 ? bits | ((hi - lo) << 8) | (lo << 10) : 0;
-    __builtin_amdgcn_wave_barrier();
-    reinterpret_cast<float*>(&bw0[a])[p] = w0;
-    reinterpret_cast<float*>(&bw1[a])[p] = w1;
-    __builtin_amdgcn_wave_barrier();
-    while (am) {
-      const int l0 = __builtin_ctzll(am);   // lane of pixel 0 of this angle
-      am &= am - 1;
-      const unsigned pm = (unsigned)(m >> l0) & 0xFu;
-      const int pkv = rl_i(pk, l0);
-      const float4 U0 = bw0[l0 >> 2], U1 = bw1[l0 >> 2];
-      const float u0[4] = {U0.x, U0.y, U0.z, U0.w}, u1[4] = {U1.x, U1.y, U1.z, U1.w};
-      // rows bmin .. bmax + 1 <= n - 1 (jr <= n - 2); span <= 2 by the bound above
-      // buffer loads from a descriptor at row bmin: lane byte offset 4z,
-      // scalar row offsets
-      const unsigned rowl = ((unsigned)(pkv >> 10) * (unsigned)g.na + (unsigned)(i0 + (l0 >> 2))) * (unsigned)g.nz;
-      const __amdgpu_buffer_rsrc_t rs = buf_rsrc(sino + (size_t)rowl, rows_b);
-      const int span = (pkv >> 8) & 3;
-      const V T0 = tload<ZV, SPLIT>(rs, zb, zb2, 0), T1 = tload<ZV, SPLIT>(rs, zb, zb2, row_b);
-      V T2 = vzero<ZV>(), T3 = vzero<ZV>();
-      if (span >= 1) T2 = tload<ZV, SPLIT>(rs, zb, zb2, 2 * row_b);
-      if (span >= 2) T3 = tload<ZV, SPLIT>(rs, zb, zb2, 3 * row_b);
-      // one branch per pixel and row offset (the offsets are wave-uniform);
-      // the empty asm markers keep the compiler from turning the branches
-      // into register selects
+    const int l0 = 4 * (ia & 15);   // lane of pixel 0 of this angle
+    const unsigned pm = (unsigned)(m >> l0) & 0xFu;
+    if (!pm) continue;              // no valid pixel at this angle
+    const int pkv = rl_i(pk, l0);
+    const float4 U0 = bw0[l0 >> 2], U1 = bw1[l0 >> 2];
+    const float u0[4] = {U0.x, U0.y, U0.z, U0.w}, u1[4] = {U1.x, U1.y, U1.z, U1.w};
+    // rows bmin .. bmax + 1 <= n - 1 (jr <= n - 2); span <= 2 by the bound above
+    // buffer loads from a descriptor at row bmin: lane byte offset 4z,
+    // scalar row offsets
+    const unsigned rowl = ((unsigned)(pkv >> 10) * (unsigned)g.na + (unsigned)ia) * (unsigned)g.nz;
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(sino + (size_t)rowl, rows_b);
+    const int span = (pkv >> 8) & 3;
+    const V T0 = tload<ZV, SPLIT>(rs, zb, zb2, 0), T1 = tload<ZV, SPLIT>(rs, zb, zb2, row_b);
+    V T2 = vzero<ZV>(), T3 = vzero<ZV>();
+    if (span >= 1) T2 = tload<ZV, SPLIT>(rs, zb, zb2, 2 * row_b);
+    if (span >= 2) T3 = tload<ZV, SPLIT>(rs, zb, zb2, 3 * row_b);
+    // one branch per pixel and row offset (the offsets are wave-uniform);
+    // the empty asm markers keep the compiler from turning the branches
+    // into register selects
 #define VAMX_BP_PIXEL(Q)                                                        \
   if ((pm >> (Q)) & 1u) {                                                      \
     const int off = (pkv >> (2 * (Q))) & 3;                                    \
@@ -935,12 +940,11 @@ __device__ __forceinline__ void bp_quad_wave(const ParDev& g, const float* __res
       acc[Q] = vfma(u1[Q], T3, vfma(u0[Q], T2, acc[Q]));                       \
     }                                                                          \
   }
-      VAMX_BP_PIXEL(0)
-      VAMX_BP_PIXEL(1)
-      VAMX_BP_PIXEL(2)
-      VAMX_BP_PIXEL(3)
+    VAMX_BP_PIXEL(0)
+    VAMX_BP_PIXEL(1)
+    VAMX_BP_PIXEL(2)
+    VAMX_BP_PIXEL(3)
 #undef VAMX_BP_PIXEL
-    }
   }
 }
 
