@@ -169,6 +169,34 @@ def test_wave_kernels_parity(gpu, case, positive):
     assert rel_l2(fw, fl) <= 1e-6 and rel_l2(bw, bl) <= 1e-6
 
 
+@pytest.mark.parametrize("na", [1, 15, 16, 17, 47])
+@pytest.mark.parametrize("nz", [40, 64, 128, 256])
+def test_angle_batch_boundaries(gpu, na, nz):
+    """The wavefront kernels evaluate geometry for 16 angles (back
+    projection) or 64 samples (forward) at a time: angle counts below, at and
+    just past a batch, with every z width (1, 2, 4 per lane), match the oracle
+    and the per-lane-geometry kernels."""
+    import torch
+    import oracle.d1 as d1
+    from vamtoolbox_amd.parallel import ParallelPlan
+    n = 21
+    ang = np.linspace(3.0, 363.0, na, endpoint=False)
+    rng = np.random.default_rng(na * 1000 + nz)
+    vol = rng.uniform(-1, 1, (n, n, nz))
+    sino = rng.uniform(0, 1, (n, na, nz))
+    kw = dict(mu=0.015, det_offset=1.0)
+    tv = torch.tensor(vol, dtype=torch.float32, device=gpu)
+    ts = torch.tensor(sino, dtype=torch.float32, device=gpu)
+    wave = ParallelPlan(n, nz, ang, device=gpu, **kw)
+    lane = ParallelPlan(n, nz, ang, device=gpu, per_lane_geometry=True, **kw)
+    fw, bw = wave.forward(tv).cpu().numpy(), wave.backward(ts).cpu().numpy()
+    fl, bl = lane.forward(tv).cpu().numpy(), lane.backward(ts).cpu().numpy()
+    e1, e2 = rel_l2(fw, d1.forward(vol, ang, **kw)), rel_l2(bw, d1.backward(sino, ang, **kw))
+    print("na %d nz %d: rel L2 forward %.2e backward %.2e" % (na, nz, e1, e2))
+    assert e1 <= TOL and e2 <= TOL
+    assert rel_l2(fw, fl) <= 1e-6 and rel_l2(bw, bl) <= 1e-6
+
+
 def test_tuning_settings_agree(gpu):
     """Every work order (tile shape, z values per lane) gives the oracle's result."""
     import torch
