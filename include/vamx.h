@@ -93,8 +93,8 @@ int vamx_par_backward_adjoint(vamx_plan* plan, const float* vol, float* sino, vo
  * give bitwise equal sinograms; opposite-angle pairing sums the second ray
  * of a pair in reverse sample order. */
 typedef struct vamx_par_tuning {
-  int32_t fp_tile_angles;   /* forward: angles per tile of consecutive waves (default 8) */
-  int32_t fp_tile_rays;     /* forward: rays per tile (default 8)                        */
+  int32_t fp_tile_angles;   /* forward: angles per tile of consecutive waves (default 10) */
+  int32_t fp_tile_rays;     /* forward: rays per tile (default 4)                         */
   int32_t fp_zv;            /* forward: z values per lane, 1, 2 or 4 (default 4)         */
   int32_t fp_reuse;         /* forward: 1 = keep the previous sample's taps in registers (default 1) */
   int32_t bp_tile;          /* back projection: tile edge in 2x2 quads of voxel columns (default 8) */

@@ -488,10 +488,10 @@ extern "C" int vamx_par_create(const vamx_par_geom* geom, vamx_plan** out_plan) 
   g.pair_first2 = first2;
   g.fp_pair = 1;
   g.z_split = 1;
-  // defaults: ParallelPlan.autotune's choice at 256^3 x 360 angles on MI355X
-  // (8 angle pairs x 8 rays, paired)
-  g.fp_ta = 8;
-  g.fp_tk = 8;
+  // defaults: the fastest measured work order at 256^3 x 360 angles on
+  // MI355X (10 angle pairs x 4 rays, paired: 10 divides the 180 pairs)
+  g.fp_ta = 10;
+  g.fp_tk = 4;
   g.fp_zv = 4;
   g.fp_reuse = 1;
   g.bp_tile = 8;

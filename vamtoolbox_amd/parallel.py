@@ -135,7 +135,8 @@ class ParallelPlan:
 
     # (angles -- or opposite-angle pairs -- per tile, rays per tile, z values
     # per lane, register tap reuse, opposite-angle pairing)
-    FP_CANDIDATES = [(8, 8, 4, 1, 1), (16, 8, 4, 1, 1), (8, 16, 4, 1, 1), (16, 4, 4, 1, 1), (4, 16, 4, 1, 1),
+    FP_CANDIDATES = [(10, 4, 4, 1, 1), (10, 8, 4, 1, 1), (8, 8, 4, 1, 1), (12, 4, 4, 1, 1), (16, 8, 4, 1, 1),
+                     (8, 16, 4, 1, 1), (16, 4, 4, 1, 1), (4, 16, 4, 1, 1),
                      (32, 8, 4, 1, 1), (8, 8, 2, 1, 1), (16, 16, 4, 0, 1), (16, 8, 4, 1, 0), (8, 16, 4, 1, 0),
                      (64, 8, 4, 1, 0), (32, 32, 4, 1, 0), (64, 16, 2, 1, 0), (32, 32, 4, 0, 0), (32, 32, 1, 0, 0)]
     # (tile edge in 2x2 voxel quads, z values per lane)
