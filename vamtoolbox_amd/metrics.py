@@ -26,7 +26,7 @@ def calcVER(target, recon, layerwise=False):
         gt, vt, num = target._gel_t, target._void_t, target._num_gv
         if target._n_gel == 0:
             raise ValueError("zero-size array to reduction operation minimum which has no identity")
-        inf = torch.tensor(float("inf"), dtype=recon.dtype, device=recon.device)
+        inf = float("inf")
         if not layerwise:
             min_gel = torch.amin(torch.where(gt, recon, inf))
             return float(torch.count_nonzero(vt & (recon >= min_gel)).item()) / num

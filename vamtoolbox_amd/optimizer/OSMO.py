@@ -34,7 +34,9 @@ def minimizeOSMO(target_geo, proj_geo, options):
         b = b / torch.amax(b)
         if bit_depth is not None:
             b = data.discretize(b, bit_depth, [torch.amin(b), torch.amax(b)])
-        return torch.where(b < min_proj_val, torch.full_like(b, min_proj_val), b)
+        # b < min_proj_val -> min_proj_val (OSMO.py:65, 103): one clamp,
+        # which also keeps NaN as the comparison does
+        return torch.clamp(b, min=min_proj_val)
 
     def reconstruct(b):
         x = A.backward(b).reshape(T.shape)      # the algebraic propagator returns flat arrays
