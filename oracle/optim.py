@@ -6,7 +6,8 @@ TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
 * ``filter_target_osmo``                    vamtoolbox/util/data.py:43-113
 * ``Response``                              vamtoolbox/response.py:12-291
 * ``osmo``                                  vamtoolbox/optimizer/OSMO.py:9-178
-* ``cal``                                   vamtoolbox/optimizer/CAL.py:10-218 (d_h given)
+* ``cal``                                   vamtoolbox/optimizer/CAL.py:10-218
+* ``cal_threshold``                         vamtoolbox/optimizer/CAL.py:39-110 (d_h None)
 * ``bclp``                                  vamtoolbox/optimizer/BCLP.py:138-641
 * ``ver``                                   vamtoolbox/metrics.py:47-87
 * ``fbp``                                   vamtoolbox/optimizer/FBP.py:9-76
@@ -155,8 +156,22 @@ def osmo(target, fwd, bwd, n_iter, d_h=0.8, d_l=0.7, filt="ram-lak", inhibition=
     return b, x, err
 
 
+def cal_threshold(x, gel, void, sigmoid):
+    """CAL.py:39-110 (d_h None): the threshold of np.linspace(min, max, 100)
+    maximising (gel voxels above it) / #gel - (void voxels above it) / #void,
+    and the sigmoid 1 / (1 + exp(-(sigmoid * (x - threshold))))."""
+    ths = np.linspace(np.amin(x), np.amax(x), 100)
+    score = np.zeros(len(ths))
+    for i, t in enumerate(ths):
+        above = np.where(x > t, 1, 0)
+        score[i] = np.sum(above[gel]) / np.count_nonzero(gel) - np.sum(above[void]) / np.count_nonzero(void)
+    th = ths[np.argmax(score)]
+    return 1 / (1 + np.exp(-(float(sigmoid) * (x - th)))), th
+
+
 def cal(target, fwd, bwd, n_iter, d_h=0.8, filt="ram-lak", lr=0.01, momentum=0.0, sigmoid=0.01):
-    """CAL.py:116-218 with d_h given (sigmoid threshold)."""
+    """CAL.py:116-218: sigmoid threshold at d_h, or (d_h None) the score-maximising
+    threshold of cal_threshold."""
     gel, void = circle_inds(target)
     err = np.full(n_iter, np.nan)
     b = np.clip(filter_sinogram(fwd(target), filt), 0, None)
@@ -166,7 +181,10 @@ def cal(target, fwd, bwd, n_iter, d_h=0.8, filt="ram-lak", lr=0.01, momentum=0.0
     err[0] = ver(gel, void, x)
     b_opt = b
     for it in range(1, n_iter):
-        xt = 1 / (1 + np.exp(-(x - d_h) * (1 / sigmoid)))
+        if d_h is None:
+            xt, _ = cal_threshold(x, gel, void, sigmoid)
+        else:
+            xt = 1 / (1 + np.exp(-(x - d_h) * (1 / sigmoid)))
         db = fwd(xt - target)
         grad = db * (1 - momentum) + db_prev * momentum / (1 - momentum ** (it + 1))
         b = np.clip(b - lr * grad, 0, None)

@@ -69,6 +69,31 @@ def test_cal_matches_oracle(gpu):
     assert e_b <= 1e-4 and e_x <= 1e-4
 
 
+def test_cal_threshold_search_matches_oracle(gpu):
+    """CAL without d_h (CAL.py:39-110): each iteration picks the threshold of
+    100 that maximises gel-above minus void-above fractions; the device
+    counts (sorted values) give the oracle's threshold and iterates."""
+    import torch
+    import oracle.optim as oo
+    import vamtoolbox_amd as vam
+    from vamtoolbox_amd.optimizer.CAL import _threshold_reconstruction
+    tg, pg, ang = _problem(seed=3)
+    gel, void = np.asarray(tg.gel_inds), np.asarray(tg.void_inds)
+    rng = np.random.default_rng(8)
+    x = rng.random(tg.array.shape).astype(np.float32)
+    xt, th = _threshold_reconstruction(torch.tensor(x, device=gpu), tg, 0.5)
+    xt_ref, th_ref = oo.cal_threshold(x.astype(np.float64), gel, void, 0.5)
+    assert th == th_ref
+    np.testing.assert_allclose(xt.cpu().numpy(), xt_ref, rtol=1e-12, atol=0)
+    opts = vam.optimize.Options(method="CAL", n_iter=6, learning_rate=0.002, sigmoid=20.0, d_h=None)
+    sino, recon, err = vam.optimize.optimize(tg, pg, opts)
+    fwd, bwd = _oracle_ops(ang)
+    b_ref, x_ref, err_ref = oo.cal(tg.array, fwd, bwd, 6, d_h=None, lr=0.002, sigmoid=20.0)
+    e_b, e_x = rel_l2(sino.array, b_ref), rel_l2(recon.array, x_ref)
+    print("CAL (threshold search) rel L2 sinogram %.2e recon %.2e" % (e_b, e_x))
+    assert e_b <= 1e-4 and e_x <= 1e-4
+
+
 def _bclp_setup(gpu, kind="analytical", n_iter=20, **pg_kw):
     import oracle.optim as oo
     import vamtoolbox_amd as vam

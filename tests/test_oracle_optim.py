@@ -128,6 +128,34 @@ def test_osmo_cal_oracle_tiny_run():
     assert np.all(np.isfinite(err2))
 
 
+def test_cal_threshold_separated_classes():
+    """CAL.py:39-110 on separable data: with every gel voxel in [0.6, 1] and
+    every void voxel in [0, 0.4], the score reaches its maximum 1 exactly for
+    thresholds in [0.4, 0.6), and argmax keeps the first of them; the sigmoid
+    is centred there.  A wrong comparison (>=), a swapped class or a last-max
+    tie rule picks another threshold."""
+    rng = np.random.default_rng(5)
+    gel = rng.random((12, 12, 3)) < 0.4
+    void = ~gel
+    x = np.where(gel, rng.uniform(0.6, 1.0, gel.shape), rng.uniform(0.0, 0.4, gel.shape))
+    x.flat[np.flatnonzero(gel)[0]] = 1.0     # min 0 <= x <= max 1: thresholds i/99
+    x.flat[np.flatnonzero(void)[0]] = 0.0
+    xt, th = oo.cal_threshold(x, gel, void, sigmoid=7.0)
+    ths = np.linspace(0.0, 1.0, 100)
+    first = ths[np.argmax(ths >= np.max(x[void]))]
+    assert first < np.min(x[gel])
+    assert th == first
+    np.testing.assert_allclose(xt, 1 / (1 + np.exp(-7.0 * (x - first))), rtol=1e-15)
+    # brute force on a tiny case: every threshold's score by explicit counting
+    xs = rng.random((4, 4, 2))
+    g2 = rng.random(xs.shape) < 0.5
+    v2 = ~g2
+    _, th2 = oo.cal_threshold(xs, g2, v2, sigmoid=1.0)
+    cand = np.linspace(xs.min(), xs.max(), 100)
+    sc = [sum(1 for a in xs[g2] if a > t) / g2.sum() - sum(1 for a in xs[v2] if a > t) / v2.sum() for t in cand]
+    assert th2 == cand[int(np.argmax(sc))]
+
+
 def test_pm_gradient_finite_differences():
     """PM loss gradient (PM.py:117-140) on an explicit linear pair where the
     forward operator is the transpose of the backward one: the gradient is

@@ -10,6 +10,19 @@ except ImportError:  # pragma: no cover
     torch = None
 
 
+def device_masks(target, device):
+    """The target's gel / void masks as device tensors and their voxel counts,
+    cached on the target (built once per device)."""
+    g = getattr(target, "_gel_t", None)
+    if g is None or g.device != torch.device(device):
+        target._gel_t = torch.as_tensor(np.asarray(target.gel_inds), device=device)
+        target._void_t = torch.as_tensor(np.asarray(target.void_inds), device=device)
+        target._n_gel = int(np.sum(target.gel_inds))
+        target._n_void = int(np.sum(target.void_inds))
+        target._num_gv = target._n_gel + target._n_void
+    return target._gel_t, target._void_t, target._n_gel, target._n_void
+
+
 def calcVER(target, recon, layerwise=False):
     """Volumetric error rate: #(void voxels >= min gel dose) / (#gel + #void)."""
     gel_inds, void_inds = target.gel_inds, target.void_inds
@@ -17,13 +30,8 @@ def calcVER(target, recon, layerwise=False):
         # masked reductions on the device (no compaction, one host sync for
         # the result); the gel / void masks and their count are cached on the
         # target
-        g = getattr(target, "_gel_t", None)
-        if g is None or g.device != recon.device:
-            target._gel_t = torch.as_tensor(np.asarray(gel_inds), device=recon.device)
-            target._void_t = torch.as_tensor(np.asarray(void_inds), device=recon.device)
-            target._num_gv = int(np.sum(gel_inds) + np.sum(void_inds))
-            target._n_gel = int(np.sum(gel_inds))
-        gt, vt, num = target._gel_t, target._void_t, target._num_gv
+        gt, vt, _, _ = device_masks(target, recon.device)
+        num = target._num_gv
         if target._n_gel == 0:
             raise ValueError("zero-size array to reduction operation minimum which has no identity")
         inf = float("inf")

@@ -12,16 +12,21 @@ from ..projectorconstructor import projectorconstructor
 from ..util import data
 
 
-def _threshold_reconstruction(x, target_geo, T, sigmoid_strength):
-    """CAL.py:217-292: threshold maximising gel-hit minus void-hit fractions."""
-    gel = torch.as_tensor(target_geo.gel_inds, device=x.device)
-    void = torch.as_tensor(target_geo.void_inds, device=x.device)
-    ths = torch.linspace(float(torch.amin(x)), float(torch.amax(x)), 100, device=x.device, dtype=torch.float64)
-    ngel = max(int(gel.sum()), 1)
-    nvoid = max(int(void.sum()), 1)
+def _threshold_reconstruction(x, target_geo, sigmoid_strength):
+    """CAL.py:39-110: the threshold (of 100 from min to max) maximising the
+    gel fraction above it minus the void fraction above it, then a sigmoid
+    around it.  The thresholds and scores are the reference's float64 NumPy
+    arithmetic on the host; the counts #(x > t) for all thresholds come from
+    the sorted gel and void values on the device."""
+    gel, void, n_gel, n_void = metrics.device_masks(target_geo, x.device)
     xd = x.double()
-    score = torch.stack([(((xd > t) & gel).sum() / ngel) - (((xd > t) & void).sum() / nvoid) for t in ths])
-    th = ths[int(torch.argmax(score))]
+    ths = np.linspace(float(torch.amin(xd)), float(torch.amax(xd)), 100)
+    t = torch.as_tensor(ths, device=x.device)
+    c_gel = (n_gel - torch.searchsorted(torch.sort(xd[gel]).values, t, right=True)).cpu().numpy()
+    c_void = (n_void - torch.searchsorted(torch.sort(xd[void]).values, t, right=True)).cpu().numpy()
+    with np.errstate(divide="ignore", invalid="ignore"):
+        score = c_gel / n_gel - c_void / n_void
+    th = float(ths[int(np.argmax(score))])
     return 1.0 / (1.0 + torch.exp(-(float(sigmoid_strength) * (xd - th)))), th
 
 
@@ -49,7 +54,7 @@ def minimizeCAL(target_geo, proj_geo, options):
         if options.d_h is not None:
             x_thresh = data.sigmoid(x - options.d_h, options.sigmoid)
         else:
-            x_thresh, _ = _threshold_reconstruction(x, target_geo, T, options.sigmoid)
+            x_thresh, _ = _threshold_reconstruction(x, target_geo, options.sigmoid)
             x_thresh = x_thresh.float()
         delta_x = x_thresh - T
         delta_b = A.forward(delta_x)
