@@ -130,6 +130,38 @@ def test_bclp_fused_matches_oracle(gpu, kind):
     assert e_g <= 1e-4 and e_d <= 1e-4 and e_l <= 1e-4
 
 
+@pytest.mark.parametrize("method", ["OSMO", "BCLP"])
+def test_fifty_iterations_match_oracle(gpu, method):
+    """50 iterations (the length the iterate tolerance is quoted at): OSMO
+    and the fused BCLP iterates within 1e-4 relative L2 of the float64
+    oracle loops."""
+    import oracle.optim as oo
+    import vamtoolbox_amd as vam
+    n_iter = 50
+    if method == "OSMO":
+        tg, pg, ang = _problem(seed=4)
+        opts = vam.optimize.Options(method="OSMO", n_iter=n_iter, d_h=0.85, d_l=0.6, filter="ram-lak")
+        sino, recon, err = vam.optimize.optimize(tg, pg, opts)
+        fwd, bwd = _oracle_ops(ang)
+        b_ref, x_ref, err_ref = oo.osmo(tg.array, fwd, bwd, n_iter, d_h=0.85, d_l=0.6)
+        e1, e2 = rel_l2(sino.array, b_ref), rel_l2(recon.array, x_ref)
+        print("OSMO x50 rel L2 sinogram %.2e recon %.2e; VER %.4f vs %.4f" % (e1, e2, err[-1], err_ref[-1]))
+    else:
+        tg, pg, ang, rm, ro, g0, fwd, bwd, lr = _bclp_setup(gpu, "analytical")
+        opts = vam.optimize.Options(method="BCLP", n_iter=n_iter, response_model=rm, eps=0.05, p=2, q=1,
+                                    learning_rate=lr, g0=g0, fused=True)
+        g, dose, mapped, logs = vam.optimize.optimize(tg, pg, opts, output="components")
+        g_ref, dose_ref, m_ref, loss_ref = oo.bclp(tg.array, fwd, bwd, ro, n_iter, g0, eps=0.05, lr=lr)
+        e1, e2 = rel_l2(g, g_ref), rel_l2(dose, dose_ref)
+        # the band is met before the end (loss 0 exactly): the loss error is
+        # measured against the initial loss
+        e_l = np.max(np.abs(logs.loss - loss_ref)) / loss_ref[0]
+        print("BCLP x50 rel L2 g %.2e dose %.2e; loss %.2e (%.4g -> %.4g)" % (e1, e2, e_l, loss_ref[0],
+                                                                              loss_ref[-1]))
+        assert e_l <= 1e-4
+    assert e1 <= 1e-4 and e2 <= 1e-4
+
+
 def test_bclp_fused_unfused_graph_agree(gpu):
     import vamtoolbox_amd as vam
     tg, pg, ang, rm, ro, g0, fwd, bwd, lr = _bclp_setup(gpu, "analytical", mu=0.01, det_offset=2.0)
