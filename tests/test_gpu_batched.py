@@ -81,6 +81,28 @@ def test_batched_osmo_matches_single_problems(gpu):
         np.testing.assert_allclose(e, err, atol=1e-6)
 
 
+def test_batched_osmo_autotuned_plans(gpu):
+    """With autotune every problem's plan takes its own fastest work order;
+    the batched projections agree with the default work order to float32
+    rounding (tile shapes are bitwise, pairing sums in another order)."""
+    import vamtoolbox_amd as vam
+    from vamtoolbox_amd.batch import OSMOBatch
+    from tests.test_gpu_optim import _ellipsoids
+    specs = [(40, 64, 60, 0.01), (36, 77, 45, 0.0), (28, 130, 30, 0.02)]
+    tgs = [vam.geometry.TargetGeometry(target=_ellipsoids(n, nz, seed=30 + i)) for i, (n, nz, _, _) in enumerate(specs)]
+    pgs = [vam.geometry.ProjectionGeometry(np.linspace(0, 360, na, endpoint=False), "parallel", CUDA=True,
+                                           absorption_per_voxel=mu) for _, _, na, mu in specs]
+    opts = vam.optimize.Options(method="OSMO", n_iter=3, d_h=0.85, d_l=0.6, filter="ram-lak")
+    tuned = OSMOBatch(tgs, pgs, opts, device=gpu, autotune=True)
+    plain = OSMOBatch(tgs, pgs, opts, device=gpu)
+    for p in tuned.batch.plans:
+        print(p.get_tuning())
+    eb = rel_l2(tuned.b.cpu().numpy(), plain.b.cpu().numpy())
+    ex = rel_l2(tuned.x.cpu().numpy(), plain.x.cpu().numpy())
+    print("autotuned vs default: sinogram %.2e recon %.2e" % (eb, ex))
+    assert eb <= 1e-6 and ex <= 1e-6
+
+
 def test_batched_osmo_graph_replays_equal_eager(gpu):
     """Every replay of the captured iteration equals the eager iteration, with
     eager work and host synchronisation between the replays."""

@@ -143,7 +143,7 @@ class OSMOBatch:
     from a captured HIP graph when ``hip_graph`` is set.
     """
 
-    def __init__(self, target_geos, proj_geos, options, device=None):
+    def __init__(self, target_geos, proj_geos, options, device=None, autotune=False):
         from ._device import require_gpu
         self.device = require_gpu(device)
         self.target_geos = list(target_geos)
@@ -151,7 +151,13 @@ class OSMOBatch:
         if len(self.target_geos) != len(self.proj_geos):
             raise ValueError("one projection geometry per target")
         self.options = options
-        self.batch = ParallelBatch([_plan_for(t, p, self.device) for t, p in zip(self.target_geos, self.proj_geos)])
+        plans = [_plan_for(t, p, self.device) for t, p in zip(self.target_geos, self.proj_geos)]
+        if autotune:
+            # each problem's work order (tile shape, z values per lane,
+            # pairing) timed on its own shape; cached per shape
+            for p in plans:
+                p.autotune(reps=1)
+        self.batch = ParallelBatch(plans)
         B = self.batch
         dev = self.device
         self.d_h, self.d_l = float(options.d_h), float(options.d_l)
@@ -227,15 +233,17 @@ class OSMOBatch:
                 for b, x, p in zip(self.b_views, self.x_views, self.proj_geos)]
 
 
-def minimizeOSMOBatched(target_geos, proj_geos, options, hip_graph: bool = True, device=None):
+def minimizeOSMOBatched(target_geos, proj_geos, options, hip_graph: bool = True, device=None, autotune=False):
     """OSMO on k independent problems at once.
 
     Returns a list of (Sinogram, Reconstruction, error) per problem, with the
     same meaning as ``minimizeOSMO``'s return value; ``error`` is the VER per
     iteration (iteration 0 = the initialisation).  With ``hip_graph`` the
-    first iteration runs eagerly and the rest replay a captured graph.
+    first iteration runs eagerly and the rest replay a captured graph.  With
+    ``autotune`` every problem's plan first times the candidate work orders
+    on its own shape (``ParallelPlan.autotune``).
     """
-    st = OSMOBatch(target_geos, proj_geos, options, device=device)
+    st = OSMOBatch(target_geos, proj_geos, options, device=device, autotune=autotune)
     n_iter = options.n_iter
     err = np.full((st.batch.k, n_iter), np.nan)
     t0 = time.perf_counter()
