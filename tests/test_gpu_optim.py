@@ -252,10 +252,11 @@ def test_autograd_dot_products(gpu):
     assert abs(lhs - rhs) / abs(lhs) <= 1e-6
 
 
-@pytest.mark.parametrize("nz", [64, 40])
+@pytest.mark.parametrize("nz", [64, 40, 128, 256])
 def test_bclp_fused_wave_path_matches_oracle(gpu, nz):
-    """nz = 64 and 40 (a partial z chunk): the fused BCLP kernels take the
-    wavefront-uniform path."""
+    """nz = 64 and 40 (a partial z chunk; one z per lane), 128 and 256 (two
+    and four z per lane, the benchmark's kernels): the fused BCLP kernels take
+    the wavefront-uniform path."""
     import oracle.optim as oo
     import vamtoolbox_amd as vam
     tg, pg, ang = _problem(n=24, nz=nz, na=30, seed=4, gray=True, absorption_per_voxel=0.01, det_offset=1.0)
