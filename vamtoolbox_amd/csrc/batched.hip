@@ -291,9 +291,9 @@ int run_batched(vamx_plan* const* plans, int n, const float* const* ins, float* 
   }
   BatchScratch& sc = g_scratch;
   if (sc.device != dev) {
-    for (auto& t : sc.tables) hipFree(t.d_items);
+    for (auto& t : sc.tables) (void)hipFree(t.d_items);
     sc.tables.clear();
-    if (sc.d_counter) hipFree(sc.d_counter);
+    if (sc.d_counter) (void)hipFree(sc.d_counter);
     sc.d_counter = nullptr;
     VAMX_HIP(hipMalloc((void**)&sc.d_counter, sizeof(unsigned long long)));
     sc.device = dev;
@@ -315,14 +315,14 @@ int run_batched(vamx_plan* const* plans, int n, const float* const* ins, float* 
     // hipStreamSynchronize fails and the error is returned)
     VAMX_HIP(hipStreamSynchronize(s));
     if (sc.tables.size() >= kMaxTables) {
-      hipFree(sc.tables.front().d_items);
+      (void)hipFree(sc.tables.front().d_items);
       sc.tables.erase(sc.tables.begin());
     }
     BatchTable nt;
     VAMX_HIP(hipMalloc((void**)&nt.d_items, bytes));
     hipError_t e = hipMemcpy(nt.d_items, raw, bytes, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-      hipFree(nt.d_items);
+      (void)hipFree(nt.d_items);
       return hip_check(e, "hipMemcpy(batch table)");
     }
     nt.bytes.assign(raw, raw + bytes);

@@ -27,22 +27,22 @@ unsigned blocks_for(long long total, int block) {
 extern "C" void vamx_destroy(vamx_plan* p) {
   if (!p) return;
   int cur = 0;
-  hipGetDevice(&cur);
-  if (cur != p->device) hipSetDevice(p->device);
-  hipDeviceSynchronize();
-  if (p->d_mats) hipFree(p->d_mats);
-  if (p->d_cs) hipFree(p->d_cs);
-  if (p->d_outside) hipFree(p->d_outside);
-  if (p->d_minmax_part) hipFree(p->d_minmax_part);
-  if (p->d_minmax) hipFree(p->d_minmax);
-  if (p->d_occ) hipFree(p->d_occ);
-  if (p->d_occ_active) hipFree(p->d_occ_active);
-  if (p->d_units) hipFree(p->d_units);
-  if (p->d_rt_tables) hipFree(p->d_rt_tables);
-  if (p->d_accum) hipFree(p->d_accum);
-  if (p->d_gmax) hipFree(p->d_gmax);
-  if (p->d_stats) hipFree(p->d_stats);
-  if (cur != p->device) hipSetDevice(cur);
+  (void)hipGetDevice(&cur);
+  if (cur != p->device) (void)hipSetDevice(p->device);
+  (void)hipDeviceSynchronize();
+  if (p->d_mats) (void)hipFree(p->d_mats);
+  if (p->d_cs) (void)hipFree(p->d_cs);
+  if (p->d_outside) (void)hipFree(p->d_outside);
+  if (p->d_minmax_part) (void)hipFree(p->d_minmax_part);
+  if (p->d_minmax) (void)hipFree(p->d_minmax);
+  if (p->d_occ) (void)hipFree(p->d_occ);
+  if (p->d_occ_active) (void)hipFree(p->d_occ_active);
+  if (p->d_units) (void)hipFree(p->d_units);
+  if (p->d_rt_tables) (void)hipFree(p->d_rt_tables);
+  if (p->d_accum) (void)hipFree(p->d_accum);
+  if (p->d_gmax) (void)hipFree(p->d_gmax);
+  if (p->d_stats) (void)hipFree(p->d_stats);
+  if (cur != p->device) (void)hipSetDevice(cur);
   delete p;
 }
 

@@ -537,7 +537,7 @@ extern "C" int vamx_par_create(const vamx_par_geom* geom, vamx_plan** out_plan) 
       e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipDeviceSynchronize();
-    if (d_ins) hipFree(d_ins);
+    if (d_ins) (void)hipFree(d_ins);
     if (e != hipSuccess) {
       vamx_destroy(p);
       return hip_check(e, "vamx_par_create: occlusion sinogram");
