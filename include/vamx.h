@@ -270,8 +270,11 @@ int vamx_sino_resample(const float* sino, int32_t n_r, int32_t n_a, int32_t nz, 
  * with vamx_par_create) in a single persistent launch; vols[i]/sinos[i] are
  * device pointers of problem i.  Work is balanced by n*n_angles*n*nz.  In the
  * forward direction a problem whose wavefront work alone fills the GPU
- * (16 waves per CU) is projected by its own single-problem launch on the same
- * stream instead (results as vamx_par_forward). */
+ * (16 waves per CU) is projected by its own single-problem launch instead
+ * (results as vamx_par_forward); those launches are spread over the caller's
+ * stream and two library-owned side streams that fork from and join back to
+ * it through events, so the call is ordered on `stream` as a whole and can be
+ * captured in a HIP graph. */
 int vamx_par_forward_batched(vamx_plan* const* plans, int32_t n_plans,
                              const float* const* vols, float* const* sinos, void* stream);
 int vamx_par_backward_batched(vamx_plan* const* plans, int32_t n_plans,

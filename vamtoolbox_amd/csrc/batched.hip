@@ -1,7 +1,8 @@
 // Batched parallel-beam projection of many independent problems of uneven
 // size in one persistent launch (gfx950); in the forward direction, problems
 // large enough to fill the GPU on their own get single-problem launches
-// instead (the same kernels and results as vamx_par_forward).
+// instead (the same kernels and results as vamx_par_forward), spread over
+// three streams.
 //
 // Each problem is a parallel plan (vamx_par_create) with its own volume and
 // sinogram.  A problem's work is cut into tiles of one 256-thread block:
@@ -205,6 +206,16 @@ struct BatchScratch {
 
 constexpr size_t kMaxTables = 16;
 
+// side streams of the batched forward (default priority), created once per
+// host thread and device and kept for the life of the process
+struct SideStreams {
+  int device = -1;
+  bool ok = false;
+  hipStream_t st[2] = {nullptr, nullptr};
+  hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
+};
+thread_local SideStreams g_side;
+
 thread_local BatchScratch g_scratch;
 
 int run_batched(vamx_plan* const* plans, int n, const float* const* ins, float* const* outs, hipStream_t s,
@@ -224,8 +235,11 @@ int run_batched(vamx_plan* const* plans, int n, const float* const* ins, float* 
   // CU) runs as its own single-problem launch; the persistent kernel, whose
   // registers cover every z width and the per-lane fallback, reaches a lower
   // occupancy than the single-problem forward kernels.  The rest share the
-  // persistent launch.
-  std::vector<int> shared;
+  // persistent launch.  The single-problem launches go round-robin to the
+  // caller's stream and two side streams (forked from and joined back to the
+  // caller's stream with events, so the call stays graph-capturable): one
+  // launch's tail overlaps the next one's start.
+  std::vector<int> shared, single;
   for (int i = 0; i < n; ++i) {
     vamx_plan* p = plans[i];
     if (forward && !p->par.incl) {
@@ -234,12 +248,40 @@ int run_batched(vamx_plan* const* plans, int n, const float* const* ins, float* 
       const long long units =
           p->par.fp_pair && p->par.pair_ok ? fp_pair_tiled(p->par) : fp_tiled_rays(p->par);
       if (wz && units * wave_zchunks(p->par.nz, wz) >= 16ll * cus) {
-        int rc = par_forward_launch(p, ins[i], outs[i], s, !(p->flags & VAMX_PAR_NO_WARP_CLIP));
-        if (rc) return rc;
+        single.push_back(i);
         continue;
       }
     }
     shared.push_back(i);
+  }
+  if (!single.empty()) {
+    SideStreams& ss = g_side;
+    if (ss.device != dev) {
+      ss.ok = hipStreamCreateWithFlags(&ss.st[0], hipStreamNonBlocking) == hipSuccess &&
+              hipStreamCreateWithFlags(&ss.st[1], hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&ss.join[0], hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&ss.join[1], hipEventDisableTiming) == hipSuccess;
+      ss.device = dev;
+    }
+    const int ns = (ss.ok && single.size() > 1) ? 3 : 1;
+    if (ns > 1) {
+      VAMX_HIP(hipEventRecord(ss.fork, s));
+      for (int j = 0; j < 2; ++j) VAMX_HIP(hipStreamWaitEvent(ss.st[j], ss.fork, 0));
+    }
+    for (size_t k = 0; k < single.size(); ++k) {
+      const int i = single[k];
+      vamx_plan* p = plans[i];
+      hipStream_t st = (k % ns == 0) ? s : ss.st[k % ns - 1];
+      int rc = par_forward_launch(p, ins[i], outs[i], st, !(p->flags & VAMX_PAR_NO_WARP_CLIP));
+      if (rc) return rc;
+    }
+    if (ns > 1) {
+      for (int j = 0; j < 2; ++j) {
+        VAMX_HIP(hipEventRecord(ss.join[j], ss.st[j]));
+        VAMX_HIP(hipStreamWaitEvent(s, ss.join[j], 0));
+      }
+    }
   }
   if (shared.empty()) return VAMX_OK;
   const int m = (int)shared.size();
