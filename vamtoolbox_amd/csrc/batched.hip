@@ -208,11 +208,12 @@ constexpr size_t kMaxTables = 16;
 
 // side streams of the batched forward (default priority), created once per
 // host thread and device and kept for the life of the process
+constexpr int kSideStreams = 2;
 struct SideStreams {
   int device = -1;
   bool ok = false;
-  hipStream_t st[2] = {nullptr, nullptr};
-  hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
+  hipStream_t st[kSideStreams] = {};
+  hipEvent_t fork = nullptr, join[kSideStreams] = {};
 };
 thread_local SideStreams g_side;
 
@@ -257,17 +258,16 @@ int run_batched(vamx_plan* const* plans, int n, const float* const* ins, float* 
   if (!single.empty()) {
     SideStreams& ss = g_side;
     if (ss.device != dev) {
-      ss.ok = hipStreamCreateWithFlags(&ss.st[0], hipStreamNonBlocking) == hipSuccess &&
-              hipStreamCreateWithFlags(&ss.st[1], hipStreamNonBlocking) == hipSuccess &&
-              hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming) == hipSuccess &&
-              hipEventCreateWithFlags(&ss.join[0], hipEventDisableTiming) == hipSuccess &&
-              hipEventCreateWithFlags(&ss.join[1], hipEventDisableTiming) == hipSuccess;
+      ss.ok = hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming) == hipSuccess;
+      for (int j = 0; j < kSideStreams && ss.ok; ++j)
+        ss.ok = hipStreamCreateWithFlags(&ss.st[j], hipStreamNonBlocking) == hipSuccess &&
+                hipEventCreateWithFlags(&ss.join[j], hipEventDisableTiming) == hipSuccess;
       ss.device = dev;
     }
-    const int ns = (ss.ok && single.size() > 1) ? 3 : 1;
+    const int ns = (ss.ok && single.size() > 1) ? 1 + kSideStreams : 1;
     if (ns > 1) {
       VAMX_HIP(hipEventRecord(ss.fork, s));
-      for (int j = 0; j < 2; ++j) VAMX_HIP(hipStreamWaitEvent(ss.st[j], ss.fork, 0));
+      for (int j = 0; j < kSideStreams; ++j) VAMX_HIP(hipStreamWaitEvent(ss.st[j], ss.fork, 0));
     }
     for (size_t k = 0; k < single.size(); ++k) {
       const int i = single[k];
@@ -277,7 +277,7 @@ int run_batched(vamx_plan* const* plans, int n, const float* const* ins, float* 
       if (rc) return rc;
     }
     if (ns > 1) {
-      for (int j = 0; j < 2; ++j) {
+      for (int j = 0; j < kSideStreams; ++j) {
         VAMX_HIP(hipEventRecord(ss.join[j], ss.st[j]));
         VAMX_HIP(hipStreamWaitEvent(s, ss.join[j], 0));
       }
