@@ -250,21 +250,6 @@ def run_bclp(args):
                                                                                    key=lambda kv: kv[1]))))
         log("tuning choice %s" % rep["choice"])
     tuning = P.plan.get_tuning()
-    if args.kernels_only:
-        plan = P.plan
-        sino = torch.empty(plan.sino_shape, device=dev)
-        vol = torch.empty(plan.vol_shape, device=dev)
-        for _ in range(5):
-            plan.forward(T, out=sino)
-        for _ in range(5):
-            plan.backward(g0, out=vol)
-        a = torch.empty(1 << 28, device=dev)
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)  # 1 GiB read + 1 GiB write: counter calibration
-        torch.cuda.synchronize()
-        log("kernels-only run done")
-        return
 
     def options(**kw):
         return vam.optimize.Options(method="BCLP", n_iter=kw.pop("n_iter"), response_model=rm, eps=0.05, p=2, q=1,
@@ -289,6 +274,26 @@ def run_bclp(args):
         _lib.check(lib.vamx_bclp_step(plan.handle, ctypes.byref(resp), ctypes.byref(prm), g.data_ptr(),
                                       bc.T.data_ptr(), 0, 0, dose.data_ptr(), loss_log.data_ptr(),
                                       counter.data_ptr(), ws.data_ptr(), _lib.stream_ptr(dev)), "vamx_bclp_step")
+
+    if args.kernels_only:
+        # each projection 5 times, the fused iteration 5 times (its two
+        # projection kernels), then a 1 GiB device copy 3 times as the
+        # reference for the traffic counters
+        sino = torch.empty(P.plan.sino_shape, device=dev)
+        vol = torch.empty(P.plan.vol_shape, device=dev)
+        for _ in range(5):
+            P.plan.forward(T, out=sino)
+        for _ in range(5):
+            P.plan.backward(g0, out=vol)
+        for _ in range(5):
+            step()
+        a = torch.empty(1 << 28, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)  # 1 GiB read + 1 GiB write: counter calibration
+        torch.cuda.synchronize()
+        log("kernels-only run done")
+        return
 
     step()
     graph = torch.cuda.CUDAGraph()
@@ -660,6 +665,20 @@ def run_raytrace(args):
     P.output_torch_tensor = True
     f = T.reshape(-1).contiguous()
     g = P.forward(f)
+    if args.kernels_only:
+        # two more integrate and two deposit traces, then a 1 GiB device copy
+        # 3 times as the reference for the traffic counters
+        for _ in range(2):
+            P.forward(f)
+        for _ in range(2):
+            P.backward(g)
+        a = torch.empty(1 << 28, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("kernels-only run done")
+        return
     fwd_ms = _time_events(lambda: P.forward(f), 3)
     steps_f = P.stats()
     bwd_ms = _time_events(lambda: P.backward(g), 3)

@@ -15,18 +15,72 @@ invariants) is listed in ``PINNING`` below; a function listed there as
 """
 
 PINNING = {
-    "d1.clip_to_circle": "exact restatement; invariant tests (mask symmetry, count)",
-    "d1.forward": "scikit-image 0.18.3 warp golden vectors; brute-force matrix; "
-                  "90-degree rotation and integer-offset invariants; mass conservation",
-    "d1.backward": "numpy.interp golden vectors; brute-force matrix; constant-sinogram closed form",
-    "d1.forward_adjoint": "dot-product test against d1.forward; brute-force matrix transpose",
-    "d1.backward_adjoint": "dot-product test against d1.backward; brute-force matrix transpose",
-    "d1 absorption": "uniform-cylinder closed form (astra3Dabsorption.getExpabs); mu=0 reduction",
-    "d1 inclination": "phi=0 reduction; z-linear invariance",
-    "filters.filter_sinogram": "closed form of the Ram-Lak kernel DFT; scipy.fft reference",
-    "filters.filter_target_osmo": "direct 2-D DFT on tiny arrays",
-    "response": "closed form, analytic derivative vs finite differences, map_inv(map(f)) = f",
-    "optimizers": "tiny-problem iterates recomputed with the explicit projector matrix",
-    "raytrace": "straight-ray limit equals line integral; W/W^T dot-product test; "
-                "step-halving order test for RK4",
+    # --- D1 projector (d1.py; Projector3DParallel.py:57-216, util/data.py:14-40) ---
+    "d1.outside_circle_mask / clip_to_circle / inside_circle_mask":
+        "tests/test_oracle_d1.py::test_clip_to_circle_counts (symmetry, complement count, points on the "
+        "circle kept at odd n)",
+    "d1.warp_matrix / sample_coords / bilinear_zero_pad / clip_warp_output / forward":
+        "scikit-image 0.18.3 warp golden vectors (test_golden_skimage_and_interp, test_warp_clip_matters); "
+        "scalar brute-force loop (test_forward_brute_force_scalar_loop); 90-degree rotation and integer-offset "
+        "invariants; mass conservation",
+    "d1._interp_taps / backward": "numpy.interp golden vectors (test_golden_skimage_and_interp); "
+                                  "constant-sinogram closed form (test_constant_sinogram_backprojection)",
+    "d1.forward_adjoint / backward_adjoint": "brute-force matrix transposes (test_adjoints_are_transposes)",
+    "d1.absorption_weight / default_radius": "uniform-cylinder closed form and entry-side orientation "
+                                             "(test_absorption_closed_form_uniform_cylinder); mu = 0 reduction",
+    "d1._z_taps (inclination)": "phi = 0 reduction (test_mu_zero_and_phi_zero_reduce_exactly); brute-force "
+                                "transposes with inclination (test_adjoints_are_transposes)",
+    "d1.occlusion_sinogram / forward(occ=) / backward(occ=)":
+        "scikit-image / numpy.interp golden vectors in the reference's loop structure "
+        "(test_occlusion_sinogram_matches_golden, test_occluded_forward_backward_match_golden); first-hit brute "
+        "force; neutral without an occluder",
+    # --- filters and response (optim.py; util/data.py:43-113, 302-438; response.py:12-291) ---
+    "optim.fourier_filter": "ideal-ramp limit and symmetry (test_ramlak_filter_closed_form)",
+    "optim.filter_sinogram": "direct zero-padded linear convolution with the spatial Ram-Lak kernel "
+                             "(test_filter_sinogram_matches_direct_convolution)",
+    "optim.filter_target_osmo": "closed form on single real Fourier modes (test_filter_target_osmo_single_mode)",
+    "optim.Response": "closed form, derivative vs central differences for nu in {1, 0.5, 2.5}, "
+                      "map_inv(map(f)) = f (test_response_closed_forms)",
+    # --- optimisers (optim.py; OSMO.py, CAL.py, BCLP.py, FBP.py, PM.py, metrics.py) ---
+    "optim.osmo": "scale invariance, fixed point, void-then-gel step order in (d_l, d_h), iteration count "
+                  "(tests/test_oracle_pin_loops.py)",
+    "optim.cal": "lr = 0 keeps the initial iterate, scale invariance, update direction and iteration count, "
+                 "first momentum step (tests/test_oracle_pin_loops.py)",
+    "optim.cal_threshold": "separable classes and brute-force counting (test_cal_threshold_separated_classes)",
+    "optim.bclp_state / bclp": "identity-operator closed forms for p = q = 2 and p = 2, q = 1, band and bounds "
+                               "(tests/test_oracle_pin_loops.py); gradient vs central differences "
+                               "(test_bclp_gradient_finite_differences_oracle)",
+    "optim.bclp_init": "filtered back projection recovers map_inv(T) (check_bclp_init_fbp_scale)",
+    "optim.ver / circle_inds": "known answer and circle geometry (check_ver_known_answer, check_circle_inds)",
+    "optim.fbp": "disk reconstruction and the offset variant (check_fbp_reconstructs)",
+    "optim.pm_loss_grad / pm": "hand-computed known answer (check_pm_known_answer); gradient vs central "
+                               "differences (test_pm_gradient_finite_differences)",
+    # --- ray tracer (raytrace.py; pyTorchRayTrace.py, medium.py) ---
+    "raytrace.coord_vec": "spacing 1 / rate, centred, 2-D (check_coord_vec_and_attenuation_models)",
+    "raytrace.GridMedium.sample / _index": "torch.nn.functional.grid_sample called as the reference calls it, "
+                                          "border and zeros padding, 3-D and 2-D (check_grid_sample_parity)",
+    "raytrace.central_difference": "exact on quadratic fields, zero faces, 2-D (check_central_difference)",
+    "raytrace.luneburg_n": "analytic profile and radial gradient (check_luneburg_profile); focus on the "
+                           "antipodal surface point (test_luneburg_lens_focus)",
+    "raytrace.att_alpha": "cylinder / ball point tests; grid = grid_sample with zeros padding "
+                          "(check_coord_vec_and_attenuation_models, check_grid_sample_parity)",
+    "raytrace.setup_rays": "orthonormal ray frame, distance D, lateral coordinates, energies "
+                           "(check_setup_rays_frame)",
+    "raytrace.Tracer._rhs / _dv / _dx / advance": "canonical, physical and optical-path parametrisations trace "
+                                                  "the same geometric ray with RK4 and symplectic Euler "
+                                                  "(check_parametrisations_same_geometry); RK4 and Euler orders "
+                                                  "(test_rk4_fourth_order_and_euler_first_order)",
+    "raytrace.Tracer.run / _taps / _alpha_att": "straight-ray closed form of integrate and deposit with "
+                                                "cylinder attenuation at the step starts, h / dx weights and "
+                                                "trilinear hats (check_straight_ray_attenuation_closed_form); "
+                                                "W / W^T transposition (test_deposit_is_transpose_of_integrate)",
+    "raytrace.Tracer._exited / exit schedule": "closed-form exit step counts, every 10 and every 3 steps, and "
+                                               "the step cap (check_exit_step_count)",
+    # --- fan-beam rebinning (rebin.py; geometry.py:736-944) ---
+    "rebin.sampling / rebin_fan_beam": "telecentric limit, Bouguer's invariant, Fresnel factor at normal "
+                                       "incidence (test_fan_beam_rebin_telecentric_limit, "
+                                       "test_fan_beam_snell_and_centre_ray)",
 }
+"""Every entry's check fails under a planted one-line mistake in the oracle:
+``tests/test_oracle_mutations.py`` plants 54 such mutations and asserts that
+the named check catches each one."""

@@ -64,9 +64,8 @@ class GridMedium:
         else:
             self.voxel = np.array([self.xv[1] - self.xv[0], self.yv[1] - self.yv[0], self.zv[1] - self.zv[0]])
         self.span = self.vmax - self.vmin
-        self.norm = 2.0 / self.span
-        if np.isinf(self.norm[2]):
-            self.norm[2] = 0.0
+        # 2 / span, and 0 for the zero z span of a 2-D grid (medium.py:69-76)
+        self.norm = np.divide(2.0, self.span, out=np.zeros(3), where=self.span > 0)
         self.grad = central_difference(self.n, self.voxel) if grad is None else np.asarray(grad, dtype=np.float64)
         self.field = np.concatenate([self.n[..., None], self.grad], axis=3)  # (nx, ny, nz, 4)
 

@@ -147,6 +147,18 @@ def test_absorption_closed_form_uniform_cylinder():
         L = np.sqrt(R ** 2 - t ** 2)
         closed = (1 - np.exp(-2 * mu * L)) / mu
         assert abs(s[n // 2 + t] - closed) / closed < 0.03
+    # orientation: the light enters at s = +sqrt(R^2 - t^2) and runs toward -s
+    # (at 0 degrees output row r samples input row r, s = r - n // 2), so a
+    # voxel at s = +20 is dimmed by exp(-mu (L - 20)), one at s = -20 by
+    # exp(-mu (L + 20)); getExpabs' convention, astra3Dabsorption.py:130-139
+    c = n // 2
+    one = np.zeros((n, n, 1))
+    one[c + 20, c, 0] = 1.0
+    two = np.zeros((n, n, 1))
+    two[c - 20, c, 0] = 1.0
+    a = d1.forward(one, [0.0], mu=mu, radius=R, clip_output=False)[c, 0, 0]
+    b = d1.forward(two, [0.0], mu=mu, radius=R, clip_output=False)[c, 0, 0]
+    assert abs(a - np.exp(-mu * (R - 20))) < 1e-12 and abs(b - np.exp(-mu * (R + 20))) < 1e-12
 
 
 def test_constant_sinogram_backprojection():
@@ -182,6 +194,10 @@ def test_clip_to_circle_counts():
     # symmetric and strictly inside-out
     assert np.array_equal(m, m.T) and np.array_equal(m, m[::-1]) and m[0, 0] and not m[128, 128]
     assert m.sum() == 256 * 256 - d1.inside_circle_mask(256, 256).sum()
+    # points exactly on the circle are kept (util/data.py:29-33 zeroes r^2 > 1):
+    # at odd n the four axis end points lie on it
+    m17 = d1.outside_circle_mask(17, 17)
+    assert not m17[0, 8] and not m17[8, 0] and not m17[16, 8] and not m17[8, 16] and m17[0, 0]
 
 
 # --------------------------------------------------------------------------

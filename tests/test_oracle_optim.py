@@ -40,6 +40,26 @@ def test_filter_sinogram_matches_direct_convolution():
     np.testing.assert_allclose(out, direct, atol=1e-12)
 
 
+def test_filter_target_osmo_single_mode():
+    """util/data.py:43-113: the OSMO target filter multiplies the fftshift-ed
+    2-D spectrum by the radial ramp sqrt(fx^2 + fy^2) on linspace(-1, 1)
+    grids.  A single real Fourier mode cos(2 pi (ky i / ny + kx j / nx)) has
+    its two spectral peaks at shifted indices (ny//2 +- ky, nx//2 +- kx), so
+    the real part of the output is the same mode scaled by the mean of the
+    ramp at those two indices (linspace(-1, 1, n) is not symmetric about
+    n // 2 for even n, so the two values differ)."""
+    ny, nx = 12, 10
+    ly = -1 + 2 * np.arange(ny) / (ny - 1)
+    lx = -1 + 2 * np.arange(nx) / (nx - 1)
+    i, j = np.mgrid[:ny, :nx]
+    for ky, kx in ((1, 2), (3, 0), (0, 4), (5, 3), (2, 1)):
+        th = 2 * np.pi * (ky * i / ny + kx * j / nx)
+        out = oo.filter_target_osmo(np.cos(th)[:, :, None], "ram-lak")
+        fp = np.hypot(ly[ny // 2 + ky], lx[nx // 2 + kx])
+        fm = np.hypot(ly[ny // 2 - ky], lx[nx // 2 - kx])
+        np.testing.assert_allclose(out[:, :, 0], 0.5 * (fp + fm) * np.cos(th), atol=1e-12)
+
+
 def test_product_filters_match_oracle_on_host():
     from vamtoolbox_amd.util import data
     rng = np.random.default_rng(1)
@@ -63,6 +83,11 @@ def test_response_closed_forms():
     m = ra.map(f[(f > 0.05) & (f < 1.0)])
     np.testing.assert_allclose(ra.map_inv(m), f[(f > 0.05) & (f < 1.0)], atol=1e-9)
     np.testing.assert_allclose(o.map(f), ra.map(f), atol=1e-15)
+    # the oracle's derivative against central differences of its own map, for nu != 1 too
+    for nu in (1.0, 0.5, 2.5):
+        og = oo.Response("analytical", B=20, M=0.6, nu=nu, A=0.1, K=0.9)
+        fdo = (og.map(f + h) - og.map(f - h)) / (2 * h)
+        np.testing.assert_allclose(og.dmap(f), fdo, rtol=1e-6, atol=1e-9)
     ri = ResponseModel()  # reference default: 512-point table, B=25, M=0.5
     oi = oo.Response("interpolation")
     np.testing.assert_allclose(ri.map(f), oi.map(f), atol=0)
