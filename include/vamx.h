@@ -266,15 +266,41 @@ int vamx_sino_resample(const float* sino, int32_t n_r, int32_t n_a, int32_t nz, 
 /* ---------------------------------------------------------------------------
  * Batched uneven parallel-beam problems in one launch
  * ------------------------------------------------------------------------- */
-/* Forward-project n_plans independent problems (each a parallel plan created
- * with vamx_par_create) in a single persistent launch; vols[i]/sinos[i] are
- * device pointers of problem i.  Work is balanced by n*n_angles*n*nz.  In the
- * forward direction a problem whose wavefront work alone fills the GPU
- * (16 waves per CU) is projected by its own single-problem launch instead
- * (results as vamx_par_forward); those launches are spread over the caller's
- * stream and two library-owned side streams that fork from and join back to
- * it through events, so the call is ordered on `stream` as a whole and can be
- * captured in a HIP graph. */
+/* A batch of n_plans independent problems, each a parallel plan created with
+ * vamx_par_create; no plan may appear twice (VAMX_ERR_ARG).  The batch owns
+ * its device tables of work items, the tile counter of its persistent
+ * launches and two side streams, all released by vamx_batch_destroy (destroy
+ * the batch before its plans).  A batch is used from one stream at a time;
+ * different batches may run concurrently on different streams.  No
+ * reference counterpart: VAMToolbox solves one target per call
+ * (vamtoolbox/optimize.py:174-217). */
+typedef struct vamx_batch vamx_batch;
+int vamx_batch_create(vamx_plan* const* plans, int32_t n_plans, vamx_batch** out_batch);
+
+/* Forward-project every problem of the batch: vols[i] / sinos[i] are device
+ * pointers of problem i.  Problems share persistent launches (at most 64
+ * problems per launch) whose blocks drain a device-side queue of tiles,
+ * largest problem first; work is balanced by n*n_angles*n*nz.  A problem
+ * whose wavefront work alone fills the GPU (16 waves per CU) is projected by
+ * its own single-problem launch instead (results as vamx_par_forward); those
+ * launches are spread over the caller's stream and the batch's two side
+ * streams, which fork from and join back to it through events, so the call is
+ * ordered on `stream` as a whole.  The buffer pointers travel as kernel
+ * arguments; the first call with a new buffer alignment or work order
+ * uploads a table and synchronises the stream, after which calls can be
+ * captured in a HIP graph (the tables live as long as the batch).  Replaces
+ * Projector3DParallelPython.forward (Projector3DParallel.py:113-156) for every
+ * problem at once. */
+int vamx_batch_forward(vamx_batch* batch, const float* const* vols, float* const* sinos, void* stream);
+
+/* Back-project every problem of the batch (one persistent launch per 64
+ * problems; Projector3DParallelPython.backward, Projector3DParallel.py:158-207). */
+int vamx_batch_backward(vamx_batch* batch, const float* const* sinos, float* const* vols, void* stream);
+void vamx_batch_destroy(vamx_batch* batch);
+
+/* The same for a list of plans without an explicit batch: the library keeps
+ * one batch per distinct list of plans and releases it when one of those
+ * plans is destroyed. */
 int vamx_par_forward_batched(vamx_plan* const* plans, int32_t n_plans,
                              const float* const* vols, float* const* sinos, void* stream);
 int vamx_par_backward_batched(vamx_plan* const* plans, int32_t n_plans,

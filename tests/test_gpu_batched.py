@@ -124,3 +124,159 @@ def test_batched_osmo_graph_replays_equal_eager(gpu):
         assert torch.equal(eager.x, graph.x), it
         assert torch.equal(eager.b, graph.b), it
         assert torch.equal(eager.x_model, graph.x_model), it
+
+
+def _filling_set(gpu, seed):
+    """three problems that each fill the GPU alone (their forwards run as
+    single-problem launches on the caller's stream and two side streams) and
+    three small ones (persistent launch)"""
+    import torch
+    from vamtoolbox_amd.parallel import ParallelPlan
+    rng = np.random.default_rng(seed)
+    specs = [(128, 64, 360, 0.01), (128, 64, 360, 0.0), (120, 77, 360, 0.02), (32, 8, 30, 0.0), (24, 40, 45, 0.01),
+             (20, 3, 20, 0.0)]
+    plans, vols, sinos = [], [], []
+    for n, nz, na, mu in specs:
+        p = ParallelPlan(n, nz, np.linspace(0, 360, na, endpoint=False), mu=mu, device=gpu)
+        plans.append(p)
+        vols.append(torch.tensor(rng.random((n, n, nz)), dtype=torch.float32, device=gpu))
+        sinos.append(torch.tensor(rng.random((n, na, nz)), dtype=torch.float32, device=gpu))
+    return plans, vols, sinos
+
+
+def test_batched_side_streams_equal_individual(gpu):
+    """The event fork / join path (three GPU-filling forwards) equals the
+    per-problem forwards bitwise, through the plan-list entry points and
+    through an explicit batch."""
+    import torch
+    from vamtoolbox_amd import _lib
+    from vamtoolbox_amd.batch import ParallelBatch
+    plans, vols, sinos = _filling_set(gpu, 41)
+    lib = _lib.load()
+    k = len(plans)
+    ref_s = [p.forward(v) for p, v in zip(plans, vols)]
+    ref_b = [p.backward(s) for p, s in zip(plans, sinos)]
+    outs = [torch.full_like(r, float("nan")) for r in ref_s]
+    H = (ctypes.c_void_p * k)(*[p.handle.value for p in plans])
+    IN = (ctypes.c_void_p * k)(*[v.data_ptr() for v in vols])
+    OUT = (ctypes.c_void_p * k)(*[o.data_ptr() for o in outs])
+    _lib.check(lib.vamx_par_forward_batched(H, k, IN, OUT, _lib.stream_ptr(gpu)), "forward_batched")
+    torch.cuda.synchronize()
+    for i in range(k):
+        assert torch.equal(outs[i], ref_s[i]), i
+    B = ParallelBatch(plans)
+    s = B.forward(B.pack_vols(vols))
+    x = B.backward(torch.cat([t.reshape(-1) for t in sinos]))
+    torch.cuda.synchronize()
+    for i, (sv, xv) in enumerate(zip(B.sino_views(s), B.vol_views(x))):
+        assert torch.equal(sv, ref_s[i]), i
+        assert torch.equal(xv, ref_b[i]), i
+
+
+def test_batched_repeated_plan_rejected(gpu):
+    """A plan may appear once per batch: its slice min/max scratch serves one problem."""
+    import torch
+    from vamtoolbox_amd import _lib
+    from vamtoolbox_amd.batch import ParallelBatch
+    from vamtoolbox_amd.parallel import ParallelPlan
+    p = ParallelPlan(24, 8, np.linspace(0, 360, 30, endpoint=False), device=gpu)
+    with pytest.raises(ValueError):
+        ParallelBatch([p, p])
+    lib = _lib.load()
+    a = torch.rand(p.vol_shape, device=gpu)
+    b = torch.rand(p.vol_shape, device=gpu)
+    oa, ob = torch.empty(p.sino_shape, device=gpu), torch.empty(p.sino_shape, device=gpu)
+    H = (ctypes.c_void_p * 2)(p.handle.value, p.handle.value)
+    IN = (ctypes.c_void_p * 2)(a.data_ptr(), b.data_ptr())
+    OUT = (ctypes.c_void_p * 2)(oa.data_ptr(), ob.data_ptr())
+    assert lib.vamx_par_forward_batched(H, 2, IN, OUT, _lib.stream_ptr(gpu)) == -1
+    assert b"repeats" in lib.vamx_last_error()
+    h = ctypes.c_void_p()
+    assert lib.vamx_batch_create(H, 2, ctypes.byref(h)) == -1
+
+
+def test_batched_concurrent_batches_on_two_streams(gpu):
+    """Two batches on two streams at once: each has its own tile counter, so
+    neither drains the other's queue."""
+    import torch
+    from vamtoolbox_amd.batch import ParallelBatch
+    from vamtoolbox_amd.parallel import ParallelPlan
+    rng = np.random.default_rng(43)
+    sets = []
+    for base in (0, 1):
+        plans = [ParallelPlan(n, nz, np.linspace(0, 360, na, endpoint=False), device=gpu)
+                 for n, nz, na in [(48 + 8 * base, 32, 90), (32, 16, 60), (40, 8, 45)]]
+        vols = [torch.tensor(rng.random(p.vol_shape), dtype=torch.float32, device=gpu) for p in plans]
+        B = ParallelBatch(plans)
+        sets.append((plans, vols, B, B.pack_vols(vols), [p.forward(v) for p, v in zip(plans, vols)]))
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(device=gpu) for _ in sets]
+    outs = [B.empty_sino() for _, _, B, _, _ in sets]
+    for rep in range(3):
+        for st, (_, _, B, flat, _), o in zip(streams, sets, outs):
+            st.wait_stream(torch.cuda.current_stream(gpu))
+            with torch.cuda.stream(st):
+                B.forward(flat, out=o)
+        torch.cuda.synchronize()
+        for (_, _, B, _, ref), o in zip(sets, outs):
+            for i, v in enumerate(B.sino_views(o)):
+                assert torch.equal(v, ref[i]), (rep, i)
+
+
+def test_batched_graph_survives_other_batches(gpu):
+    """A graph that captured a batched forward keeps replaying correctly after
+    many other batches ran on the thread (each batch owns its tables)."""
+    import torch
+    from vamtoolbox_amd.batch import ParallelBatch
+    from vamtoolbox_amd.parallel import ParallelPlan
+    plans, vols, _ = _filling_set(gpu, 47)
+    B = ParallelBatch(plans)
+    flat = B.pack_vols(vols)
+    out = B.empty_sino()
+    B.forward(flat, out=out)           # uploads the tables eagerly
+    ref = out.clone()
+    s = torch.cuda.Stream(device=gpu)
+    s.wait_stream(torch.cuda.current_stream(gpu))
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            B.forward(flat, out=out)
+    torch.cuda.current_stream(gpu).wait_stream(s)
+    for rep in range(3):
+        # twenty other batches, each with its own tables
+        for j in range(20):
+            ps = [ParallelPlan(16 + j % 5, 4 + j % 3, np.linspace(0, 180, 10 + j, endpoint=False), device=gpu)]
+            Bo = ParallelBatch(ps)
+            Bo.forward(Bo.pack_vols([np.ones(ps[0].vol_shape)]))
+            Bo.close()
+        out.fill_(float("nan"))
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), rep
+
+
+def test_batched_osmo_graph_with_side_streams(gpu):
+    """OSMOBatch capture and replay over a set whose forwards use the side
+    streams equals the eager iteration bitwise."""
+    import torch
+    import vamtoolbox_amd as vam
+    from vamtoolbox_amd.batch import OSMOBatch
+    from tests.test_gpu_optim import _ellipsoids
+    specs = [(128, 64, 360, 0.01), (128, 64, 360, 0.0), (120, 77, 360, 0.02), (32, 8, 30, 0.0), (24, 40, 45, 0.01)]
+    tgs = [vam.geometry.TargetGeometry(target=_ellipsoids(n, nz, seed=50 + i)) for i, (n, nz, _, _) in enumerate(specs)]
+    pgs = [vam.geometry.ProjectionGeometry(np.linspace(0, 360, na, endpoint=False), "parallel", CUDA=True,
+                                           absorption_per_voxel=mu) for _, _, na, mu in specs]
+    opts = vam.optimize.Options(method="OSMO", n_iter=4, d_h=0.85, d_l=0.6, filter="ram-lak")
+    eager = OSMOBatch(tgs, pgs, opts, device=gpu)
+    graph = OSMOBatch(tgs, pgs, opts, device=gpu)
+    eager.step()
+    graph.step()
+    graph.capture()
+    for it in range(3):
+        eager.step()
+        torch.cuda.synchronize()
+        graph.graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(eager.x, graph.x), it
+        assert torch.equal(eager.b, graph.b), it

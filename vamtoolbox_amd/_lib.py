@@ -144,6 +144,10 @@ SIGNATURES = {
                                            ctypes.POINTER(c_void_p), c_void_p]),
     "vamx_par_backward_batched": (c_int32, [ctypes.POINTER(c_void_p), c_int32, ctypes.POINTER(c_void_p),
                                             ctypes.POINTER(c_void_p), c_void_p]),
+    "vamx_batch_create": (c_int32, [ctypes.POINTER(c_void_p), c_int32, ctypes.POINTER(c_void_p)]),
+    "vamx_batch_forward": (c_int32, [c_void_p, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), c_void_p]),
+    "vamx_batch_backward": (c_int32, [c_void_p, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), c_void_p]),
+    "vamx_batch_destroy": (None, [c_void_p]),
     "vamx_destroy": (None, [_PLAN_P]),
     "vamx_last_error": (ctypes.c_char_p, []),
     "vamx_version": (c_int32, []),

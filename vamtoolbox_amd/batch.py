@@ -5,9 +5,11 @@ VAMToolbox solves one target at a time (vamtoolbox/optimize.py:174-217).  A
 batch holds k problems -- each its own ``ParallelPlan`` with its own size,
 angle set and absorption -- with all volumes in one flat float32 buffer and
 all sinograms in another.  ``ParallelBatch.forward`` / ``backward`` project
-every problem with ``vamx_par_forward_batched`` / ``vamx_par_backward_batched``:
-one launch whose blocks take work tiles from a device-side queue, largest
-problem first, so small problems fill the gaps the large ones leave.
+every problem with ``vamx_batch_forward`` / ``vamx_batch_backward`` on a batch
+handle (``vamx_batch_create``) that owns the batch's device tables, tile
+counter and side streams: one launch whose blocks take work tiles from a
+device-side queue, largest problem first, so small problems fill the gaps the
+large ones leave.
 
 ``minimizeOSMOBatched`` runs the OSMO iteration of
 vamtoolbox/optimizer/OSMO.py:9-178 for every problem (the same steps as
@@ -44,6 +46,8 @@ class ParallelBatch:
         for p in self.plans:
             if p.device != self.device:
                 raise ValueError("all plans of a batch must live on one device")
+        if len({id(p) for p in self.plans}) != len(self.plans):
+            raise ValueError("a plan may appear only once in a batch (its slice min/max scratch serves one problem)")
         self.k = len(self.plans)
         self.lib = _lib.load()
         self.vol_size = [int(np.prod(p.vol_shape)) for p in self.plans]
@@ -54,6 +58,23 @@ class ParallelBatch:
         self.n_sino = int(sum(self.sino_size))
         self._handles = (ctypes.c_void_p * self.k)(*[p.handle.value for p in self.plans])
         self._ptrs = {}
+        # the batch's device tables, tile counter and side streams (released
+        # in close(), before the plans)
+        self.handle = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(self.lib.vamx_batch_create(self._handles, self.k, ctypes.byref(self.handle)),
+                       "vamx_batch_create")
+
+    def close(self):
+        h, self.handle = getattr(self, "handle", None), None
+        if h and h.value:
+            self.lib.vamx_batch_destroy(h)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     # -- buffers -----------------------------------------------------------
     def empty_vol(self) -> torch.Tensor:
@@ -98,9 +119,9 @@ class ParallelBatch:
         self._check(vol_flat, self.n_vol, "vol_flat")
         out = self.empty_sino() if out is None else out
         self._check(out, self.n_sino, "out")
-        _lib.check(self.lib.vamx_par_forward_batched(self._handles, self.k, self._pointer_table(vol_flat, self.vol_off),
-                                                     self._pointer_table(out, self.sino_off),
-                                                     _lib.stream_ptr(self.device)), "vamx_par_forward_batched")
+        _lib.check(self.lib.vamx_batch_forward(self.handle, self._pointer_table(vol_flat, self.vol_off),
+                                               self._pointer_table(out, self.sino_off), _lib.stream_ptr(self.device)),
+                   "vamx_batch_forward")
         return out
 
     def backward(self, sino_flat: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
@@ -108,10 +129,9 @@ class ParallelBatch:
         self._check(sino_flat, self.n_sino, "sino_flat")
         out = self.empty_vol() if out is None else out
         self._check(out, self.n_vol, "out")
-        _lib.check(self.lib.vamx_par_backward_batched(self._handles, self.k,
-                                                      self._pointer_table(sino_flat, self.sino_off),
-                                                      self._pointer_table(out, self.vol_off),
-                                                      _lib.stream_ptr(self.device)), "vamx_par_backward_batched")
+        _lib.check(self.lib.vamx_batch_backward(self.handle, self._pointer_table(sino_flat, self.sino_off),
+                                                self._pointer_table(out, self.vol_off), _lib.stream_ptr(self.device)),
+                   "vamx_batch_backward")
         return out
 
 

@@ -1,4 +1,6 @@
 // Error reporting, plan lifetime and small helpers of libvamx_hip.so.
+#include <atomic>
+
 #include "vamx_internal.h"
 
 namespace vamx {
@@ -17,6 +19,11 @@ int hip_check(hipError_t e, const char* what) {
 
 bool check_plan(vamx_plan* p, int kind) { return p != nullptr && p->kind == kind; }
 
+unsigned long long next_plan_serial() {
+  static std::atomic<unsigned long long> next{1};
+  return next.fetch_add(1);
+}
+
 unsigned blocks_for(long long total, int block) {
   long long b = (total + block - 1) / block;
   return (unsigned)(b < 1 ? 1 : b);
@@ -26,6 +33,7 @@ unsigned blocks_for(long long total, int block) {
 
 extern "C" void vamx_destroy(vamx_plan* p) {
   if (!p) return;
+  if (p->kind == vamx::PLAN_PARALLEL) vamx::batch_forget_plan(p);
   int cur = 0;
   (void)hipGetDevice(&cur);
   if (cur != p->device) (void)hipSetDevice(p->device);

@@ -69,11 +69,16 @@ struct RtDev {
   double radial_offset;      // proj_plane_radial_offset
 };
 
+// serial number of a new plan (unique for the process: a batch names its
+// plans by serial, so a new plan at a freed plan's address is another plan)
+unsigned long long next_plan_serial();
+
 }  // namespace vamx
 
 struct vamx_plan {
   int kind = 0;
   int device = 0;
+  unsigned long long serial = vamx::next_plan_serial();
   // parallel
   vamx::ParDev par{};
   int flags = 0;
@@ -101,6 +106,8 @@ int set_error(int code, const std::string& msg);
 int hip_check(hipError_t e, const char* what);
 bool check_plan(vamx_plan* p, int kind);
 unsigned blocks_for(long long total, int block);
+// releases the batches of vamx_par_{forward,backward}_batched that hold p
+void batch_forget_plan(const vamx_plan* p);
 }  // namespace vamx
 
 #define VAMX_HIP(call)                                         \
