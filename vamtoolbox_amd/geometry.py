@@ -120,10 +120,20 @@ class Volume:
             self.resolution = self.nZ
 
     def save(self, name: str):
-        """Save the array (and its type) as ``name + file_extension + '.npz'``."""
+        """Save the array, its type and the projection geometry's plain fields
+        (angles, ray type, numbers and strings; model objects are not saved)
+        as ``name + file_extension + '.npz'`` (the reference pickles the whole
+        object with dill, geometry.py:348-363)."""
         if self.file_extension:
             name = name + self.file_extension
-        np.savez(name + ".npz", array=np.asarray(self.array), vol_type=str(self.vol_type))
+        extra = {}
+        if self.proj_geo is not None:
+            import json
+            pg = self.proj_geo
+            fields = {k: v for k, v in vars(pg).items()
+                      if k not in ("angles", "n_angles") and (v is None or isinstance(v, (bool, int, float, str)))}
+            extra = {"proj_angles": np.asarray(pg.angles, dtype=np.float64), "proj_fields": json.dumps(fields)}
+        np.savez(name + ".npz", array=np.asarray(self.array), vol_type=str(self.vol_type), **extra)
 
     def constructCoordVec(self, spatial_sampling_rate=None, device=None):
         """Coordinate vectors centred on the object, geometry.py:365-430."""
@@ -201,11 +211,23 @@ class Reconstruction(Volume):
 
 
 def loadVolume(file_name: str):
-    """Load a Volume saved with ``Volume.save`` (npz; no pickles)."""
+    """Load a Volume saved with ``Volume.save`` (npz; no pickles): a target as
+    ``TargetGeometry`` (its saved array, not clipped again), a sinogram or
+    reconstruction with its projection geometry's saved fields
+    (geometry.py:662-679 loads the dill pickle of the object)."""
     z = np.load(file_name, allow_pickle=False)
     vt = str(z["vol_type"])
-    cls = {"target": Reconstruction, "recon": Reconstruction, "sino": Sinogram}.get(vt, Reconstruction)
-    return cls(z["array"], None)
+    if vt == "target":
+        return TargetGeometry(target=z["array"], clip_to_circle=False)
+    proj_geo = None
+    if "proj_angles" in z.files:
+        import json
+        fields = json.loads(str(z["proj_fields"]))
+        proj_geo = ProjectionGeometry(z["proj_angles"], fields.pop("ray_type", "parallel"), CUDA=fields.pop("CUDA", False))
+        for k, v in fields.items():
+            setattr(proj_geo, k, v)
+    cls = Sinogram if vt == "sino" else Reconstruction
+    return cls(z["array"], proj_geo)
 
 
 def getCircleMask(target):

@@ -80,8 +80,7 @@ class Projector3DParallelHIP:
         if zd is None:
             return None
         if self._zd_t is None:
-            self._zd_t = torch.as_tensor(np.ascontiguousarray(np.broadcast_to(zd, self.sino_shape)),
-                                         device=self.device)
+            self._zd_t = torch.as_tensor(np.array(np.broadcast_to(zd, self.sino_shape)), device=self.device)
         return self._zd_t
 
     # -- operators --------------------------------------------------------
