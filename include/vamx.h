@@ -164,6 +164,24 @@ int vamx_bclp_eval(vamx_plan* plan, const vamx_response* resp, const vamx_bclp_p
                    const float* weight_arr, float* dose, float* operand, double* loss_out,
                    void* workspace, void* stream);
 
+/* vamx_bclp_step without a workspace argument and without any allocation:
+ * no counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same kernels and results as vamx_bclp_step; the BCLP
+ * operand, an intermediate inside vamx_bclp_step's workspace, is written to
+ * the caller's device buffer `operand` [V], and the loss partials and the
+ * gradient scale use a 1 MiB device array of the library (one per GPU,
+ * held until the process ends): at most 131040 partial slots, i.e. one per
+ * 256-thread block of the back-projection epilogue (n*n*nz / 256 on the
+ * per-lane path, e.g. 384 x 384 x 227 at most); larger problems fail with
+ * VAMX_ERR_ARG before anything is launched.  Calls that overlap on one GPU
+ * race on that array: use one stream at a time per GPU.  The plan's device
+ * must be the current device. */
+int vamx_bclp_step_noalloc(vamx_plan* plan, const vamx_response* resp, const vamx_bclp_params* prm,
+                           float* g, const float* target, const float* eps_arr, const float* weight_arr,
+                           float* dose, float* operand, double* loss_log, int32_t* iter_counter,
+                           void* stream);
+
 /* ---------------------------------------------------------------------------
  * Curved-ray tracer (vamtoolbox/projector/pyTorchRayTrace.py)
  * ------------------------------------------------------------------------- */

@@ -130,6 +130,9 @@ SIGNATURES = {
     "vamx_bclp_eval": (c_int32, [_PLAN_P, ctypes.POINTER(Response), ctypes.POINTER(BclpParams),
                                  c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_void_p, c_void_p]),
+    "vamx_bclp_step_noalloc": (c_int32, [_PLAN_P, ctypes.POINTER(Response), ctypes.POINTER(BclpParams),
+                                         c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                         c_void_p, c_void_p, c_void_p]),
     "vamx_rt_create": (c_int32, [ctypes.POINTER(RtGeom), ctypes.POINTER(c_void_p)]),
     "vamx_rt_integrate": (c_int32, [_PLAN_P, c_void_p, c_void_p, c_void_p]),
     "vamx_rt_deposit": (c_int32, [_PLAN_P, c_void_p, c_void_p, c_void_p]),

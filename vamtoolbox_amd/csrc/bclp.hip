@@ -12,6 +12,8 @@
 //   C  forward projection of the operand (or the exact transpose of the back
 //      projection) fused with g <- clip(g - lr * scale * grad, glb, gub).
 // The per-voxel epilogue runs in float64, as the reference does in NumPy.
+#include <mutex>
+
 #include "par_core.h"
 
 namespace vamx {
@@ -374,8 +376,10 @@ int fill_devs(const vamx_response* resp, const vamx_bclp_params* prm, const floa
   return VAMX_OK;
 }
 
+// Launches kernel A; its per-block loss partials go to partials[0, nparts).
+// Fails before launching when nparts would exceed max_parts.
 int launch_epilogue(vamx_plan* p, const RespDev& r, const BclpDev& b, const float* g, const float* target,
-                    float* dose, float* operand, double* partials, int& nparts, hipStream_t s) {
+                    float* dose, float* operand, double* partials, int max_parts, int& nparts, hipStream_t s) {
   const ParDev& gd = p->par;
   const int clipc = (p->flags & VAMX_PAR_NO_CIRCLE_CLIP) ? 0 : 1;
   const bool v4 = !gd.incl && gd.nz % 4 == 0 && ((uintptr_t)g | (uintptr_t)target | (uintptr_t)dose |
@@ -390,6 +394,9 @@ int launch_epilogue(vamx_plan* p, const RespDev& r, const BclpDev& b, const floa
   if (wz) {
     const long long threads = bp_tiled_quads(gd) * wave_zchunks(gd.nz, wz) * 64;
     nparts = (int)blocks_for(threads, 256);
+    if (nparts > max_parts)
+      return set_error(VAMX_ERR_ARG, "bclp: " + std::to_string(nparts) + " loss partials exceed the " +
+                                          std::to_string(max_parts) + " slots of the scratch");
     const dim3 grid(nparts), block(256);
     if (wz == 4)
       hipLaunchKernelGGL(bclp_bp_epilogue_wave<4>, grid, block, 0, s, gd, clipc, r, b, g, target, dose, operand,
@@ -405,6 +412,9 @@ int launch_epilogue(vamx_plan* p, const RespDev& r, const BclpDev& b, const floa
   }
   const long long total = (long long)gd.n * gd.n * (gd.nz / (v4 ? 4 : 1));
   nparts = (int)blocks_for(total, 256);
+  if (nparts > max_parts)
+    return set_error(VAMX_ERR_ARG, "bclp: " + std::to_string(nparts) + " loss partials exceed the " +
+                                        std::to_string(max_parts) + " slots of the scratch");
   const dim3 grid(nparts), block(256);
   if (gd.incl)
     hipLaunchKernelGGL((bclp_bp_epilogue<1, true>), grid, block, 0, s, gd, clipc, r, b, g, target, dose, operand,
@@ -428,24 +438,14 @@ extern "C" int vamx_bclp_workspace_bytes(vamx_plan* plan, int64_t* bytes) {
   return VAMX_OK;
 }
 
-extern "C" int vamx_bclp_step(vamx_plan* plan, const vamx_response* resp, const vamx_bclp_params* prm, float* g,
-                              const float* target, const float* eps_arr, const float* weight_arr, float* dose,
-                              double* loss_log, int32_t* iter_counter, void* workspace, void* stream) {
-  if (!check_plan(plan, PLAN_PARALLEL)) return set_error(VAMX_ERR_STATE, "vamx_bclp_step: not a parallel plan");
-  if (!g || !target || !dose || !loss_log || !iter_counter || !workspace)
-    return set_error(VAMX_ERR_ARG, "vamx_bclp_step: null buffer");
-  RespDev r;
-  BclpDev b;
-  int rc = fill_devs(resp, prm, eps_arr, weight_arr, r, b);
-  if (rc) return rc;
-  hipStream_t s = (hipStream_t)stream;
-  const WsLayout w = ws_layout(plan);
-  char* ws = (char*)workspace;
-  double* partials = (double*)(ws + w.partials_off);
-  double* scale = (double*)(ws + w.scale_off);
-  float* operand = (float*)(ws + w.operand_off);
+namespace {
+
+// Kernels A, B, C of one iteration; partials has max_parts slots, scale one.
+int bclp_step_impl(vamx_plan* plan, const RespDev& r, const BclpDev& b, const vamx_bclp_params* prm, float* g,
+                   const float* target, float* dose, float* operand, double* partials, int max_parts,
+                   double* scale, double* loss_log, int32_t* iter_counter, hipStream_t s) {
   int nparts = 0;
-  rc = launch_epilogue(plan, r, b, g, target, dose, operand, partials, nparts, s);
+  int rc = launch_epilogue(plan, r, b, g, target, dose, operand, partials, max_parts, nparts, s);
   if (rc) return rc;
   hipLaunchKernelGGL(bclp_reduce, dim3(1), dim3(256), 0, s, partials, nparts, b.dvol, b.p, b.q, (double*)nullptr,
                      iter_counter, loss_log, scale);
@@ -533,6 +533,29 @@ extern "C" int vamx_bclp_step(vamx_plan* plan, const vamx_response* resp, const 
   return VAMX_OK;
 }
 
+
+}  // namespace
+
+extern "C" int vamx_bclp_step(vamx_plan* plan, const vamx_response* resp, const vamx_bclp_params* prm, float* g,
+                              const float* target, const float* eps_arr, const float* weight_arr, float* dose,
+                              double* loss_log, int32_t* iter_counter, void* workspace, void* stream) {
+  if (!check_plan(plan, PLAN_PARALLEL)) return set_error(VAMX_ERR_STATE, "vamx_bclp_step: not a parallel plan");
+  if (!g || !target || !dose || !loss_log || !iter_counter || !workspace)
+    return set_error(VAMX_ERR_ARG, "vamx_bclp_step: null buffer");
+  RespDev r;
+  BclpDev b;
+  int rc = fill_devs(resp, prm, eps_arr, weight_arr, r, b);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const WsLayout w = ws_layout(plan);
+  char* ws = (char*)workspace;
+  double* partials = (double*)(ws + w.partials_off);
+  double* scale = (double*)(ws + w.scale_off);
+  float* operand = (float*)(ws + w.operand_off);
+  return bclp_step_impl(plan, r, b, prm, g, target, dose, operand, partials, w.nparts, scale, loss_log,
+                        iter_counter, s);
+}
+
 extern "C" int vamx_bclp_eval(vamx_plan* plan, const vamx_response* resp, const vamx_bclp_params* prm,
                               const float* g, const float* target, const float* eps_arr, const float* weight_arr,
                               float* dose, float* operand, double* loss_out, void* workspace, void* stream) {
@@ -550,10 +573,70 @@ extern "C" int vamx_bclp_eval(vamx_plan* plan, const vamx_response* resp, const 
   double* scale = (double*)(ws + w.scale_off);
   float* op = operand ? operand : (float*)(ws + w.operand_off);
   int nparts = 0;
-  rc = launch_epilogue(plan, r, b, g, target, dose, op, partials, nparts, s);
+  rc = launch_epilogue(plan, r, b, g, target, dose, op, partials, w.nparts, nparts, s);
   if (rc) return rc;
   hipLaunchKernelGGL(bclp_reduce, dim3(1), dim3(256), 0, s, partials, nparts, b.dvol, b.p, b.q, loss_out,
                      (int32_t*)nullptr, (double*)nullptr, scale);
   VAMX_HIP(hipGetLastError());
   return VAMX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free variant of vamx_bclp_step.  The loss partials and the
+// gradient scale live in `scratch`, a file-scope device array that every GPU
+// the library runs on holds until the process ends (1 MiB per GPU).  Host
+// code never uses the array's name as an address: its address on each device
+// comes from hipGetSymbolAddress, called once per device with that device
+// current, and only that pointer is passed to kernels.  Every call on one GPU
+// shares the array, so calls that overlap on one GPU race (one stream at a
+// time per GPU).  Each call writes every partial slot it reads and the scale
+// slot before reading them, so nothing has to be cleared between calls.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kScratchDoubles = 131072;              // 1 MiB
+constexpr int kScratchParts = kScratchDoubles - 32;  // loss partial slots; the scale sits after them
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// device address of `scratch` on the current device (looked up once per device)
+int scratch_on_device(int dev, double** out) {
+  if (dev < 0 || dev >= kMaxScratchDevices) return set_error(VAMX_ERR_ARG, "bclp noalloc: device id out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_check(e, "hipGetSymbolAddress(scratch)");
+    if (!ptr) return set_error(VAMX_ERR_HIP, "bclp noalloc: scratch has no device address");
+    g_scratch_ptr[dev] = (double*)ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return VAMX_OK;
+}
+
+}  // namespace
+
+extern "C" int vamx_bclp_step_noalloc(vamx_plan* plan, const vamx_response* resp, const vamx_bclp_params* prm,
+                                      float* g, const float* target, const float* eps_arr, const float* weight_arr,
+                                      float* dose, float* operand, double* loss_log, int32_t* iter_counter,
+                                      void* stream) {
+  if (!check_plan(plan, PLAN_PARALLEL)) return set_error(VAMX_ERR_STATE, "vamx_bclp_step_noalloc: not a parallel plan");
+  if (!g || !target || !dose || !operand || !loss_log || !iter_counter)
+    return set_error(VAMX_ERR_ARG, "vamx_bclp_step_noalloc: null buffer");
+  RespDev r;
+  BclpDev b;
+  int rc = fill_devs(resp, prm, eps_arr, weight_arr, r, b);
+  if (rc) return rc;
+  int dev = -1;
+  VAMX_HIP(hipGetDevice(&dev));
+  if (dev != plan->device)
+    return set_error(VAMX_ERR_ARG, "vamx_bclp_step_noalloc: the plan's device must be the current device");
+  double* sp = nullptr;
+  rc = scratch_on_device(dev, &sp);
+  if (rc) return rc;
+  return bclp_step_impl(plan, r, b, prm, g, target, dose, operand, sp, kScratchParts, sp + kScratchParts, loss_log,
+                        iter_counter, (hipStream_t)stream);
 }
