@@ -559,6 +559,152 @@ __device__ __forceinline__ float bpT_bin_incl(const ParDev& g, const float* __re
 // the fused multiply-adds.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int rl_i(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+
+// ---------------------------------------------------------------------------
+// Exact transposes, wavefront-uniform (zero inclination).  A wavefront owns
+// one detector bin (B^T) or one voxel column (A^T) and 64*ZV consecutive z.
+// Lane l evaluates, in float64 and with exactly the per-lane cores'
+// arithmetic, the taps of row a0 + l of the bin's strip (B^T) or of angle
+// i0 + l (A^T): at most four (voxel or detector bin, weight) pairs with a
+// nonzero weight, in the per-lane cores' order.  They pass through a
+// per-wavefront LDS buffer; the wave then walks the rows (angles) in order
+// with one coalesced load per tap, so each z sums the same terms in the same
+// order as bpT_bin / fpT_voxel: the results are bitwise equal.
+// ---------------------------------------------------------------------------
+struct AdjSlot {
+  float4* w;   // [64] tap weights (0 = no tap)
+  int4* o;     // [64] tap voxel column (B^T) or sinogram row (A^T), -1 = none
+};
+__device__ __forceinline__ AdjSlot adj_lds_slot() {
+  __shared__ float4 w[4][64];
+  __shared__ int4 o[4][64];
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) & 3;
+  return {w[wv], o[wv]};
+}
+
+__device__ __forceinline__ void adj_put(float4& w, int4& o, int m, float wk, int off) {
+  if (m == 0) { w.x = wk; o.x = off; }
+  else if (m == 1) { w.y = wk; o.y = off; }
+  else if (m == 2) { w.z = wk; o.z = off; }
+  else { w.w = wk; o.w = off; }
+}
+
+template <int ZV>
+__device__ __forceinline__ typename VecT<ZV>::T adj_taps(const float* __restrict__ base, size_t stride, int zl,
+                                                         const AdjSlot& sl, int rows, typename VecT<ZV>::T acc) {
+  for (int r = 0; r < rows; ++r) {
+    const float4 w = sl.w[r];
+    const int4 o = sl.o[r];
+    const int o0 = __builtin_amdgcn_readfirstlane(o.x), o1 = __builtin_amdgcn_readfirstlane(o.y);
+    const int o2 = __builtin_amdgcn_readfirstlane(o.z), o3 = __builtin_amdgcn_readfirstlane(o.w);
+    if (o0 >= 0) acc = vfma(w.x, vload<ZV>(base + (size_t)o0 * stride + zl), acc);
+    if (o1 >= 0) acc = vfma(w.y, vload<ZV>(base + (size_t)o1 * stride + zl), acc);
+    if (o2 >= 0) acc = vfma(w.z, vload<ZV>(base + (size_t)o2 * stride + zl), acc);
+    if (o3 >= 0) acc = vfma(w.w, vload<ZV>(base + (size_t)o3 * stride + zl), acc);
+  }
+  return acc;
+}
+
+// B^T of bin (k, i): the voxels of each row (column, when |sin| > |cos|) of
+// the strip |t - t_k| < 1, rows in order (bpT_bin)
+template <int ZV>
+__device__ __forceinline__ typename VecT<ZV>::T bpT_bin_wave(const ParDev& g, const float* __restrict__ vol, int k,
+                                                              int i, int zl, int lane, int clipc) {
+  using V = typename VecT<ZV>::T;
+  const int n = g.n;
+  const double co = g.cs[2 * i], sn = g.cs[2 * i + 1];
+  const double xk = (double)(k - g.c) - g.det_offset;
+  const bool rows_major = fabs(co) >= fabs(sn);
+  const AdjSlot sl = adj_lds_slot();
+  V acc = vzero<ZV>();
+  for (int a0 = 0; a0 < n; a0 += 64) {
+    const int a = a0 + lane;
+    float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
+    int4 o = make_int4(-1, -1, -1, -1);
+    if (a < n) {
+      int b0, b1;
+      bpT_range(g, co, sn, xk, a, rows_major, b0, b1);
+      int m = 0;
+      for (int bb = b0; bb <= b1 && m < 4; ++bb) {
+        const int row = rows_major ? a : bb;
+        const int col = rows_major ? bb : a;
+        const int pix = row * n + col;
+        if (clipc && g.outside[pix]) continue;
+        const double x = (double)(col - g.c), y = (double)(row - g.c);
+        const double t = x * co - y * sn;
+        int j, j1;
+        float wj;
+        if (!interp_node(g, t, j, j1, wj)) continue;
+        float wk = 0.0f;
+        if (j == k) wk += 1.0f - wj;
+        if (j1 == k) wk += wj;
+        if (wk == 0.0f) continue;
+        if (g.mu != 0.0) wk *= abs_weight(g, t, x * sn + y * co);
+        adj_put(w, o, m, wk, pix);
+        ++m;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    sl.w[lane] = w;
+    sl.o[lane] = o;
+    __builtin_amdgcn_wave_barrier();
+    acc = adj_taps<ZV>(vol, (size_t)g.nz, zl, sl, n - a0 < 64 ? n - a0 : 64, acc);
+  }
+  return acc;
+}
+
+// A^T of voxel column pix: the detector bins of each angle whose samples
+// tap the voxel, angles in order (fpT_voxel)
+template <int ZV>
+__device__ __forceinline__ typename VecT<ZV>::T fpT_voxel_wave(const ParDev& g, const float* __restrict__ sino,
+                                                                int pix, int zl, int lane) {
+  using V = typename VecT<ZV>::T;
+  const int n = g.n;
+  const int row = pix / n, col = pix % n;
+  const double X = (double)(col - g.c), Y = (double)(row - g.c);
+  const AdjSlot sl = adj_lds_slot();
+  V acc = vzero<ZV>();
+  for (int i0 = 0; i0 < g.na; i0 += 64) {
+    const int i = i0 + lane;
+    float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
+    int4 o = make_int4(-1, -1, -1, -1);
+    if (i < g.na) {
+      const double* M = g.mats + 6 * i;
+      const double co = M[0], sn = M[1];
+      const double kstar = co * X - sn * Y + (double)g.c + g.det_offset;
+      const double rstar = sn * X + co * Y + (double)g.c;
+      const int kb = (int)floor(kstar) - 1, rb = (int)floor(rstar) - 1;
+      int m = 0;
+      for (int kk = kb; kk < kb + 4; ++kk) {
+        if (kk < 0 || kk >= n) continue;
+        const double kp = (double)kk - g.det_offset;
+        const double a0 = M[0] * kp, a1 = M[3] * kp;
+        const double t = (double)(kk - g.c) - g.det_offset;
+        float wk = 0.0f;
+        for (int r = rb; r < rb + 4; ++r) {
+          if (r < 0 || r >= n) continue;
+          const double rr = (double)r;
+          const double colS = (a0 + M[1] * rr) + M[2];
+          const double rowS = (a1 + M[4] * rr) + M[5];
+          if (!(fabs(rowS - (double)row) < 1.0 && fabs(colS - (double)col) < 1.0)) continue;
+          const float wr = tap_weight_axis(rowS, row);
+          const float wc = tap_weight_axis(colS, col);
+          wk = __builtin_fmaf(wr * wc, abs_weight(g, t, (double)(r - g.c)), wk);
+        }
+        if (wk != 0.0f) {
+          adj_put(w, o, m, wk, kk * g.na + i);
+          ++m;
+        }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    sl.w[lane] = w;
+    sl.o[lane] = o;
+    __builtin_amdgcn_wave_barrier();
+    acc = adj_taps<ZV>(sino, (size_t)g.nz, zl, sl, g.na - i0 < 64 ? g.na - i0 : 64, acc);
+  }
+  return acc;
+}
 __device__ __forceinline__ float rl_f(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }

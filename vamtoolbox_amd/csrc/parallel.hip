@@ -282,6 +282,40 @@ __global__ __launch_bounds__(256) void par_fpT_kernel(ParDev g, const float* __r
   vstore<ZV>(vol + (size_t)pix * g.nz + z, fpT_voxel<ZV>(g, sino, pix, z));
 }
 
+// wavefront-uniform exact transposes: one wavefront = one voxel column (A^T)
+// or one detector bin (B^T) x 64*ZV z; bitwise equal to the per-lane kernels
+template <int ZV>
+__global__ __launch_bounds__(256) void par_fpT_wave_kernel(ParDev g, const float* __restrict__ sino,
+                                                           float* __restrict__ vol) {
+  const int lane = threadIdx.x & 63;
+  const long long w = wave_index(xcd_remap(blockIdx.x, gridDim.x));
+  const int nch = wave_zchunks(g.nz, ZV);
+  if (w >= (long long)g.n * g.n * nch) return;   // whole wavefronts
+  const int pix = (int)(w / nch);
+  const int z = (int)(w % nch) * 64 * ZV + lane * ZV;
+  int zl, zl2;
+  wave_zload<ZV, false>(z, g.nz, zl, zl2);
+  const typename VecT<ZV>::T acc = fpT_voxel_wave<ZV>(g, sino, pix, zl, lane);
+  wave_zstore<ZV, false>(vol + (size_t)pix * g.nz, z, g.nz, acc);
+}
+
+template <int ZV>
+__global__ __launch_bounds__(256) void par_bpT_wave_kernel(ParDev g, const float* __restrict__ vol,
+                                                           float* __restrict__ sino, int clipc) {
+  const int lane = threadIdx.x & 63;
+  const long long w = wave_index(xcd_remap(blockIdx.x, gridDim.x));
+  const int nch = wave_zchunks(g.nz, ZV);
+  if (w >= (long long)g.n * g.na * nch) return;  // whole wavefronts
+  // consecutive wavefronts: neighbouring bins of one angle (overlapping strips)
+  const long long bin = w / nch;
+  const int k = (int)(bin % g.n), i = (int)(bin / g.n);
+  const int z = (int)(w % nch) * 64 * ZV + lane * ZV;
+  int zl, zl2;
+  wave_zload<ZV, false>(z, g.nz, zl, zl2);
+  const typename VecT<ZV>::T acc = bpT_bin_wave<ZV>(g, vol, k, i, zl, lane, clipc);
+  wave_zstore<ZV, false>(sino + ((size_t)k * g.na + i) * g.nz, z, g.nz, acc);
+}
+
 __global__ __launch_bounds__(256) void par_fpT_incl_kernel(ParDev g, const float* __restrict__ sino,
                                                            float* __restrict__ vol) {
   const long long total = (long long)g.n * g.n * g.nz;
@@ -702,7 +736,14 @@ extern "C" int vamx_par_forward_adjoint(vamx_plan* plan, const float* sino, floa
   if (plan->par.occ) return set_error(VAMX_ERR_STATE, "vamx_par_forward_adjoint: not available with occlusion");
   const ParDev& g = plan->par;
   hipStream_t s = (hipStream_t)stream;
-  if (g.incl) {
+  const int wz = (!g.incl && !(plan->flags & VAMX_PAR_PER_LANE_GEOMETRY)) ? par_wave_zv(plan, sino, vol, 4) : 0;
+  if (wz) {
+    const long long threads = (long long)g.n * g.n * wave_zchunks(g.nz, wz) * 64;
+    const dim3 grid(blocks_for(threads, 256)), block(256);
+    if (wz == 4) hipLaunchKernelGGL((par_fpT_wave_kernel<4>), grid, block, 0, s, g, sino, vol);
+    else if (wz == 2) hipLaunchKernelGGL((par_fpT_wave_kernel<2>), grid, block, 0, s, g, sino, vol);
+    else hipLaunchKernelGGL((par_fpT_wave_kernel<1>), grid, block, 0, s, g, sino, vol);
+  } else if (g.incl) {
     const long long total = (long long)g.n * g.n * g.nz;
     hipLaunchKernelGGL(par_fpT_incl_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, s, g, sino, vol);
   } else if (par_use_vec4(plan, sino, vol)) {
@@ -724,7 +765,14 @@ extern "C" int vamx_par_backward_adjoint(vamx_plan* plan, const float* vol, floa
   const ParDev& g = plan->par;
   hipStream_t s = (hipStream_t)stream;
   const int clipc = (plan->flags & VAMX_PAR_NO_CIRCLE_CLIP) ? 0 : 1;
-  if (g.incl) {
+  const int wz = (!g.incl && !(plan->flags & VAMX_PAR_PER_LANE_GEOMETRY)) ? par_wave_zv(plan, vol, sino, 4) : 0;
+  if (wz) {
+    const long long threads = (long long)g.n * g.na * wave_zchunks(g.nz, wz) * 64;
+    const dim3 grid(blocks_for(threads, 256)), block(256);
+    if (wz == 4) hipLaunchKernelGGL((par_bpT_wave_kernel<4>), grid, block, 0, s, g, vol, sino, clipc);
+    else if (wz == 2) hipLaunchKernelGGL((par_bpT_wave_kernel<2>), grid, block, 0, s, g, vol, sino, clipc);
+    else hipLaunchKernelGGL((par_bpT_wave_kernel<1>), grid, block, 0, s, g, vol, sino, clipc);
+  } else if (g.incl) {
     const long long total = (long long)g.n * g.na * g.nz;
     hipLaunchKernelGGL(par_bpT_incl_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, s, g, vol, sino, clipc);
   } else if (par_use_vec4(plan, vol, sino)) {
