@@ -329,3 +329,43 @@ def test_pm_matches_oracle(gpu):
     print("PM rel L2 sinogram %.2e recon %.2e; loss %.4g -> %.4g" % (e_b, e_x, L0, L1))
     assert L1 < L0
     assert e_b <= 1e-3 and e_x <= 1e-3
+
+
+def test_bclp_eval_then_update_equals_step(gpu):
+    """vamx_bclp_eval (operand left in the workspace) followed by
+    vamx_bclp_update is vamx_bclp_step bit for bit, iteration after iteration."""
+    import ctypes
+    import torch
+    from vamtoolbox_amd import _lib
+    from vamtoolbox_amd.parallel import ParallelPlan
+    from vamtoolbox_amd.response import ResponseModel
+    rng = np.random.default_rng(8)
+    n, nz, na = 48, 64, 90
+    plan = ParallelPlan(n, nz, np.linspace(0, 360, na, endpoint=False), mu=0.01, det_offset=3.0, device=gpu)
+    T = torch.tensor(np.clip(rng.random((n, n, nz)) * 1.2 - 0.1, 0, 1), dtype=torch.float32, device=gpu)
+    g0 = torch.tensor(rng.random((n, na, nz)) * 0.05, dtype=torch.float32, device=gpu)
+    lib = _lib.load()
+    resp, _ = ResponseModel(type="analytical", B=20, M=0.6).vamx_struct(gpu)
+    prm = _lib.BclpParams(p=2.0, q=1.0, eps=0.05, weight=1.0, learning_rate=1e-3, glb=0.0, gub=np.inf, dvol=1.0,
+                          grad_op=0)
+    nbytes = ctypes.c_int64()
+    _lib.check(lib.vamx_bclp_workspace_bytes(plan.handle, ctypes.byref(nbytes)), "workspace")
+    st = _lib.stream_ptr(gpu)
+    ws1 = torch.empty(int(nbytes.value), device=gpu, dtype=torch.uint8)
+    ws2 = torch.empty_like(ws1)
+    g1, g2 = g0.clone(), g0.clone()
+    d1, d2 = torch.empty(plan.vol_shape, device=gpu), torch.empty(plan.vol_shape, device=gpu)
+    log = torch.zeros(4, device=gpu, dtype=torch.float64)
+    cnt = torch.zeros(1, device=gpu, dtype=torch.int32)
+    l2 = torch.zeros(1, device=gpu, dtype=torch.float64)
+    for it in range(3):
+        _lib.check(lib.vamx_bclp_step(plan.handle, ctypes.byref(resp), ctypes.byref(prm), g1.data_ptr(), T.data_ptr(),
+                                      0, 0, d1.data_ptr(), log.data_ptr(), cnt.data_ptr(), ws1.data_ptr(), st), "step")
+        _lib.check(lib.vamx_bclp_eval(plan.handle, ctypes.byref(resp), ctypes.byref(prm), g2.data_ptr(), T.data_ptr(),
+                                      0, 0, d2.data_ptr(), 0, l2.data_ptr(), ws2.data_ptr(), st), "eval")
+        _lib.check(lib.vamx_bclp_update(plan.handle, ctypes.byref(prm), g2.data_ptr(), ws2.data_ptr(), st), "update")
+        torch.cuda.synchronize()
+        assert torch.equal(d1, d2), it
+        assert float(log[it].item()) == float(l2.item()), it
+        assert torch.equal(g1, g2), it
+    assert not torch.equal(g1, g0)

@@ -1,0 +1,208 @@
+"""The GPU against the float64 oracle at the benchmark configurations' real
+sizes, on sampled z-slices (at zero inclination every slice is an
+independent problem, so a slice of the full-size result is checked against
+the oracle run on that slice alone):
+
+* the 64 uneven problems of the batched configuration ({128..384}^2 x
+  {32..128} slices, 90-720 angles, absorption 0-0.02 per voxel) through the
+  batch entry points;
+* the BCLP configuration's geometry (256^3 x 360 angles, absorption 0.01 per
+  voxel, detector offset 10): forward, back projection and one fused BCLP
+  iteration at the benchmark's learning rate;
+* fused BCLP iterations at that learning rate, where the loss is not
+  monotone, against the oracle and against the oracle's own sensitivity to
+  float32 rounding.
+"""
+
+import numpy as np
+import pytest
+
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _config4_specs(seed=4):
+    """64 problems drawn as the batched configuration: nx = ny in {128, 192,
+    256, 384}, nz in 32..128, angles in {90, 180, 360, 720} over 360 deg,
+    absorption uniform in [0, 0.02] per voxel."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(64):
+        n = int(rng.choice([128, 192, 256, 384]))
+        nz = int(rng.integers(32, 129))
+        na = int(rng.choice([90, 180, 360, 720]))
+        mu = float(rng.uniform(0.0, 0.02))
+        out.append((n, nz, na, mu))
+    return out
+
+
+def test_config4_batch_against_oracle(gpu):
+    """Batched forward and back projection of the 64-problem mix: one sampled
+    slice of every eighth problem against the oracle (relative L2 <= 1e-5),
+    every problem bitwise against its own single-problem launch."""
+    import torch
+    import oracle.d1 as d1
+    from vamtoolbox_amd.batch import ParallelBatch
+    from vamtoolbox_amd.parallel import ParallelPlan
+    specs = _config4_specs()
+    plans = [ParallelPlan(n, nz, np.linspace(0, 360, na, endpoint=False), mu=mu, device=gpu)
+             for n, nz, na, mu in specs]
+    B = ParallelBatch(plans)
+    gen = torch.Generator(device=gpu).manual_seed(11)
+    vol = torch.rand(B.n_vol, device=gpu, generator=gen)
+    sino = torch.rand(B.n_sino, device=gpu, generator=gen)
+    fp = B.forward(vol)
+    bp = B.backward(sino)
+    torch.cuda.synchronize()
+    vv, sv, fv, bv = B.vol_views(vol), B.sino_views(sino), B.sino_views(fp), B.vol_views(bp)
+    rng = np.random.default_rng(5)
+    worst = 0.0
+    for i, (p, (n, nz, na, mu)) in enumerate(zip(plans, specs)):
+        assert torch.equal(fv[i], p.forward(vv[i])), i
+        assert torch.equal(bv[i], p.backward(sv[i])), i
+        if i % 8:
+            continue
+        z = int(rng.integers(0, nz))
+        ang = np.linspace(0, 360, na, endpoint=False)
+        print("problem %d: n %d nz %d angles %d mu %.4f, slice %d" % (i, n, nz, na, mu, z), flush=True)
+        x = vv[i][:, :, z:z + 1].double().cpu().numpy()
+        s = sv[i][:, :, z:z + 1].double().cpu().numpy()
+        e_f = rel_l2(fv[i][:, :, z:z + 1].cpu().numpy(), d1.forward(x, ang, mu=mu))
+        e_b = rel_l2(bv[i][:, :, z:z + 1].cpu().numpy(), d1.backward(s, ang, mu=mu))
+        print("   rel L2 forward %.2e backward %.2e" % (e_f, e_b), flush=True)
+        worst = max(worst, e_f, e_b)
+    print("worst rel L2 %.2e" % worst)
+    assert worst <= TOL
+
+
+def _bench_problem(gpu):
+    """The BCLP configuration: 256^3 smoothed ellipsoid target in [0, 1], 360
+    angles, absorption 0.01 per voxel, detector offset 10, sigmoid response
+    (M = 0.6, B = 20); initial sinogram = forward projection of the target,
+    scaled so that the mean dose inside the target is 0.7."""
+    import torch
+    from scipy.ndimage import gaussian_filter
+    from vamtoolbox_amd.parallel import ParallelPlan
+    from tests.test_gpu_optim import _ellipsoids
+    n, nz, na = 256, 256, 360
+    T = np.clip(gaussian_filter(_ellipsoids(n, nz, seed=3, count=8), 2.0), 0, 1).astype(np.float32)
+    plan = ParallelPlan(n, nz, np.arange(na, dtype=np.float64), mu=0.01, det_offset=10.0, device=gpu)
+    Tt = torch.tensor(T, device=gpu)
+    g0 = torch.clamp(plan.forward(Tt), min=0)
+    d0 = plan.backward(g0)
+    g0 = (g0 * (0.7 / float(d0[Tt > 0.5].mean()))).contiguous()
+    return plan, T, Tt, g0
+
+
+def test_bclp_benchmark_geometry_against_oracle(gpu):
+    """256^3 x 360, absorption 0.01, offset 10: the forward and back projection
+    on two slices, and one fused BCLP iteration (learning rate 0.01): the
+    updated sinogram's slices against the oracle iteration given the GPU's
+    loss, and the GPU's loss against its float64 evaluation from the GPU's
+    dose."""
+    import ctypes
+    import torch
+    import oracle.d1 as d1
+    import oracle.optim as oo
+    from vamtoolbox_amd import _lib
+    from vamtoolbox_amd.response import ResponseModel
+    plan, T, Tt, g0 = _bench_problem(gpu)
+    ang = plan.angles_deg
+    kw = dict(mu=0.01, det_offset=10.0)
+    fp = plan.forward(Tt)
+    bp = plan.backward(g0)
+    lib = _lib.load()
+    rm = ResponseModel(type="analytical", B=20, M=0.6)
+    resp, _ = rm.vamx_struct(gpu)
+    lr = 0.01
+    prm = _lib.BclpParams(p=2.0, q=1.0, eps=0.05, weight=1.0, learning_rate=lr, glb=0.0, gub=np.inf, dvol=1.0,
+                          grad_op=0)
+    nbytes = ctypes.c_int64()
+    _lib.check(lib.vamx_bclp_workspace_bytes(plan.handle, ctypes.byref(nbytes)), "workspace")
+    ws = torch.empty(int(nbytes.value), device=gpu, dtype=torch.uint8)
+    g = g0.clone()
+    dose = torch.empty(plan.vol_shape, device=gpu)
+    loss = torch.zeros(2, device=gpu, dtype=torch.float64)
+    cnt = torch.zeros(1, device=gpu, dtype=torch.int32)
+    _lib.check(lib.vamx_bclp_step(plan.handle, ctypes.byref(resp), ctypes.byref(prm), g.data_ptr(), Tt.data_ptr(),
+                                  0, 0, dose.data_ptr(), loss.data_ptr(), cnt.data_ptr(), ws.data_ptr(),
+                                  _lib.stream_ptr(gpu)), "vamx_bclp_step")
+    torch.cuda.synchronize()
+    L_gpu = float(loss[0].item())
+    # the loss in float64 from the GPU's dose (the whole volume)
+    ro = oo.Response("analytical", B=20, M=0.6)
+    dd = dose.double().cpu().numpy()
+    eb = np.abs(ro.map(dd) - T) - 0.05
+    L_host = float(np.sqrt(np.sum((eb > 0) * eb ** 2)))
+    e_loss = abs(L_gpu - L_host) / L_host
+    print("loss GPU %.9e float64 from the GPU dose %.9e: rel %.2e" % (L_gpu, L_host, e_loss), flush=True)
+    assert e_loss <= 1e-6
+    scale = 1.0 * L_gpu ** ((1.0 - 2.0) / 1.0)
+    g0h, gh = g0.double().cpu().numpy(), g.double().cpu().numpy()
+    worst = 0.0
+    for z in (40, 128, 201):
+        sl = slice(z, z + 1)
+        e_f = rel_l2(fp[:, :, sl].cpu().numpy(), d1.forward(T[:, :, sl].astype(np.float64), ang, **kw))
+        e_b = rel_l2(bp[:, :, sl].cpu().numpy(), d1.backward(g0h[:, :, sl], ang, **kw))
+        # one BCLP iteration on the slice (BCLP.py:477-534) with the GPU's loss scale
+        d_ref, _, _, op_ref = oo.bclp_state(g0h[:, :, sl], T[:, :, sl], lambda s: d1.backward(s, ang, **kw), ro,
+                                            0.05, 1.0, 2.0, 1.0)
+        g_ref = np.clip(g0h[:, :, sl] - lr * scale * d1.forward(op_ref, ang, **kw), 0.0, None)
+        e_d = rel_l2(dose[:, :, sl].cpu().numpy(), d_ref)
+        e_g = rel_l2(gh[:, :, sl], g_ref)
+        e_dg = rel_l2(gh[:, :, sl] - g0h[:, :, sl], g_ref - g0h[:, :, sl])
+        print("slice %d: rel L2 forward %.2e backward %.2e dose %.2e g %.2e (update %.2e)" % (
+            z, e_f, e_b, e_d, e_g, e_dg), flush=True)
+        assert e_f <= TOL and e_b <= TOL and e_d <= TOL
+        assert e_g <= 1e-4 and e_dg <= 1e-4
+        worst = max(worst, e_f, e_b, e_d)
+    print("worst projection / dose rel L2 %.2e" % worst)
+
+
+def test_bclp_at_benchmark_learning_rate(gpu):
+    """Fused iterations at learning rate 0.01 from an initial sinogram scaled
+    so that the mean dose in the target is 0.7 (as the benchmark does): the
+    loss oscillates instead of falling monotonically.  There the iteration
+    amplifies any perturbation (about x2.4 per iteration after the tenth): the
+    float64 oracle started from the float32 rounding of the same initial
+    sinogram (a 2.5e-8 change) drifts 5.5e-6 from itself by iteration 20 and
+    18 % by iteration 50, and the float32 rounding of every iteration grows
+    alike (1.3e-4 by iteration 20).  The GPU iterates match the oracle to 1e-4
+    over the first 15 iterations; at 50 the float32-rounded start is as far
+    from the float64 run as the GPU is, so the 1e-4 bar is a property of the
+    problem's conditioning there, not of the kernels."""
+    import oracle.d1 as d1
+    import oracle.optim as oo
+    import vamtoolbox_amd as vam
+    from tests.test_gpu_optim import _problem
+    tg, pg, ang = _problem(n=32, nz=8, na=45, seed=2, gray=True)
+    fwd = lambda v: d1.forward(v, ang)          # noqa: E731
+    bwd = lambda s: d1.backward(s, ang)         # noqa: E731
+    ro = oo.Response("analytical", B=20, M=0.6)
+    g0 = oo.bclp_init(tg.array, fwd, oo.Response("interpolation", B=20, M=0.6), len(ang))
+    g0 = g0 * (0.7 / bwd(g0)[tg.array > 0.5].mean())
+    lr = 0.01
+    rm = vam.response.ResponseModel(type="analytical", B=20, M=0.6)
+    out = {}
+    for n_iter in (15, 50):
+        opts = vam.optimize.Options(method="BCLP", n_iter=n_iter, response_model=rm, eps=0.05, p=2, q=1,
+                                    learning_rate=lr, g0=g0, fused=True)
+        g, dose, mapped, logs = vam.optimize.optimize(tg, pg, opts, output="components")
+        g_ref, dose_ref, m_ref, loss_ref = oo.bclp(tg.array, fwd, bwd, ro, n_iter, g0, eps=0.05, lr=lr)
+        g_f32, dose_f32, _, _ = oo.bclp(tg.array, fwd, bwd, ro, n_iter, g0.astype(np.float32).astype(np.float64),
+                                        eps=0.05, lr=lr)
+        e_g, e_d = rel_l2(g, g_ref), rel_l2(dose, dose_ref)
+        s_g = rel_l2(g_f32, g_ref)
+        e_l = np.max(np.abs(logs.loss - loss_ref) / np.abs(loss_ref))
+        print("BCLP x%d at lr 0.01: GPU vs oracle rel L2 g %.2e dose %.2e loss %.2e; oracle from float32(g0) vs "
+              "oracle %.2e" % (n_iter, e_g, e_d, e_l, s_g))
+        out[n_iter] = (e_g, e_d, e_l, s_g, loss_ref)
+    e_g, e_d, e_l, s_g, loss_ref = out[15]
+    assert not np.all(np.diff(loss_ref) < 0), "the loss must oscillate at this learning rate"
+    assert e_g <= 1e-4 and e_d <= 1e-4 and e_l <= 1e-4
+    e_g, e_d, e_l, s_g, loss_ref = out[50]
+    assert s_g > 1e-2, "the oracle's own sensitivity at 50 iterations"
+    assert e_g <= 10 * s_g

@@ -14,104 +14,10 @@
 // The per-voxel epilogue runs in float64, as the reference does in NumPy.
 #include <mutex>
 
+#include "bclp_core.h"
 #include "par_core.h"
 
 namespace vamx {
-
-struct RespDev {
-  int kind;
-  double A, K, B, M, nu, lin_M, lin_C;
-  int n_pts;
-  const double* tf;
-  const double* tm;
-  const double* td;
-};
-
-struct BclpDev {
-  double p, q, eps, weight, dvol;
-  const float* eps_arr;
-  const float* w_arr;
-};
-
-// np.interp(x, xp, fp, left, right) for increasing xp (numpy's branch order)
-__device__ double np_interp(double x, const double* __restrict__ xp, const double* __restrict__ fp, int n,
-                            double left, double right) {
-  if (x > xp[n - 1]) return right;
-  if (x < xp[0]) return left;
-  if (x == xp[n - 1]) return fp[n - 1];
-  int lo = 0, hi = n - 1;  // xp[lo] <= x < xp[hi]
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (xp[mid] <= x) lo = mid; else hi = mid;
-  }
-  if (xp[lo] == x) return fp[lo];
-  const double slope = (fp[lo + 1] - fp[lo]) / (xp[lo + 1] - xp[lo]);
-  return slope * (x - xp[lo]) + fp[lo];
-}
-
-// response map and derivative (response.py:191-218, 233-253, 256-279)
-__device__ void response_eval(const RespDev& r, double f, double& m, double& dm) {
-  if (r.kind == VAMX_RESP_GLF) {
-    const double e = exp(-r.B * (f - r.M));
-    if (r.nu == 1.0) {
-      m = r.A + (r.K - r.A) / (1.0 + e);
-      const double c1 = 1.0 / (r.K - r.A);
-      const double d = m - r.A;
-      dm = c1 * r.B * (d * d) * e;
-    } else {
-      m = r.A + (r.K - r.A) / pow(1.0 + e, 1.0 / r.nu);
-      const double c1 = pow(1.0 / (r.K - r.A), r.nu);
-      const double c2 = r.B / r.nu;
-      const double c3 = pow(m - r.A, r.nu + 1.0);
-      dm = c1 * c2 * c3 * e;
-    }
-  } else if (r.kind == VAMX_RESP_LINEAR) {
-    m = r.lin_M * f + r.lin_C;
-    dm = r.lin_M;
-  } else if (r.kind == VAMX_RESP_IDENTITY) {
-    m = f;
-    dm = 1.0;
-  } else {
-    const int n = r.n_pts;
-    m = np_interp(f, r.tf, r.tm, n, r.tm[0], r.tm[n - 1]);
-    dm = np_interp(f, r.tf, r.td, n, r.td[0], r.td[n - 1]);
-  }
-}
-
-// BCLP epilogue for one voxel: returns the loss integrand, writes the operand
-__device__ __forceinline__ double bclp_voxel(const RespDev& r, const BclpDev& b, float dose, float target,
-                                             size_t idx, float& operand) {
-  double m, dm;
-  response_eval(r, (double)dose, m, dm);
-  const double eT = m - (double)target;                       // BCLP.py:298-300
-  const double eps = b.eps_arr ? (double)b.eps_arr[idx] : b.eps;
-  const double eb = fabs(eT) - eps;                            // 305-307
-  const bool v = eb > 0.0;                                     // 311
-  if (!v) {
-    operand = 0.0f;
-    return 0.0;
-  }
-  const double w = b.w_arr ? (double)b.w_arr[idx] : b.weight;
-  const double sgn = eT > 0.0 ? 1.0 : (eT < 0.0 ? -1.0 : 0.0);
-  const double ab = fabs(eb);
-  if (b.p == 2.0) {
-    operand = (float)(w * ab * sgn * dm);
-    return w * (ab * ab);
-  }
-  operand = (float)(w * pow(ab, b.p - 1.0) * sgn * dm);  // 361-367
-  return w * pow(ab, b.p);                               // 340-342
-}
-
-__device__ __forceinline__ double block_sum_256(double v, double* sh) {
-  const int t = threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (t < s) sh[t] += sh[t + s];
-    __syncthreads();
-  }
-  return sh[0];
-}
 
 // A: back projection + response + band/Lp epilogue + loss partials
 template <int ZV, bool INCL>
@@ -332,6 +238,33 @@ __global__ __launch_bounds__(256) void bclp_bpT_update(ParDev g, int clipc, cons
   vstore<ZV>(gp, gv);
 }
 
+int bclp_fill_devs(const vamx_response* resp, const vamx_bclp_params* prm, const float* eps_arr, const float* w_arr,
+                   RespDev& r, BclpDev& b) {
+  if (!resp || !prm) return set_error(VAMX_ERR_ARG, "bclp: null response or params");
+  if (resp->kind < VAMX_RESP_GLF || resp->kind > VAMX_RESP_TABLE)
+    return set_error(VAMX_ERR_ARG, "bclp: unknown response kind");
+  if (resp->kind == VAMX_RESP_TABLE && (resp->n_pts < 2 || !resp->table_f || !resp->table_map || !resp->table_dmap))
+    return set_error(VAMX_ERR_ARG, "bclp: interpolation response needs n_pts >= 2 and three device tables");
+  if (!(prm->p > 0.0) || !(prm->q > 0.0)) return set_error(VAMX_ERR_ARG, "bclp: p and q must be > 0");
+  r.kind = resp->kind;
+  r.A = resp->A; r.K = resp->K; r.B = resp->B; r.M = resp->M; r.nu = resp->nu;
+  r.lin_M = resp->lin_M; r.lin_C = resp->lin_C;
+  r.n_pts = resp->n_pts;
+  r.tf = resp->table_f; r.tm = resp->table_map; r.td = resp->table_dmap;
+  b.p = prm->p; b.q = prm->q; b.eps = prm->eps; b.weight = prm->weight; b.dvol = prm->dvol;
+  b.eps_arr = eps_arr;
+  b.w_arr = w_arr;
+  return VAMX_OK;
+}
+
+int bclp_reduce_launch(const double* partials, int nparts, const BclpDev& b, double* loss_dst, int32_t* counter,
+                       double* loss_log, double* scale_out, hipStream_t s) {
+  hipLaunchKernelGGL(bclp_reduce, dim3(1), dim3(256), 0, s, partials, nparts, b.dvol, b.p, b.q, loss_dst, counter,
+                     loss_log, scale_out);
+  VAMX_HIP(hipGetLastError());
+  return VAMX_OK;
+}
+
 }  // namespace vamx
 
 using namespace vamx;
@@ -355,25 +288,6 @@ WsLayout ws_layout(const vamx_plan* p) {
   w.operand_off = w.scale_off + 256;
   w.total = w.operand_off + (size_t)items * sizeof(float);
   return w;
-}
-
-int fill_devs(const vamx_response* resp, const vamx_bclp_params* prm, const float* eps_arr, const float* w_arr,
-              RespDev& r, BclpDev& b) {
-  if (!resp || !prm) return set_error(VAMX_ERR_ARG, "bclp: null response or params");
-  if (resp->kind < VAMX_RESP_GLF || resp->kind > VAMX_RESP_TABLE)
-    return set_error(VAMX_ERR_ARG, "bclp: unknown response kind");
-  if (resp->kind == VAMX_RESP_TABLE && (resp->n_pts < 2 || !resp->table_f || !resp->table_map || !resp->table_dmap))
-    return set_error(VAMX_ERR_ARG, "bclp: interpolation response needs n_pts >= 2 and three device tables");
-  if (!(prm->p > 0.0) || !(prm->q > 0.0)) return set_error(VAMX_ERR_ARG, "bclp: p and q must be > 0");
-  r.kind = resp->kind;
-  r.A = resp->A; r.K = resp->K; r.B = resp->B; r.M = resp->M; r.nu = resp->nu;
-  r.lin_M = resp->lin_M; r.lin_C = resp->lin_C;
-  r.n_pts = resp->n_pts;
-  r.tf = resp->table_f; r.tm = resp->table_map; r.td = resp->table_dmap;
-  b.p = prm->p; b.q = prm->q; b.eps = prm->eps; b.weight = prm->weight; b.dvol = prm->dvol;
-  b.eps_arr = eps_arr;
-  b.w_arr = w_arr;
-  return VAMX_OK;
 }
 
 // Launches kernel A; its per-block loss partials go to partials[0, nparts).
@@ -440,16 +354,11 @@ extern "C" int vamx_bclp_workspace_bytes(vamx_plan* plan, int64_t* bytes) {
 
 namespace {
 
-// Kernels A, B, C of one iteration; partials has max_parts slots, scale one.
-int bclp_step_impl(vamx_plan* plan, const RespDev& r, const BclpDev& b, const vamx_bclp_params* prm, float* g,
-                   const float* target, float* dose, float* operand, double* partials, int max_parts,
-                   double* scale, double* loss_log, int32_t* iter_counter, hipStream_t s) {
-  int nparts = 0;
-  int rc = launch_epilogue(plan, r, b, g, target, dose, operand, partials, max_parts, nparts, s);
-  if (rc) return rc;
-  hipLaunchKernelGGL(bclp_reduce, dim3(1), dim3(256), 0, s, partials, nparts, b.dvol, b.p, b.q, (double*)nullptr,
-                     iter_counter, loss_log, scale);
-  VAMX_HIP(hipGetLastError());
+// Kernel C (with the slice min/max pre-pass of the warp clip): the update
+// g <- clip(g - lr * scale * A operand) with the device scale
+int bclp_update_impl(vamx_plan* plan, const vamx_bclp_params* prm, float* g, const float* operand,
+                     const double* scale, hipStream_t s) {
+  int rc = VAMX_OK;
   const ParDev& gd = plan->par;
   const float lr = (float)prm->learning_rate, glb = (float)prm->glb, gub = (float)prm->gub;
   const bool v4 = !gd.incl && gd.nz % 4 == 0 && ((uintptr_t)g | (uintptr_t)operand) % 16 == 0;
@@ -533,6 +442,19 @@ int bclp_step_impl(vamx_plan* plan, const RespDev& r, const BclpDev& b, const va
   return VAMX_OK;
 }
 
+// Kernels A, B, C of one iteration; partials has max_parts slots, scale one.
+int bclp_step_impl(vamx_plan* plan, const RespDev& r, const BclpDev& b, const vamx_bclp_params* prm, float* g,
+                   const float* target, float* dose, float* operand, double* partials, int max_parts,
+                   double* scale, double* loss_log, int32_t* iter_counter, hipStream_t s) {
+  int nparts = 0;
+  int rc = launch_epilogue(plan, r, b, g, target, dose, operand, partials, max_parts, nparts, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(bclp_reduce, dim3(1), dim3(256), 0, s, partials, nparts, b.dvol, b.p, b.q, (double*)nullptr,
+                     iter_counter, loss_log, scale);
+  VAMX_HIP(hipGetLastError());
+  return bclp_update_impl(plan, prm, g, operand, scale, s);
+}
+
 
 }  // namespace
 
@@ -544,7 +466,7 @@ extern "C" int vamx_bclp_step(vamx_plan* plan, const vamx_response* resp, const 
     return set_error(VAMX_ERR_ARG, "vamx_bclp_step: null buffer");
   RespDev r;
   BclpDev b;
-  int rc = fill_devs(resp, prm, eps_arr, weight_arr, r, b);
+  int rc = bclp_fill_devs(resp, prm, eps_arr, weight_arr, r, b);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
   const WsLayout w = ws_layout(plan);
@@ -556,6 +478,16 @@ extern "C" int vamx_bclp_step(vamx_plan* plan, const vamx_response* resp, const 
                         iter_counter, s);
 }
 
+extern "C" int vamx_bclp_update(vamx_plan* plan, const vamx_bclp_params* prm, float* g, void* workspace,
+                                void* stream) {
+  if (!check_plan(plan, PLAN_PARALLEL)) return set_error(VAMX_ERR_STATE, "vamx_bclp_update: not a parallel plan");
+  if (!prm || !g || !workspace) return set_error(VAMX_ERR_ARG, "vamx_bclp_update: null argument");
+  const WsLayout w = ws_layout(plan);
+  char* ws = (char*)workspace;
+  return bclp_update_impl(plan, prm, g, (const float*)(ws + w.operand_off), (const double*)(ws + w.scale_off),
+                          (hipStream_t)stream);
+}
+
 extern "C" int vamx_bclp_eval(vamx_plan* plan, const vamx_response* resp, const vamx_bclp_params* prm,
                               const float* g, const float* target, const float* eps_arr, const float* weight_arr,
                               float* dose, float* operand, double* loss_out, void* workspace, void* stream) {
@@ -564,7 +496,7 @@ extern "C" int vamx_bclp_eval(vamx_plan* plan, const vamx_response* resp, const 
     return set_error(VAMX_ERR_ARG, "vamx_bclp_eval: null buffer");
   RespDev r;
   BclpDev b;
-  int rc = fill_devs(resp, prm, eps_arr, weight_arr, r, b);
+  int rc = bclp_fill_devs(resp, prm, eps_arr, weight_arr, r, b);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
   const WsLayout w = ws_layout(plan);
@@ -628,7 +560,7 @@ extern "C" int vamx_bclp_step_noalloc(vamx_plan* plan, const vamx_response* resp
     return set_error(VAMX_ERR_ARG, "vamx_bclp_step_noalloc: null buffer");
   RespDev r;
   BclpDev b;
-  int rc = fill_devs(resp, prm, eps_arr, weight_arr, r, b);
+  int rc = bclp_fill_devs(resp, prm, eps_arr, weight_arr, r, b);
   if (rc) return rc;
   int dev = -1;
   VAMX_HIP(hipGetDevice(&dev));
