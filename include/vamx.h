@@ -164,6 +164,14 @@ int vamx_bclp_eval(vamx_plan* plan, const vamx_response* resp, const vamx_bclp_p
                    const float* weight_arr, float* dose, float* operand, double* loss_out,
                    void* workspace, void* stream);
 
+/* The update half of vamx_bclp_step: g <- clip(g - lr * s * A operand, glb, gub)
+ * with the operand and the gradient scale s that the last vamx_bclp_step, or
+ * vamx_bclp_eval called with operand == NULL, left in this workspace
+ * (BCLP.py:353-375, 486-487, 532).  vamx_bclp_eval (operand NULL) followed by
+ * vamx_bclp_update on one stream is one vamx_bclp_step, bit for bit, without
+ * the iteration counter and loss log. */
+int vamx_bclp_update(vamx_plan* plan, const vamx_bclp_params* prm, float* g, void* workspace, void* stream);
+
 /* vamx_bclp_step without a workspace argument and without any allocation:
  * no counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
@@ -199,6 +207,9 @@ int vamx_bclp_step_noalloc(vamx_plan* plan, const vamx_response* resp, const vam
 #define VAMX_RT_ATT_BALL 2           /* medium.py:907-915 */
 #define VAMX_RT_ATT_GRID 3           /* grid_sample, zeros padding, medium.py:161-185 */
 
+#define VAMX_RT_INDEX_GRID 0         /* n and grad n sampled from n_grad (medium.py:161-211) */
+#define VAMX_RT_INDEX_LUNEBURG 1     /* analytical Luneburg lens (medium.py:486-549) */
+
 typedef struct vamx_rt_geom {
   int32_t nx, ny, nz;            /* medium grid (axis 0 = x)                           */
   double xmin, ymin, zmin;       /* physical coordinates of voxel (0,0,0)              */
@@ -219,6 +230,9 @@ typedef struct vamx_rt_geom {
   double att_alpha, att_radius;  /* analytic attenuation                                 */
   const float* att_grid;         /* device [nx*ny*nz] for VAMX_RT_ATT_GRID               */
   const float* abs_grid;         /* device [nx*ny*nz] absorption at grid points (alpha_abs) */
+  int32_t index_kind;            /* VAMX_RT_INDEX_*; with VAMX_RT_INDEX_LUNEBURG, n_grad may be NULL
+                                    and n(x) = n_sur (2 - (|x|/R)^p)^(1/p) for |x| < R, n_sur outside */
+  double index_R, index_p, index_n_sur;   /* Luneburg lens: physical radius, exponent, surrounding index */
 } vamx_rt_geom;
 
 /* Replaces PyTorchRayTracingPropagator.__init__ / RayState.setupRaysParallel
@@ -239,6 +253,23 @@ int vamx_rt_deposit(vamx_plan* plan, const float* g, float* dose, void* stream);
  * max_steps (the reference logs an error for those, pyTorchRayTrace.py:566-569).
  * Synchronises the plan's stream. */
 int vamx_rt_stats(vamx_plan* plan, int64_t* steps);
+
+/* Workspace bytes of vamx_rt_bclp_step for this plan. */
+int vamx_rt_bclp_workspace_bytes(vamx_plan* plan, int64_t* bytes);
+
+/* One BCLP iteration on the ray tracer (BCLPNorm with
+ * PyTorchRayTracingPropagator, vamtoolbox/optimizer/BCLP.py:277-386, 477-534;
+ * pyTorchRayTrace.py:87-133): dose = alpha_abs W^T g (the deposit), the
+ * response, band and Lp weighting fused into the deposit's finalisation,
+ * the loss logged at loss_log[*iter_counter], and the gradient
+ * W (alpha_abs operand) fused with g <- clip(g - lr * s * gradient) in the
+ * integrate pass; ++*iter_counter.  g is the flat ray sinogram (nt, n_angles,
+ * nzr), target and dose the medium grid (nx, ny, nz).  prm->grad_op must be
+ * 0 (the forward is the exact transpose of the deposit).  No host
+ * synchronisation: the step can be captured in a HIP graph. */
+int vamx_rt_bclp_step(vamx_plan* plan, const vamx_response* resp, const vamx_bclp_params* prm,
+                      float* g, const float* target, const float* eps_arr, const float* weight_arr,
+                      float* dose, double* loss_log, int32_t* iter_counter, void* workspace, void* stream);
 
 /* Propagation matrix of the ray tracer (PyTorchRayTracingPropagator.buildPropagationMatrix,
  * pyTorchRayTrace.py:168-197 -> recordEnergyUntilExit 573-690), in two passes

@@ -62,6 +62,9 @@ PINNING = {
     "raytrace.central_difference": "exact on quadratic fields, zero faces, 2-D (check_central_difference)",
     "raytrace.luneburg_n": "analytic profile and radial gradient (check_luneburg_profile); focus on the "
                            "antipodal surface point (test_luneburg_lens_focus)",
+    "raytrace.LuneburgMedium.sample": "closed-form lens: gradient vs central differences for p = 2 and 3, "
+                                      "centre and rim values, grid values, focus on the antipodal surface point "
+                                      "(check_luneburg_analytic)",
     "raytrace.att_alpha": "cylinder / ball point tests; grid = grid_sample with zeros padding "
                           "(check_coord_vec_and_attenuation_models, check_grid_sample_parity)",
     "raytrace.setup_rays": "orthonormal ray frame, distance D, lateral coordinates, energies "
@@ -76,11 +79,14 @@ PINNING = {
                                                 "W / W^T transposition (test_deposit_is_transpose_of_integrate)",
     "raytrace.Tracer._exited / exit schedule": "closed-form exit step counts, every 10 and every 3 steps, and "
                                                "the step cap (check_exit_step_count)",
+    "raytrace_torch.TorchCPUTracer (CPU baseline only)": "equals the float64 oracle on rays off the voxel "
+                                                         "centres, integrate and deposit "
+                                                         "(test_torch_cpu_port_matches_oracle)",
     # --- fan-beam rebinning (rebin.py; geometry.py:736-944) ---
     "rebin.sampling / rebin_fan_beam": "telecentric limit, Bouguer's invariant, Fresnel factor at normal "
                                        "incidence (test_fan_beam_rebin_telecentric_limit, "
                                        "test_fan_beam_snell_and_centre_ray)",
 }
 """Every entry's check fails under a planted one-line mistake in the oracle:
-``tests/test_oracle_mutations.py`` plants 54 such mutations and asserts that
+``tests/test_oracle_mutations.py`` plants 56 such mutations and asserts that
 the named check catches each one."""

@@ -22,7 +22,8 @@ Reference: ``vamtoolbox/projector/pyTorchRayTrace.py`` and ``vamtoolbox/medium.p
               n and grad n, "zeros" padding for an attenuation grid
               (medium.py:161-211); central differences with zero boundary
               faces for grad n (224-273); analytic cylinder / ball attenuation
-              (895-915); Luneburg lens (486-549).
+              (895-915); Luneburg lens (486-549), presampled on the grid
+              (luneburg_n) or in closed form at every sample (LuneburgMedium).
 """
 
 from __future__ import annotations
@@ -119,6 +120,29 @@ def luneburg_n(cv, R=1.0, p=2.0, n_sur=1.0):
     g[..., 1] = fac * Y
     g[..., 2] = fac * Z
     return n, g, Rp
+
+
+class LuneburgMedium(GridMedium):
+    """IndexModel(type="analytical", form="luneburg_lens") (medium.py:486-549):
+    n and grad n in closed form at every sample position (no grid sampling);
+    the grid of ``cv`` serves the interaction taps and the exit box."""
+
+    def __init__(self, cv, R=1.0, p=2.0, n_sur=1.0):
+        nf, g, Rp = luneburg_n(cv, R, p, n_sur)
+        super().__init__(cv, nf, grad=g)
+        self.Rp, self.p, self.n_sur = Rp, float(p), float(n_sur)
+
+    def sample(self, x, field=None, padding="border"):
+        if field is not None:
+            return super().sample(x, field, padding)
+        r = np.sqrt(x[:, 0] ** 2 + x[:, 1] ** 2 + x[:, 2] ** 2)
+        n = np.full(r.shape, self.n_sur)
+        inside = r < self.Rp
+        n[inside] = self.n_sur * (2 - (r[inside] / self.Rp) ** self.p) ** (1 / self.p)
+        g = np.zeros(x.shape)
+        c = (self.n_sur / self.Rp) ** self.p
+        g[inside] = (-c * (n[inside] ** (1 - self.p)) * (r[inside] ** (self.p - 2)))[:, None] * x[inside]
+        return np.concatenate([n[:, None], g], axis=1)
 
 
 def att_alpha(x, kind, alpha=0.0, radius=np.inf, grid=None, medium=None):

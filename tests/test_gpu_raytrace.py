@@ -112,3 +112,158 @@ def test_two_dimensional(gpu):
     f = np.random.default_rng(2).random((n, n, 1))
     Wf = P.forward(torch.tensor(f, dtype=torch.float32, device=gpu)).cpu().numpy()
     assert rel_l2(Wf, tr.run(x0, v0, e0, f=f)) <= TOL
+
+
+MEDIA = ["ball_attenuation", "attenuation_grid", "ray_density", "luneburg_grid", "luneburg_analytic"]
+
+
+@pytest.mark.parametrize("medium", MEDIA)
+def test_media_variants(gpu, medium):
+    """The media and ray options beyond the cylinder case against the oracle:
+    homogeneous-ball attenuation (medium.py:905-915), an interpolated
+    attenuation grid with zeros padding (medium.py:161-185), ray density 1.5
+    (more rays, energy 1/density^2; pyTorchRayTrace.py:1679-1686), and the
+    Luneburg lens presampled on the grid and in closed form
+    (medium.py:486-549)."""
+    import torch
+    import oracle.raytrace as ort
+    import vamtoolbox_amd as vam
+    from vamtoolbox_amd.projector.pyTorchRayTrace import PyTorchRayTracingPropagator
+    print("medium", medium, flush=True)
+    rng = np.random.default_rng(21)
+    luneburg = medium.startswith("luneburg")
+    n, nz = (24, 24) if luneburg else (20, 8)
+    ang = np.arange(0.0, 180.0, 15.0)
+    tg = vam.geometry.TargetGeometry(target=np.ones((n, n, nz)))
+    cv = tg.constructCoordVec()
+    vox = cv[0][1] - cv[0][0]
+    am, att, rd = None, ("none", 0.0, 0.0, None), None
+    param, solver = "canonical", "rk4"
+    if luneburg:
+        kind = "interpolation" if medium == "luneburg_grid" else "analytical"
+        im = vam.medium.IndexModel(cv, type=kind, form="luneburg_lens")
+        nf, gl, R = ort.luneburg_n(cv)
+        med = ort.GridMedium(cv, nf, grad=gl) if kind == "interpolation" else ort.LuneburgMedium(cv)
+        param = "physical_path_length"
+    else:
+        X, Y, Z = np.meshgrid(*cv, indexing="ij")
+        nf = 1.5 + 0.02 * np.exp(-((X - 3 * vox) ** 2 + Y ** 2 + Z ** 2) / (2 * (6 * vox) ** 2))
+        im = vam.medium.IndexModel(cv, type="interpolation", form="freeform", n_x=nf)
+        med = ort.GridMedium(cv, nf)
+        if medium == "ball_attenuation":
+            am = vam.medium.AttenuationModel(cv, type="analytical", form="homogeneous_ball", R=0.8,
+                                             alpha_internal=0.02 / vox)
+            att = ("ball", am.params["alpha_internal"], am.params["R_physical"], None)
+        elif medium == "attenuation_grid":
+            a = rng.uniform(0.0, 0.03 / vox, (n, n, nz))
+            am = vam.medium.AttenuationModel(cv, type="interpolation", form="freeform", alpha_x=a)
+            att = ("grid", 0.0, 0.0, a)
+        else:
+            rd = 1.5
+    pg = vam.geometry.ProjectionGeometry(ang, "ray_trace", CUDA=True, index_model=im, attenuation_model=am,
+                                         ray_trace_method="eikonal", eikonal_parametrization=param,
+                                         ray_trace_ode_solver=solver, ray_trace_ray_config="parallel",
+                                         ray_density=rd)
+    P = PyTorchRayTracingPropagator(tg, pg, output_torch_tensor=True)
+    tr = ort.Tracer(med, param=param, solver=solver, att=att)
+    x0, v0, shape, e0 = ort.setup_rays(cv, ang, ray_density=1.0 if rd is None else rd)
+    assert tuple(P.sino_shape) == tuple(shape)
+    f = rng.random((n, n, nz))
+    g = rng.random(int(np.prod(shape)))
+    Wf = P.forward(torch.tensor(f, dtype=torch.float32, device=gpu)).cpu().numpy()
+    steps, still = P.stats()
+    WTg = P.backward(torch.tensor(g, dtype=torch.float32, device=gpu)).cpu().numpy()
+    ref_f, ref_steps, ref_still = tr.run(x0, v0, e0, f=f, return_steps=True)
+    ref_g = tr.run(x0, v0, e0, g=g)
+    e1, e2 = rel_l2(Wf, ref_f), rel_l2(WTg, ref_g)
+    print("rel L2 integrate %.2e deposit %.2e steps %d/%d still %d/%d" % (e1, e2, steps, ref_steps, still, ref_still))
+    assert e1 <= TOL and e2 <= TOL
+    assert steps == ref_steps and still == ref_still
+
+
+def _rt_bclp_problem():
+    """16 x 16 x 4 graded medium, 12 angles, RK4 canonical, cylinder
+    attenuation and absorption, a smooth target in [0, 1]."""
+    import oracle.raytrace as ort
+    import vamtoolbox_amd as vam
+    n, nz = 16, 4
+    ang = np.arange(0.0, 180.0, 15.0)
+    tg0 = vam.geometry.TargetGeometry(target=np.ones((n, n, nz)))
+    cv = tg0.constructCoordVec()
+    vox = cv[0][1] - cv[0][0]
+    X, Y, Z = np.meshgrid(*cv, indexing="ij")
+    nf = 1.5 + 0.02 * np.exp(-((X - 2 * vox) ** 2 + Y ** 2 + Z ** 2) / (2 * (4 * vox) ** 2))
+    T = np.clip(1.2 * np.exp(-(X ** 2 + (Y - vox) ** 2) / (2 * (4 * vox) ** 2)) - 0.1, 0, 1)
+    alpha = 0.005 / vox
+    im = vam.medium.IndexModel(cv, type="interpolation", form="freeform", n_x=nf)
+    am = vam.medium.AttenuationModel(cv, type="analytical", form="homogeneous_cylinder", alpha_internal=alpha)
+    ab = vam.medium.AbsorptionModel(cv, type="analytical", form="homogeneous_cylinder", alpha_internal=alpha)
+    pg = vam.geometry.ProjectionGeometry(ang, "ray_trace", CUDA=True, index_model=im, attenuation_model=am,
+                                         absorption_model=ab, ray_trace_method="eikonal",
+                                         eikonal_parametrization="canonical", ray_trace_ode_solver="rk4",
+                                         ray_trace_ray_config="parallel")
+    tg = vam.geometry.TargetGeometry(target=T, clip_to_circle=False)
+    tr = ort.Tracer(ort.GridMedium(cv, nf), param="canonical", solver="rk4",
+                    att=("cylinder", am.params["alpha_internal"], am.params["R_physical"], None),
+                    abs_grid=ab.alpha_at_grid())
+    x0, v0, shape, e0 = ort.setup_rays(cv, ang)
+    return tg, pg, tr, (x0, v0, shape, e0), ab.alpha_at_grid(), ang
+
+
+def test_inverse_backward_matches_oracle(gpu):
+    """The initial sinogram of the ray-tracer BCLP (pyTorchRayTrace.py:135-166):
+    f / alpha^2 (0 where alpha = 0), forward, Ram-Lak filter along t,
+    times pi / (2 n_angles)."""
+    import torch
+    import oracle.optim as oo
+    from vamtoolbox_amd.projector.pyTorchRayTrace import PyTorchRayTracingPropagator
+    tg, pg, tr, (x0, v0, shape, e0), ab, ang = _rt_bclp_problem()
+    P = PyTorchRayTracingPropagator(tg, pg, output_torch_tensor=True)
+    f = np.asarray(tg.array, dtype=np.float64)
+    g = P.inverseBackward(torch.tensor(f, dtype=torch.float32, device=gpu)).cpu().numpy()
+    with np.errstate(divide="ignore", invalid="ignore"):
+        fa = f / ab ** 2
+    fa[~np.isfinite(fa)] = 0.0
+    ref = oo.filter_sinogram(tr.run(x0, v0, e0, f=fa).reshape(shape), "ram-lak").ravel() * (np.pi / (2 * len(ang)))
+    e = rel_l2(g, ref)
+    print("inverseBackward rel L2 %.2e" % e)
+    assert e <= TOL
+
+
+def test_raytrace_bclp_fused_matches_oracle(gpu):
+    """20 iterations of the fused ray-tracer BCLP (vamx_rt_bclp_step: deposit
+    with the response / band / Lp epilogue, integrate fused with the update)
+    against the oracle's BCLP loop on the float64 tracer, from the
+    inverseBackward initial sinogram; and the graph-replayed loop equals the
+    eager one."""
+    import oracle.optim as oo
+    import vamtoolbox_amd as vam
+    tg, pg, tr, (x0, v0, shape, e0), ab, ang = _rt_bclp_problem()
+    T = np.asarray(tg.array, dtype=np.float64)
+    fwd = lambda f: tr.run(x0, v0, e0, f=f)      # noqa: E731
+    bwd = lambda g: tr.run(x0, v0, e0, g=g)      # noqa: E731
+    ri = oo.Response("interpolation", B=20, M=0.6)
+    f = ri.map_inv(T)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        f = f / ab ** 2
+    f[~np.isfinite(f)] = 0.0
+    g0 = np.clip(oo.filter_sinogram(fwd(f).reshape(shape), "ram-lak").ravel() * (np.pi / (2 * len(ang))), 0, None)
+    lr, n_iter = 1e-4, 20
+    ro = oo.Response("analytical", B=20, M=0.6)
+    g_ref, dose_ref, _, loss_ref = oo.bclp(T, fwd, bwd, ro, n_iter, g0, eps=0.05, lr=lr)
+    assert np.all(np.diff(loss_ref) < 0), "test problem must descend"
+    rm = vam.response.ResponseModel(type="analytical", B=20, M=0.6)
+    res = {}
+    for graph in (False, True):
+        opts = vam.optimize.Options(method="BCLP", n_iter=n_iter, response_model=rm, eps=0.05, p=2, q=1,
+                                    learning_rate=lr, g0=g0, fused=True, hip_graph=graph)
+        g, dose, mapped, logs = vam.optimize.optimize(tg, pg, opts, output="components")
+        res[graph] = (np.asarray(g), np.asarray(dose), np.asarray(logs.loss))
+    g, dose, loss = res[True]
+    e_g, e_d = rel_l2(g.ravel(), g_ref.ravel()), rel_l2(dose.ravel(), dose_ref.ravel())
+    e_l = np.max(np.abs(loss - loss_ref) / np.abs(loss_ref))
+    print("ray-trace BCLP x%d: rel L2 g %.2e dose %.2e loss %.2e (%.4g -> %.4g)" % (n_iter, e_g, e_d, e_l,
+                                                                                   loss_ref[0], loss_ref[-1]))
+    assert e_g <= 1e-4 and e_d <= 1e-4 and e_l <= 1e-4
+    np.testing.assert_array_equal(res[False][0], res[True][0])
+    np.testing.assert_array_equal(res[False][2], res[True][2])

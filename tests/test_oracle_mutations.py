@@ -89,6 +89,9 @@ MUTATIONS = [
     ("oracle.raytrace", "central_difference", "(n[:, :, 2:] - n[:, :, :-2])", "(n[:, :, 2:] - n[:, :, 1:-1])", RAYS,
      "check_central_difference"),
     ("oracle.raytrace", "luneburg_n", "fac = -c *", "fac = c *", RAYS, "check_luneburg_profile"),
+    ("oracle.raytrace", "LuneburgMedium.sample", "(n[inside] ** (1 - self.p))", "(n[inside] ** (self.p - 1))", RAYS,
+     "check_luneburg_analytic"),
+    ("oracle.raytrace", "LuneburgMedium.sample", "** (1 / self.p)", "** (1 / 2)", RAYS, "check_luneburg_analytic"),
     ("oracle.raytrace", "att_alpha", "np.linalg.norm(x, axis=1) < radius", "np.linalg.norm(x[:, :2], axis=1) < radius",
      RAYS, "check_coord_vec_and_attenuation_models"),
     # --- ray geometry and stepping (pyTorchRayTrace.py) ---

@@ -323,3 +323,46 @@ def check_coord_vec_and_attenuation_models():
 
 def test_coord_vec_and_attenuation_models():
     check_coord_vec_and_attenuation_models()
+
+
+def check_luneburg_analytic():
+    """The closed-form lens: grad n equals central differences of n (p = 2 and
+    p = 3, n_sur = 1 and 1.3), n(0) = 2^(1/p) n_sur, n -> n_sur at r = R, the
+    grid values equal the presampled profile, and parallel rays focus on the
+    antipodal surface point."""
+    cv = rt.coord_vec(32, 32, 32)
+    rng = np.random.default_rng(3)
+    for p, ns in ((2.0, 1.0), (3.0, 1.3)):
+        med = rt.LuneburgMedium(cv, R=0.9, p=p, n_sur=ns)
+        R = med.Rp
+        x = rng.uniform(-0.6, 0.6, (50, 3)) * R
+        f = med.sample(x)
+        h = 1e-6 * R
+        for k in range(3):
+            e = np.zeros(3)
+            e[k] = h
+            fd = (med.sample(x + e)[:, 0] - med.sample(x - e)[:, 0]) / (2 * h)
+            np.testing.assert_allclose(f[:, 1 + k], fd, rtol=1e-6, atol=1e-6 * ns / R)
+        assert abs(med.sample(np.zeros((1, 3)))[0, 0] - 2 ** (1 / p) * ns) < 1e-14
+        edge = med.sample(np.array([[R * (1 - 1e-9), 0.0, 0.0], [R * 1.01, 0.0, 0.0]]))
+        assert abs(edge[0, 0] - ns) < 1e-6 and edge[1, 0] == ns and np.all(edge[1, 1:] == 0)
+        X, Y, Z = np.meshgrid(*cv, indexing="ij")
+        pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+        np.testing.assert_allclose(med.sample(pts)[:, 0], med.n.ravel(), rtol=1e-14)
+    med = rt.LuneburgMedium(rt.coord_vec(64, 64, 64))
+    tr = rt.Tracer(med, param="physical_path_length", solver="rk4", step=med.voxel[0] / 4)
+    R = med.Rp
+    b = np.array([0.1, 0.2, 0.3]) * R
+    x = np.stack([np.full(3, 1.2 * R), b, np.zeros(3)], axis=1)
+    v = np.tile([-1.0, 0.0, 0.0], (3, 1))
+    for _ in range(4000):
+        xn, vn = tr.advance(x, v)
+        if np.all(xn[:, 1] <= 0) or xn[0, 0] < -1.3 * R:
+            break
+        x, v = xn, vn
+    assert np.all(np.abs(x[:, 1]) < 0.02 * R)
+    assert np.all(np.abs(x[:, 0] + R) < 0.05 * R)
+
+
+def test_luneburg_analytic():
+    check_luneburg_analytic()

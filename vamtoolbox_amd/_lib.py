@@ -101,6 +101,10 @@ class RtGeom(ctypes.Structure):
         ("att_radius", c_double),
         ("att_grid", c_void_p),
         ("abs_grid", c_void_p),
+        ("index_kind", c_int32),
+        ("index_R", c_double),
+        ("index_p", c_double),
+        ("index_n_sur", c_double),
     ]
 
 
@@ -112,6 +116,7 @@ RESP_GLF, RESP_LINEAR, RESP_IDENTITY, RESP_TABLE = 0, 1, 2, 3
 RT_SYMPLECTIC_EULER, RT_FORWARD_EULER, RT_LEAPFROG, RT_RK4 = 0, 1, 2, 3
 RT_CANONICAL, RT_PHYSICAL, RT_OPTICAL = 0, 1, 2
 RT_ATT_NONE, RT_ATT_CYLINDER, RT_ATT_BALL, RT_ATT_GRID = 0, 1, 2, 3
+RT_INDEX_GRID, RT_INDEX_LUNEBURG = 0, 1
 
 # every symbol declared in include/vamx.h with (restype, argtypes)
 _PLAN_P = c_void_p
@@ -130,6 +135,7 @@ SIGNATURES = {
     "vamx_bclp_eval": (c_int32, [_PLAN_P, ctypes.POINTER(Response), ctypes.POINTER(BclpParams),
                                  c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_void_p, c_void_p]),
+    "vamx_bclp_update": (c_int32, [_PLAN_P, ctypes.POINTER(BclpParams), c_void_p, c_void_p, c_void_p]),
     "vamx_bclp_step_noalloc": (c_int32, [_PLAN_P, ctypes.POINTER(Response), ctypes.POINTER(BclpParams),
                                          c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                          c_void_p, c_void_p, c_void_p]),
@@ -137,6 +143,10 @@ SIGNATURES = {
     "vamx_rt_integrate": (c_int32, [_PLAN_P, c_void_p, c_void_p, c_void_p]),
     "vamx_rt_deposit": (c_int32, [_PLAN_P, c_void_p, c_void_p, c_void_p]),
     "vamx_rt_stats": (c_int32, [_PLAN_P, ctypes.POINTER(ctypes.c_int64)]),
+    "vamx_rt_bclp_workspace_bytes": (c_int32, [_PLAN_P, ctypes.POINTER(ctypes.c_int64)]),
+    "vamx_rt_bclp_step": (c_int32, [_PLAN_P, ctypes.POINTER(Response), ctypes.POINTER(BclpParams),
+                                    c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_void_p, c_void_p]),
     "vamx_rt_record_counts": (c_int32, [_PLAN_P, c_void_p, c_void_p]),
     "vamx_rt_record": (c_int32, [_PLAN_P, c_void_p, c_void_p, c_void_p, c_void_p]),
     "vamx_sino_resample": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p,

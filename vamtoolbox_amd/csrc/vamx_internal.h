@@ -53,12 +53,15 @@ struct RtDev {
   double inv_dx, inv_dy, inv_dz;
   double span_x, span_y, span_z;
   double xmax, ymax, zmax;
+  double ixmax, iymax, izmax;  // largest index coordinate: size - 1
   int nt, na, nzr;
   double incl;               // radians
   double step;
   int max_steps, exit_every, solver, param;
   float energy0;
   const float4* field;       // (n, gx, gy, gz)
+  int index_kind;            // VAMX_RT_INDEX_*
+  double lune_R, lune_p, lune_nsur, lune_c;   // analytical Luneburg lens; lune_c = (n_sur / R)^p
   int att_kind;
   double att_alpha, att_radius;
   const float* att_grid;

@@ -88,10 +88,8 @@ class IndexModel(MediumModel):
             n = self._n_homo(x)
             grad = np.zeros_like(x)
         elif form == "luneburg_lens":
-            if type == "analytical":
-                raise NotImplementedError(
-                    "the ray-trace kernel samples the index on the grid; use type='interpolation' for the "
-                    "Luneburg lens")
+            # analytical: the kernel evaluates the closed form at every sample;
+            # interpolation: the closed form presampled on the grid
             n = self._n_lune(x)
             grad = self._grad_n_lune(x)
         elif form == "freeform":
@@ -137,11 +135,29 @@ class IndexModel(MediumModel):
         return grad
 
     def n(self, x):
-        """n at positions x (num, 3) by trilinear grid sampling (border padding)."""
-        return _grid_sample(self, self.presampled_scalar_field_3D[..., None], np.asarray(x), "border")[:, 0]
+        """n at positions x (num, 3): the closed form for the analytical
+        models, trilinear grid sampling (border padding) otherwise."""
+        x = np.asarray(x, dtype=np.float64)
+        if self.type == "analytical":
+            return self._n_homo(x) if self.form == "homogeneous" else self._n_lune(x)
+        return _grid_sample(self, self.presampled_scalar_field_3D[..., None], x, "border")[:, 0]
 
     def grad_n(self, x):
-        return _grid_sample(self, self.presampled_vector_field_4D, np.asarray(x), "border")
+        x = np.asarray(x, dtype=np.float64)
+        if self.type == "analytical":
+            return np.zeros_like(x) if self.form == "homogeneous" else self._grad_n_lune(x)
+        return _grid_sample(self, self.presampled_vector_field_4D, x, "border")
+
+    def kernel_index_spec(self):
+        """(kind, R, p, n_sur) of the ray-trace kernel's index model: the
+        analytical Luneburg lens is evaluated in closed form; every other model
+        (the homogeneous one included, whose grid samples are exact) is
+        sampled from the float4 grid."""
+        from . import _lib
+        if self.type == "analytical" and self.form == "luneburg_lens":
+            return (_lib.RT_INDEX_LUNEBURG, float(self.params["R_physical"]), float(self.params["p"]),
+                    float(self.params["n_sur"]))
+        return _lib.RT_INDEX_GRID, 0.0, 0.0, 0.0
 
     def device_field(self, device):
         """float32 (nx, ny, nz, 4) tensor (n, dn/dx, dn/dy, dn/dz) on ``device``."""

@@ -9,14 +9,15 @@ loop) and outputs ("packaged", "full", "components").
 Three execution modes for the gradient-descent loop, all on GPU-resident
 tensors:
 
-* fused (default for the parallel-beam projector): one ``vamx_bclp_step``
-  per iteration -- the back projection fused with response, band and Lp
-  weighting and the loss partials, a fixed-order loss reduction, and the
-  forward projection fused with the sinogram update -- optionally replayed
-  from a captured HIP graph; the loss log stays on the device.
+* fused (default): one ``vamx_bclp_step`` (parallel beam) or
+  ``vamx_rt_bclp_step`` (ray tracer) per iteration -- the back projection
+  (deposit) fused with response, band and Lp weighting and the loss
+  partials, a fixed-order loss reduction, and the forward projection
+  (integrate) fused with the sinogram update -- optionally replayed from a
+  captured HIP graph; the loss log stays on the device.
 * unfused: projector calls plus torch element-wise operations with the
-  reference's host-side branches (used for the ray tracer, ``bit_depth``,
-  and as the plain GPU loop).
+  reference's host-side branches (used for ``bit_depth``, and as the plain
+  GPU loop).
 * ``gradient="exact"``: the gradient is taken through the exact transpose of
   the back projector (the true gradient of the loss) instead of the forward
   projector the reference uses in its place (BCLP.py:372).
@@ -100,7 +101,9 @@ class BCLPNorm:
         if self.gradient == "exact" and not self.parallel:
             # the ray tracer's forward is already the exact transpose of its backward
             self.gradient = "reference"
-        self.fused = bool(_opt(options, "fused", True)) and self.parallel and self.bit_depth is None
+        # fused iteration on the GPU: vamx_bclp_step (parallel beam) or
+        # vamx_rt_bclp_step (ray tracer)
+        self.fused = bool(_opt(options, "fused", True)) and self.bit_depth is None
         self.hip_graph = bool(_opt(options, "hip_graph", True))
         self.log_metrics = bool(_opt(options, "log_metrics", not self.fused))
         dev = self.device
@@ -229,25 +232,36 @@ class BCLPNorm:
 
     def _gradient_descent_fused(self):
         lib = _lib.load()
-        plan = self.P.plan
         dev = self.device
         n_iter = self.logs.options.n_iter
-        g = self.g0.reshape(plan.sino_shape).contiguous().clone()
-        dose = torch.empty(plan.vol_shape, device=dev, dtype=torch.float32)
+        if self.parallel:
+            plan = self.P.plan
+            handle = plan.handle
+            g = self.g0.reshape(plan.sino_shape).contiguous().clone()
+            dose = torch.empty(plan.vol_shape, device=dev, dtype=torch.float32)
+            ws_fn, step_name = lib.vamx_bclp_workspace_bytes, "vamx_bclp_step"
+        else:
+            handle = self.P.handle
+            g = self.g0.reshape(-1).contiguous().clone()
+            dose = torch.empty(self.P.vol_shape, device=dev, dtype=torch.float32)
+            ws_fn, step_name = lib.vamx_rt_bclp_workspace_bytes, "vamx_rt_bclp_step"
+        step_fn = getattr(lib, step_name)
+        if self.T.numel() != dose.numel():
+            raise ValueError("the target and the dose grid differ in size")
+        T = self.T.contiguous()
         loss_log = torch.full((n_iter + 1,), float("nan"), device=dev, dtype=torch.float64)
         counter = torch.zeros(1, device=dev, dtype=torch.int32)
         nbytes = ctypes.c_int64()
-        _lib.check(lib.vamx_bclp_workspace_bytes(plan.handle, ctypes.byref(nbytes)), "vamx_bclp_workspace_bytes")
+        _lib.check(ws_fn(handle, ctypes.byref(nbytes)), "workspace bytes")
         ws = torch.empty(int(nbytes.value), device=dev, dtype=torch.uint8)
         resp, prm = self._vamx_args()
         eps_p = 0 if self.eps_t is None else self.eps_t.data_ptr()
         w_p = 0 if self.w_t is None else self.w_t.data_ptr()
 
         def step():
-            _lib.check(lib.vamx_bclp_step(plan.handle, ctypes.byref(resp), ctypes.byref(prm), g.data_ptr(),
-                                          self.T.data_ptr(), eps_p, w_p, dose.data_ptr(), loss_log.data_ptr(),
-                                          counter.data_ptr(), ws.data_ptr(), _lib.stream_ptr(dev)),
-                       "vamx_bclp_step")
+            _lib.check(step_fn(handle, ctypes.byref(resp), ctypes.byref(prm), g.data_ptr(), T.data_ptr(), eps_p, w_p,
+                               dose.data_ptr(), loss_log.data_ptr(), counter.data_ptr(), ws.data_ptr(),
+                               _lib.stream_ptr(dev)), step_name)
 
         per_iter_host = self.exit_param is not None or self.log_metrics or self.verbose in ("time", "plot")
         use_graph = self.hip_graph and not per_iter_host and n_iter > 1

@@ -69,7 +69,8 @@ class PyTorchRayTracingPropagator:
         self.sino_shape = (nt, int(np.size(proj_geo.angles)), nzr)
         self.num_rays = int(np.prod(self.sino_shape))
         self.ray_energy_0 = rd ** (-1.0) if nz == 1 else rd ** (-2.0)
-        self._field = im.device_field(self.device)
+        ikind, iR, ip, insur = im.kernel_index_spec()
+        self._field = None if ikind != _lib.RT_INDEX_GRID else im.device_field(self.device)
         att = self.attenuation_model
         if att is None:
             akind, aalpha, arad, agrid = _lib.RT_ATT_NONE, 0.0, 0.0, None
@@ -91,7 +92,8 @@ class PyTorchRayTracingPropagator:
             angles_deg=self._angles.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
             incl_deg=float(incl), step=self.tracing_step_size, max_steps=self.max_num_step,
             exit_every=self.num_step_per_exit_check, solver=_SOLVERS[solver], param=_PARAMS[param],
-            ray_energy0=float(self.ray_energy_0), n_grad=self._field.data_ptr(), att_kind=akind,
+            ray_energy0=float(self.ray_energy_0), n_grad=0 if self._field is None else self._field.data_ptr(),
+            index_kind=ikind, index_R=iR, index_p=ip, index_n_sur=insur, att_kind=akind,
             att_alpha=aalpha, att_radius=arad, att_grid=0 if agrid is None else agrid.data_ptr(),
             abs_grid=0 if self._abs is None else self._abs.data_ptr())
         h = ctypes.c_void_p()
