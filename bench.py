@@ -188,6 +188,38 @@ def cpu_baseline(target_np, g_np, angles, k_slices=2):
                       "iteration" % (k_slices, NZ, NA, dt, NZ // k_slices)}
 
 
+def _cpu_slice_iteration(job):
+    """One z-slice of the BCLP iteration on the CPU port (a worker of the
+    all-core baseline): back projection, epilogue, forward projection."""
+    import oracle.d1 as d1
+    import oracle.optim as oo
+    target_sl, g_sl, angles = job
+    resp = oo.Response("analytical", B=20, M=0.6)
+    dose = d1.backward(g_sl.astype(np.float64), angles, det_offset=OFFSET, mu=MU)
+    eT = resp.map(dose) - target_sl
+    eb = np.abs(eT) - 0.05
+    operand = (eb > 0) * np.abs(eb) * np.sign(eT) * resp.dmap(dose)
+    d1.forward(operand, angles, det_offset=OFFSET, mu=MU)
+    return 0
+
+
+def cpu_baseline_all_cores(pool, workers, target_np, g_np, angles):
+    """The z-slab variant: the iteration is independent per z-slice at zero
+    inclination, so `workers` processes each run one slice of the port at
+    once; the wall time of those slices is scaled to all NZ slices."""
+    jobs = [(target_np[:, :, z:z + 1], g_np[:, :, z:z + 1], angles) for z in range(workers)]
+    pool.map(_cpu_slice_iteration, jobs[:1])          # warm the workers' imports
+    t0 = time.perf_counter()
+    pool.map(_cpu_slice_iteration, jobs, chunksize=1)
+    dt = time.perf_counter() - t0
+    per_iter = dt * (NZ / workers)
+    return {"value": 1.0 / per_iter, "unit": "iters/s", "cores": workers, "kind": "port",
+            "sample": "float64 NumPy port of Projector3DParallelPython back+forward projection and the BCLP "
+                      "epilogue, %d worker processes (one z-slice each, OMP_NUM_THREADS=1) over %d of %d z-slices "
+                      "(all %d angles), %.1f s wall, scaled x%.0f to one iteration" % (
+                          workers, workers, NZ, NA, dt, NZ / workers)}
+
+
 # ---------------------------------------------------------------------------
 def main():
     ap = argparse.ArgumentParser()
@@ -213,7 +245,27 @@ def main():
               "and --gpus N with N > 1 is not supported",
               file=sys.stderr)
         sys.exit(2)
-    {"bclp": run_bclp, "osmo": run_osmo, "batched": run_batched, "raytrace": run_raytrace}[args.workload](args)
+    args.cpu_pool = None
+    if args.workload == "bclp" and not args.no_cpu and not args.kernels_only:
+        # the all-core CPU baseline's worker processes start before this
+        # process touches the GPU (they run NumPy only)
+        import multiprocessing as mp
+        args.cpu_workers = max(1, min(16, os.cpu_count() or 1))
+        omp = os.environ.get("OMP_NUM_THREADS")
+        os.environ["OMP_NUM_THREADS"] = "1"
+        try:
+            args.cpu_pool = mp.get_context("spawn").Pool(args.cpu_workers)
+        finally:
+            if omp is None:
+                del os.environ["OMP_NUM_THREADS"]
+            else:
+                os.environ["OMP_NUM_THREADS"] = omp
+    try:
+        {"bclp": run_bclp, "osmo": run_osmo, "batched": run_batched, "raytrace": run_raytrace}[args.workload](args)
+    finally:
+        if args.cpu_pool is not None:
+            args.cpu_pool.terminate()
+            args.cpu_pool.join()
 
 
 def run_bclp(args):
@@ -366,23 +418,42 @@ def run_bclp(args):
     bp_bytes = 4.0 * (NA * V + V)
     log("FP %.3f ms (%.0f GB/s algorithmic)  BP %.3f ms (%.0f GB/s algorithmic)" % (
         fp_ms, fp_bytes / fp_ms / 1e6, bp_ms, bp_bytes / bp_ms / 1e6))
-    # the workload's 360 angles at 1 deg form 180 opposite pairs and 2 x the
-    # 10-voxel offset is an integer, so the paired forward kernel runs unless
-    # the tuning switches pairing off
-    fp_name = "par_fp_pair_kernel" if tuning.get("fp_pair", 0) else "par_fp_wave_kernel"
-    dom_name, dom_ms, dom_bytes = (fp_name, fp_ms, fp_bytes) if fp_ms >= bp_ms else \
-        ("par_bp_wave_kernel", bp_ms, bp_bytes)
-    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-    traffic = None
-    tfile = os.path.join(HERE, "profiles", "traffic.txt")
-    if os.path.exists(tfile):
-        for line in open(tfile):
-            parts = line.split()
-            if len(parts) >= 2 and parts[0] == dom_name:
-                try:
-                    traffic = float(parts[1])
-                except ValueError:
-                    pass
+    # the two kernels the timed iteration replays, each timed alone on the
+    # stream it runs on: kernel A (back projection + epilogue, and the
+    # one-block loss reduction) through vamx_bclp_eval, kernel C (forward
+    # projection of the operand fused with the update, and the 16 us slice
+    # min/max pre-pass of the warp clip) through vamx_bclp_update
+    l_out = torch.zeros(1, device=dev, dtype=torch.float64)
+    g_t = g.clone()
+    st = _lib.stream_ptr(dev)
+    ws_t = torch.empty_like(ws)
+    eval_fn = lambda: _lib.check(lib.vamx_bclp_eval(plan.handle, ctypes.byref(resp), ctypes.byref(prm),  # noqa
+                                                    g_t.data_ptr(), bc.T.data_ptr(), 0, 0, vol_out.data_ptr(), 0,
+                                                    l_out.data_ptr(), ws_t.data_ptr(), st), "vamx_bclp_eval")
+    upd_fn = lambda: _lib.check(lib.vamx_bclp_update(plan.handle, ctypes.byref(prm), g_t.data_ptr(),  # noqa
+                                                     ws_t.data_ptr(), st), "vamx_bclp_update")
+    epi_ms = time_kernel(eval_fn)
+    eval_fn()
+    upd_ms = time_kernel(upd_fn)
+    epi_alg = 4.0 * (NA * V + 4 * V)        # sinogram samples + target read, dose and operand written
+    upd_alg = 4.0 * (NA * V + 3 * S)        # operand samples + g read and written + sinogram output
+    prof = _profile_table()
+
+    def roof(name, ms, alg):
+        tr = prof.get(name, {})
+        traffic = tr.get("bytes")
+        achieved = traffic / (ms * 1e-3) / 1e9 if traffic else None
+        return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic, "kernel": name,
+                "kernel_ms": ms, "algorithmic_bytes": alg, "algorithmic_gbs": alg / (ms * 1e-3) / 1e9,
+                "td_busy": tr.get("td_busy"), "ta_busy": tr.get("ta_busy"),
+                "limiter": "texture data path (TD busy %.0f %% of the kernel)" % (100 * tr["td_busy"])
+                if tr.get("td_busy") else None}
+
+    roof_c = roof("bclp_fp_update_pair", upd_ms, upd_alg)
+    roof_a = roof("bclp_bp_epilogue_wave", epi_ms, epi_alg)
+    log("kernel C (FP + update) %.3f ms: %s GB/s measured traffic; kernel A (BP + epilogue) %.3f ms" % (
+        upd_ms, "%.0f" % roof_c["achieved"] if roof_c["achieved"] else "n/a", epi_ms))
 
     # ---- plain PyTorch loop (no custom kernels) -------------------------------
     torch_ms = None
@@ -416,10 +487,16 @@ def run_bclp(args):
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (NumPy port, bounded sample)")
-        cpu = cpu_baseline(bc.T.cpu().numpy().astype(np.float64), g.cpu().numpy(), angles)
-        cpu["cores"] = 1
+        tnp, gnp = bc.T.cpu().numpy().astype(np.float64), g.cpu().numpy()
+        one = cpu_baseline(tnp, gnp, angles)
+        log("CPU, one core: %.3e iters/s" % one["value"])
+        if args.cpu_pool is not None:
+            cpu = cpu_baseline_all_cores(args.cpu_pool, args.cpu_workers, tnp, gnp, angles)
+            cpu["one_core"] = {"value": one["value"], "sample": one["sample"]}
+        else:
+            cpu = one
         cpu["host"] = "%d host cores visible (%s)" % (os.cpu_count() or 0, _cpu_model())
-        log("CPU: %.3e iters/s" % cpu["value"])
+        log("CPU: %.3e iters/s on %d cores" % (cpu["value"], cpu["cores"]))
 
     out = {
         "metric": "BCLP iters/s at 256³×360 angles (rel err ≤1e-4); fwd/back HBM GB/s vs peak",
@@ -438,17 +515,18 @@ def run_bclp(args):
                                "sigmoid response (M=0.6, B=20), eps=0.05, p=2, q=1, absorption 0.01/voxel, "
                                "detector offset 10 voxels, fused HIP-graph iteration",
                    "n": N, "nz": NZ, "n_angles": NA},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom_name,
-                     "kernel_ms": dom_ms, "algorithmic_bytes": dom_bytes,
-                     # the HBM rate of the bytes the kernel actually moves (measured
-                     # traffic / event time): the algorithmic rate exceeds the HBM peak
-                     # because L2 and the opposite-ray pairing serve repeated tap reads
-                     "traffic_gbs": traffic / (dom_ms * 1e-3) / 1e9 if traffic else None,
-                     "traffic_frac": traffic / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic else None},
+        # the dominant kernel of the timed iteration (kernel C); achieved =
+        # its counter-measured HBM/fabric bytes per launch (profiles/) over
+        # its event-timed duration; the algorithmic rate (one 4-byte read per
+        # interpolated sample) is a separate field: L2 and the opposite-ray
+        # pairing serve the repeated tap reads, so it exceeds the HBM peak
+        "roofline": roof_c,
+        "roofline_bp": roof_a,
         "cpu_baseline": cpu,
         "forward_ms": fp_ms,
         "backward_ms": bp_ms,
+        "bp_epilogue_ms": epi_ms,
+        "fp_update_ms": upd_ms,
         "forward_gbs": fp_bytes / fp_ms / 1e6,
         "backward_gbs": bp_bytes / bp_ms / 1e6,
         "unfused_ms_per_iter": unf_ms,
@@ -462,6 +540,26 @@ def run_bclp(args):
         "tuning": tuning,
     }
     print(json.dumps(out), flush=True)
+
+
+def _profile_table():
+    """profiles/traffic.txt: kernel, HBM/fabric bytes per launch (calibrated
+    FETCH_SIZE + WRITE_SIZE), TA and TD busy fractions, from the committed
+    counter passes."""
+    out = {}
+    tfile = os.path.join(HERE, "profiles", "traffic.txt")
+    if os.path.exists(tfile):
+        for line in open(tfile):
+            parts = line.split()
+            if len(parts) >= 2 and not line.startswith("#"):
+                try:
+                    e = {"bytes": float(parts[1])}
+                    if len(parts) >= 4:
+                        e["ta_busy"], e["td_busy"] = float(parts[2]), float(parts[3])
+                    out[parts[0]] = e
+                except ValueError:
+                    pass
+    return out
 
 
 def _line(metric, value, unit, ms, args, config, roofline, cpu, **extra):
@@ -686,13 +784,16 @@ def run_raytrace(args):
     in_box = float(steps_f[0])
     log("ray trace: integrate %.1f ms, deposit %.1f ms, %.3e ray-steps per trace (%d unfinished)" % (
         fwd_ms, bwd_ms, in_box, steps_f[1]))
-    # BCLP on the ray tracer (unfused loop: trace-deposit, response, trace-integrate)
+    # BCLP on the ray tracer: the fused iteration (vamx_rt_bclp_step: deposit
+    # with the response / band / Lp epilogue, integrate fused with the
+    # update) replayed from a HIP graph, and the unfused loop (trace-deposit,
+    # torch element-wise response, trace-integrate) for comparison
     rm = vam.response.ResponseModel(type="analytical", B=20, M=0.6)
     g0 = torch.clamp(P.forward(f), min=0)
     d0 = P.backward(g0).reshape(T.shape)
     g0 = (g0 * (0.7 / float(d0[T > 0.5].mean()))).reshape(-1).contiguous()
     opts = vam.optimize.Options(method="BCLP", n_iter=20, response_model=rm, eps=0.05, p=2, q=1,
-                                learning_rate=0.01, g0=g0, log_metrics=False)
+                                learning_rate=0.01, g0=g0, log_metrics=False, fused=False)
     bc = BCLPNorm(tg, pg, opts)
     gi = bc.g0.reshape(-1).clone()
     # step size: the first step moves g by 5 % of its mean magnitude
@@ -700,7 +801,7 @@ def run_raytrace(args):
     G0 = bc._grad(op0, float(loss0.item()))
     lr = 0.05 * float(gi.abs().mean()) / max(float(G0.abs().mean()), 1e-30)
     times, losses = [], []
-    for it in range(20):
+    for it in range(6):
         torch.cuda.synchronize()
         ta = time.perf_counter()
         d_, m_, eT, eb, vmask, loss, operand = bc._eval_state(gi)
@@ -710,8 +811,47 @@ def run_raytrace(args):
         torch.cuda.synchronize()
         times.append(time.perf_counter() - ta)
         losses.append(L)
-    it_ms = 1e3 * float(np.mean(times[2:]))
-    log("ray-trace BCLP: %.1f ms per iteration; loss %.4g -> %.4g" % (it_ms, losses[0], losses[-1]))
+    unf_ms = 1e3 * float(np.mean(times[2:]))
+    log("ray-trace BCLP, unfused: %.1f ms per iteration" % unf_ms)
+    lib = _lib.load()
+    resp, prm = bc._vamx_args()
+    prm.learning_rate = lr
+    gf = bc.g0.reshape(-1).contiguous().clone()
+    Tt = bc.T.contiguous()
+    dose_f = torch.empty(P.vol_shape, device=dev)
+    n_it = args.steps + args.warmup + 1
+    loss_log = torch.zeros(n_it + 1, device=dev, dtype=torch.float64)
+    counter = torch.zeros(1, device=dev, dtype=torch.int32)
+    nb = ctypes.c_int64()
+    _lib.check(lib.vamx_rt_bclp_workspace_bytes(P.handle, ctypes.byref(nb)), "workspace")
+    ws = torch.empty(int(nb.value), device=dev, dtype=torch.uint8)
+
+    def rt_step():
+        _lib.check(lib.vamx_rt_bclp_step(P.handle, ctypes.byref(resp), ctypes.byref(prm), gf.data_ptr(),
+                                         Tt.data_ptr(), 0, 0, dose_f.data_ptr(), loss_log.data_ptr(),
+                                         counter.data_ptr(), ws.data_ptr(), _lib.stream_ptr(dev)),
+                   "vamx_rt_bclp_step")
+
+    rt_step()
+    graph = torch.cuda.CUDAGraph()
+    sg = torch.cuda.Stream(device=dev)
+    sg.wait_stream(torch.cuda.current_stream(dev))
+    with torch.cuda.stream(sg):
+        with torch.cuda.graph(graph, stream=sg):
+            rt_step()
+    torch.cuda.current_stream(dev).wait_stream(sg)
+    for _ in range(max(args.warmup - 1, 0)):
+        graph.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    steps_t = max(args.steps, 1)
+    for _ in range(steps_t):
+        graph.replay()
+    torch.cuda.synchronize()
+    it_ms = (time.perf_counter() - t0) / steps_t * 1e3
+    fl = loss_log[:steps_t + args.warmup].cpu().numpy()
+    log("ray-trace BCLP, fused (graph): %.1f ms per iteration; loss %.4g -> %.4g" % (it_ms, fl[0], fl[-1]))
+    losses = [float(fl[0]), float(fl[-1])]
     bytes_per_step = 72.0
     achieved = in_box * bytes_per_step / (fwd_ms * 1e-3) / 1e9
     cpu = None
@@ -720,32 +860,43 @@ def run_raytrace(args):
     _line("ray-trace BCLP iters/s at 192^3 x 360 angles (RK4)", 1e3 / it_ms, "iters/s", it_ms, args,
           {"workload": "curved-ray tracer, 192^3 medium n = 1.5 + 6 Gaussian bumps (+-0.02, sigma 10-30 voxels), "
                        "attenuation and absorption 0.005/voxel in the cylinder, 360 angles x 192 x 192 rays, RK4, "
-                       "step 0.5 voxel; BCLP iteration = deposit + response + integrate",
+                       "step 0.5 voxel; BCLP iteration = deposit fused with the response / band / Lp epilogue, "
+                       "integrate fused with the update, replayed from a HIP graph",
            "n": n, "n_angles": na, "rays": int(192 * 192 * na)},
           {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "rt_trace_kernel<false>",
            "kernel_ms": fwd_ms, "algorithmic_bytes": in_box * bytes_per_step},
           cpu, integrate_ms=fwd_ms, deposit_ms=bwd_ms, ray_steps=in_box, ray_steps_per_s=in_box / (fwd_ms * 1e-3),
-          unfinished_rays=int(steps_f[1]), loss_first=losses[0], loss_last=losses[-1], learning_rate=lr)
+          unfinished_rays=int(steps_f[1]), loss_first=losses[0], loss_last=losses[-1], learning_rate=lr,
+          unfused_ms_per_iter=unf_ms, speedup_vs_unfused=unf_ms / it_ms)
 
 
-def _raytrace_cpu_baseline(tg, pg, cv, nf, am, ab, na, k_angles=1):
-    """float64 NumPy port of the reference tracer (RK4 here) on k_angles of the
-    360 angles, scaled to one integrate pass."""
+def _raytrace_cpu_baseline(tg, pg, cv, nf, am, ab, na, k_angles=2):
+    """The reference's CPU path (pyTorchRayTrace.py:378-571 on CPU torch:
+    symplectic Euler, grid_sample, sparse-COO deposit) restated in
+    oracle/raytrace_torch.py, on all host cores of the GPU's share, for
+    k_angles of the angles, scaled to one integrate pass."""
+    import torch
     import oracle.raytrace as ort
-    med = ort.GridMedium(cv, nf)
-    tr = ort.Tracer(med, param="canonical", solver="rk4",
-                    att=("cylinder", am.params["alpha_internal"], am.params["R_physical"], None),
-                    abs_grid=ab.alpha_at_grid())
-    x0, v0, shape, e0 = ort.setup_rays(cv, pg.angles[:k_angles])
-    f = np.asarray(tg.array, dtype=np.float64)
-    t0 = time.perf_counter()
-    tr.run(x0, v0, e0, f=f)
-    dt = time.perf_counter() - t0
+    import oracle.raytrace_torch as rtt
+    threads = max(1, min(16, os.cpu_count() or 1))
+    prev = torch.get_num_threads()
+    torch.set_num_threads(threads)
+    try:
+        tr = rtt.TorchCPUTracer(cv, nf, att_alpha=am.params["alpha_internal"], att_radius=am.params["R_physical"],
+                                abs_grid=ab.alpha_at_grid(), param="canonical")
+        x0, v0, shape, e0 = ort.setup_rays(cv, pg.angles[:k_angles])
+        f = np.asarray(tg.array, dtype=np.float64)
+        t0 = time.perf_counter()
+        tr.run(x0, v0, e0, f=f)
+        dt = time.perf_counter() - t0
+    finally:
+        torch.set_num_threads(prev)
     per = dt * (na / k_angles)
-    return {"value": 1.0 / per, "unit": "integrate passes/s", "cores": 1, "kind": "port",
-            "sample": "float64 NumPy port of the reference ray tracer (RK4, canonical), %d of %d angles "
-                      "(%d rays), %.1f s measured, scaled x%d" % (k_angles, na, x0.shape[0], dt, na // k_angles),
+    return {"value": 1.0 / per, "unit": "integrate passes/s", "cores": threads, "kind": "port",
+            "sample": "the reference's CPU-torch tracer (symplectic Euler, grid_sample, sparse-COO deposit; "
+                      "oracle/raytrace_torch.py) with torch.set_num_threads(%d), %d of %d angles (%d rays), "
+                      "%.1f s measured, scaled x%d" % (threads, k_angles, na, x0.shape[0], dt, na // k_angles),
             "host": "%d host cores visible (%s)" % (os.cpu_count() or 0, _cpu_model())}
 
 
