@@ -53,6 +53,22 @@ PINNING = {
     "optim.bclp_init": "filtered back projection recovers map_inv(T) (check_bclp_init_fbp_scale)",
     "optim.ver / circle_inds": "known answer and circle geometry (check_ver_known_answer, check_circle_inds)",
     "optim.fbp": "disk reconstruction and the offset variant (check_fbp_reconstructs)",
+    # --- image-sequence export (imageseq.py; imagesequence.py:85-399) ---
+    "imageseq.place": "hand-computed windows for one, three and two arrayed copies and the out-of-screen "
+        "errors (check_place_known_answer)",
+    "imageseq.rotate": "is scipy.ndimage.rotate, the reference's own call; quarter turn = index permutation "
+        "(check_rotate_is_scipy)",
+    "imageseq.resize_linear": "closed forms: same size = copy, constants, 2x ramp upscale at half-pixel centres "
+        "with edge clamps, 2x downscale = pair means (check_resize_closed_forms); OpenCV itself is absent, so "
+        "its exact float rounding is unpinned",
+    "imageseq.frames / to_unsigned / percentile": "hand-computed bytes for truncation, clipping, 10-bit "
+        "wrap modulo 256, 100th-percentile normalisation, flips (check_frames_known_answer); percentile is "
+        "np.percentile",
+    "optim.discretize": "hand-computed levels incl. the index -1 wrap, brute force over 64 levels "
+        "(check_discretize_known_answer)",
+    "optim.osmo / cal / fbp / bclp with bit_depth": "one-bit two-level projections inside OSMO, returned "
+        "sinograms on the [0, max] levels, quantisation never raises a value, 20 bits reproduce the "
+        "unquantised loops (check_osmo_bit_depth, check_cal_fbp_bclp_bit_depth)",
     "optim.pm_loss_grad / pm": "hand-computed known answer (check_pm_known_answer); gradient vs central "
                                "differences (test_pm_gradient_finite_differences)",
     # --- ray tracer (raytrace.py; pyTorchRayTrace.py, medium.py) ---
@@ -88,5 +104,5 @@ PINNING = {
                                        "test_fan_beam_snell_and_centre_ray)",
 }
 """Every entry's check fails under a planted one-line mistake in the oracle:
-``tests/test_oracle_mutations.py`` plants 56 such mutations and asserts that
+``tests/test_oracle_mutations.py`` plants 69 such mutations and asserts that
 the named check catches each one."""

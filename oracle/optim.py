@@ -4,6 +4,7 @@ TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
 
 * ``fourier_filter`` / ``filter_sinogram``  vamtoolbox/util/data.py:302-438
 * ``filter_target_osmo``                    vamtoolbox/util/data.py:43-113
+* ``discretize``                            vamtoolbox/util/data.py:495-523
 * ``Response``                              vamtoolbox/response.py:12-291
 * ``osmo``                                  vamtoolbox/optimizer/OSMO.py:9-178
 * ``cal``                                   vamtoolbox/optimizer/CAL.py:10-218
@@ -112,6 +113,16 @@ class Response:
         return np.interp(m, self.m0, self.f0, left=self.f0[0], right=self.f0[-1])
 
 
+def discretize(x, bit_depth, lo, hi):
+    """util/data.py:495-523: 2**bit_depth equally spaced levels from lo to hi
+    (both included); each value becomes the largest level <= it, values at or
+    above hi become hi, and values below lo take index -1, which NumPy's
+    indexing wraps to the last level (hi)."""
+    levels = np.linspace(lo, hi, 2 ** bit_depth, endpoint=True)
+    k = np.digitize(x, bins=levels) - 1
+    return levels[k]
+
+
 def circle_inds(target):
     n0, n1, n2 = target.shape
     cy, cx, _ = np.meshgrid(np.linspace(-1, 1, n0), np.linspace(-1, 1, n1), np.linspace(-1, 1, n2))
@@ -123,8 +134,10 @@ def ver(gel, void, x):
     return np.sum(x[void] >= np.min(x[gel])) / (np.sum(gel) + np.sum(void))
 
 
-def osmo(target, fwd, bwd, n_iter, d_h=0.8, d_l=0.7, filt="ram-lak", inhibition=0.0):
-    """OSMO.py:111-178 (no bit depth, no exit_param)."""
+def osmo(target, fwd, bwd, n_iter, d_h=0.8, d_l=0.7, filt="ram-lak", inhibition=0.0, bit_depth=None):
+    """OSMO.py:111-178 (no exit_param); ``bit_depth`` discretises each
+    normalised projection over [min, max] (OSMO.py:61-64, 99-102) and the
+    returned sinogram over [0, max] (OSMO.py:164-165)."""
     gel, void = circle_inds(target)
     err = np.full(n_iter, np.nan)
     xm = np.real(filter_target_osmo(target, filt))
@@ -136,6 +149,8 @@ def osmo(target, fwd, bwd, n_iter, d_h=0.8, d_l=0.7, filt="ram-lak", inhibition=
     def step(xm):
         b = fwd(xm)
         b = b / np.amax(b)
+        if bit_depth is not None:
+            b = discretize(b, bit_depth, np.min(b), np.max(b))
         b[b < inhibition] = inhibition
         xu = bwd(b)
         xu = xu / np.amax(xu)
@@ -153,6 +168,8 @@ def osmo(target, fwd, bwd, n_iter, d_h=0.8, d_l=0.7, filt="ram-lak", inhibition=
         xm[gel] = xm[gel] + gd
         x, b = step(xm)
         err[it] = ver(gel, void, x)
+    if bit_depth is not None:
+        b = discretize(b, bit_depth, 0.0, np.max(b))
     return b, x, err
 
 
@@ -169,9 +186,10 @@ def cal_threshold(x, gel, void, sigmoid):
     return 1 / (1 + np.exp(-(float(sigmoid) * (x - th)))), th
 
 
-def cal(target, fwd, bwd, n_iter, d_h=0.8, filt="ram-lak", lr=0.01, momentum=0.0, sigmoid=0.01):
+def cal(target, fwd, bwd, n_iter, d_h=0.8, filt="ram-lak", lr=0.01, momentum=0.0, sigmoid=0.01, bit_depth=None):
     """CAL.py:116-218: sigmoid threshold at d_h, or (d_h None) the score-maximising
-    threshold of cal_threshold."""
+    threshold of cal_threshold; ``bit_depth`` discretises each updated
+    sinogram over [0, max] (CAL.py:171-173)."""
     gel, void = circle_inds(target)
     err = np.full(n_iter, np.nan)
     b = np.clip(filter_sinogram(fwd(target), filt), 0, None)
@@ -188,6 +206,8 @@ def cal(target, fwd, bwd, n_iter, d_h=0.8, filt="ram-lak", lr=0.01, momentum=0.0
         db = fwd(xt - target)
         grad = db * (1 - momentum) + db_prev * momentum / (1 - momentum ** (it + 1))
         b = np.clip(b - lr * grad, 0, None)
+        if bit_depth is not None:
+            b = discretize(b, bit_depth, 0.0, np.max(b))
         x = bwd(b)
         x = x / np.amax(x)
         db_prev = db
@@ -212,13 +232,19 @@ def bclp_state(g, target, bwd, resp, eps, weight, p, q, dvol=1.0):
 
 
 def bclp(target, fwd, bwd, resp, n_iter, g0, eps=0.1, weight=1.0, p=2.0, q=1.0, lr=0.01, glb=0.0, gub=None,
-         grad_op=None):
+         grad_op=None, bit_depth=None):
     """BCLPNorm.gradientDescent (BCLP.py:477-526) from a given initial sinogram.
 
     ``grad_op`` (default ``fwd``) maps the operand to sinogram space.
     """
     grad_op = fwd if grad_op is None else grad_op
-    g = np.clip(np.asarray(g0, dtype=np.float64), glb, gub)
+
+    def constrain(g):                       # imposeSinogramConstraints, BCLP.py:528-533
+        if bit_depth is not None:
+            g = discretize(g, bit_depth, 0.0, np.amax(g))
+        return np.clip(g, glb, gub)
+
+    g = constrain(np.asarray(g0, dtype=np.float64))
     losses = np.full(n_iter + 1, np.nan)
     for k in range(n_iter):
         dose, m, loss, operand = bclp_state(g, target, bwd, resp, eps, weight, p, q)
@@ -226,7 +252,7 @@ def bclp(target, fwd, bwd, resp, n_iter, g0, eps=0.1, weight=1.0, p=2.0, q=1.0, 
         G = grad_op(operand)
         if loss != 0.0:
             G = (q * loss ** ((q - p) / q)) * G
-        g = np.clip(g - lr * G, glb, gub)
+        g = constrain(g - lr * G)
     dose, m, loss, _ = bclp_state(g, target, bwd, resp, eps, weight, p, q)
     losses[n_iter] = loss
     return g, dose, m, losses
@@ -239,8 +265,9 @@ def bclp_init(target, fwd, resp, n_angles, filt="ram-lak", glb=0.0, gub=None):
     return np.clip(g0, glb, gub)
 
 
-def fbp(target, fwd, bwd, filt="ram-lak", offset=False):
-    """FBP.py:42-63: filtered forward projection, non-negative, normalised; dose normalised."""
+def fbp(target, fwd, bwd, filt="ram-lak", offset=False, bit_depth=None):
+    """FBP.py:42-63: filtered forward projection, non-negative, normalised
+    (``bit_depth``: discretised over [0, max], FBP.py:56-57); dose normalised."""
     gel, void = circle_inds(target)
     b = filter_sinogram(fwd(target), filt)
     if offset:
@@ -248,6 +275,8 @@ def fbp(target, fwd, bwd, filt="ram-lak", offset=False):
     else:
         b = np.clip(b, 0, None)
     b = b / np.max(b)
+    if bit_depth is not None:
+        b = discretize(b, bit_depth, 0.0, np.max(b))
     x = bwd(b)
     x = x / np.max(x)
     return b, x, ver(gel, void, x)

@@ -170,3 +170,32 @@ def test_volume_save_load_round_trip(tmp_path):
     np.testing.assert_array_equal(sb.proj_geo.angles, pg.angles)
     assert sb.proj_geo.ray_type == "parallel" and sb.proj_geo.CUDA is True
     assert sb.proj_geo.det_offset == 1.5 and sb.proj_geo.absorption_per_voxel == 0.01
+
+
+def test_absorption_mask_known_answer():
+    """geometry.py:161-193 on a 5 x 5 grid of pixel 1 (grid radius r = 2.5,
+    samples at 0, +-1.25, +-2.5) in a container of radius R = 4: the mask is
+    exp(-mu (R - rho)) at in-grid radius rho <= r and 0 beyond it (the
+    corners); it is broadcast along z; missing radii or R < r raise."""
+    tg = vam.geometry.TargetGeometry(target=np.ones((5, 5, 3)))
+    pg = vam.geometry.ProjectionGeometry(np.arange(0, 180, 10.0), "parallel", CUDA=True, absorption_coeff=0.3,
+                                         container_radius=4.0, projector_pixel_size=1.0)
+    pg.calcAbsorptionMask(tg)
+    m = np.asarray(pg.absorption_mask)
+    assert m.shape == (5, 5, 3)
+    assert np.all(m == m[:, :, :1])
+    mu, R = 0.3, 4.0
+    assert m[2, 2, 0] == pytest.approx(np.exp(-mu * R), rel=1e-15)
+    assert m[0, 2, 0] == pytest.approx(np.exp(-mu * (R - 2.5)), rel=1e-15)
+    assert m[2, 4, 1] == pytest.approx(np.exp(-mu * (R - 2.5)), rel=1e-15)
+    assert m[1, 1, 2] == pytest.approx(np.exp(-mu * (R - np.hypot(1.25, 1.25))), rel=1e-15)
+    assert m[0, 0, 0] == 0.0 and m[4, 0, 0] == 0.0 and m[4, 4, 2] == 0.0
+    # radial: invariant under the grid's symmetries
+    np.testing.assert_array_equal(m[:, :, 0], m[::-1, :, 0])
+    np.testing.assert_array_equal(m[:, :, 0], m[:, :, 0].T)
+    with pytest.raises(Exception):
+        vam.geometry.ProjectionGeometry(np.arange(3.0), "parallel", absorption_coeff=0.3).calcAbsorptionMask(tg)
+    small = vam.geometry.ProjectionGeometry(np.arange(3.0), "parallel", absorption_coeff=0.3,
+                                            container_radius=2.0, projector_pixel_size=1.0)
+    with pytest.raises(Exception):
+        small.calcAbsorptionMask(tg)

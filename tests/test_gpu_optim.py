@@ -369,3 +369,129 @@ def test_bclp_eval_then_update_equals_step(gpu):
         assert float(log[it].item()) == float(l2.item()), it
         assert torch.equal(g1, g2), it
     assert not torch.equal(g1, g0)
+
+
+def _on_levels_close(a, ref, bits, top, max_frac, max_levels=1):
+    """Quantised iterates: float32 rounding moves a value across a level
+    boundary now and then, so at most max_frac of the entries may differ, and
+    each by at most max_levels level steps; all others agree to 1e-5 of the
+    top."""
+    a, ref = np.asarray(a, np.float64).ravel(), np.asarray(ref, np.float64).ravel()
+    step = top / (2 ** bits - 1)
+    d = np.abs(a - ref)
+    off = d > 1e-5 * top
+    print("  %d of %d entries one level apart (max diff %.3g, level %.3g)" % (off.sum(), a.size, d.max(), step))
+    assert off.mean() <= max_frac
+    assert d.max() <= (max_levels + 0.01) * step + 1e-5 * top
+
+
+def test_discretize_on_device_matches_oracle(gpu):
+    """util/data.py:495-523 on a device tensor: bucketise against the float64
+    levels, index -1 (below the range) wrapping to the top level as NumPy's
+    indexing does; bit for bit equal to the oracle's levels."""
+    import torch
+    import oracle.optim as oo
+    from vamtoolbox_amd.util import data
+    rng = np.random.default_rng(3)
+    lo, hi = 0.0, 1.7
+    lev = np.linspace(lo, hi, 2 ** 8)
+    x = np.concatenate([rng.uniform(-0.2, 1.9, 100000), lev, [lo, hi, -1e-30, hi * 2]]).astype(np.float32)
+    for bits in (1, 4, 8, 12):
+        got = data.discretize(torch.as_tensor(x, device="cuda"), bits, [lo, hi]).cpu().numpy()
+        ref = oo.discretize(x.astype(np.float64), bits, lo, hi).astype(np.float32)
+        np.testing.assert_array_equal(got, ref)
+    # range given as device scalars, as the loops pass torch.amin / torch.amax
+    t = torch.as_tensor(x, device="cuda")
+    got = data.discretize(t, 8, [torch.amin(t), torch.amax(t)]).cpu().numpy()
+    ref = oo.discretize(x.astype(np.float64), 8, float(x.min()), float(x.max())).astype(np.float32)
+    np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("method", ["OSMO", "CAL", "FBP", "BCLP"])
+def test_bit_depth_matches_oracle(gpu, method):
+    """The optimisers with bit_depth = 8 against the oracle loops with the
+    same quantisation (OSMO.py:61-64,99-102,164-165, CAL.py:171-173,
+    FBP.py:56-57, BCLP.py:528-533).  Quantisation inside a loop makes it
+    discontinuous: a value that a rounding moves across a level boundary
+    changes the next projection, which moves more values across boundaries.
+    On this problem the float64 oracle itself, with relative noise of 3e-6 on
+    its projections, ends 4 OSMO iterations with 1.5 % of the sinogram
+    entries on another level.  The loops are therefore compared over their
+    first quantised iterations (OSMO: one void and one gel step)."""
+    import oracle.optim as oo
+    import vamtoolbox_amd as vam
+    bits = 8
+    if method == "BCLP":
+        tg, pg, ang, rm, ro, g0, fwd, bwd, lr = _bclp_setup(gpu, "analytical")
+        n_iter = 3
+        opts = vam.optimize.Options(method="BCLP", n_iter=n_iter, response_model=rm, eps=0.05, p=2, q=1,
+                                    learning_rate=lr, g0=g0, bit_depth=bits)
+        g, dose, mapped, logs = vam.optimize.optimize(tg, pg, opts, output="components")
+        g_ref, dose_ref, _, loss_ref = oo.bclp(tg.array, fwd, bwd, ro, n_iter, g0, eps=0.05, lr=lr, bit_depth=bits)
+        print("BCLP bit depth 8: rel L2 g %.2e dose %.2e" % (rel_l2(g, g_ref), rel_l2(dose, dose_ref)))
+        _on_levels_close(g, g_ref, bits, g_ref.max(), 0.005)
+        assert rel_l2(dose, dose_ref) <= 1e-3
+        return
+    tg, pg, ang = _problem(seed=4)
+    fwd, bwd = _oracle_ops(ang)
+    if method == "OSMO":
+        opts = vam.optimize.Options(method="OSMO", n_iter=2, d_h=0.85, d_l=0.6, bit_depth=bits)
+        b_ref, x_ref, _ = oo.osmo(tg.array, fwd, bwd, 2, d_h=0.85, d_l=0.6, bit_depth=bits)
+    elif method == "CAL":
+        opts = vam.optimize.Options(method="CAL", n_iter=3, learning_rate=0.002, sigmoid=0.01, bit_depth=bits)
+        b_ref, x_ref, _ = oo.cal(tg.array, fwd, bwd, 3, lr=0.002, sigmoid=0.01, bit_depth=bits)
+    else:
+        opts = vam.optimize.Options(method="FBP", bit_depth=bits)
+        b_ref, x_ref, _ = oo.fbp(tg.array, fwd, bwd, bit_depth=bits)
+    sino, recon, _ = vam.optimize.optimize(tg, pg, opts)
+    print("%s bit depth 8: rel L2 sinogram %.2e recon %.2e" % (method, rel_l2(sino.array, b_ref),
+                                                              rel_l2(recon.array, x_ref)))
+    top = b_ref.max()
+    # OSMO re-quantises the [min, max]-level projection on [0, max] levels at
+    # the end: one boundary crossing inside the loop can show as up to 3 levels
+    # (so does the float64 oracle with 1e-6 noise on its projections)
+    _on_levels_close(sino.array, b_ref, bits, top, 0.005, 3 if method == "OSMO" else 1)
+    assert rel_l2(recon.array, x_ref) <= 1e-3
+
+
+@pytest.mark.parametrize("kind", ["glf", "glf_nu", "linear", "identity", "interpolation"])
+def test_response_map_autograd(gpu, kind):
+    """ResponseMap (vamx_response_map): map and map' of response.py:191-291 on
+    the device against the oracle's float64 response, and its autograd
+    backward grad * map'(x)."""
+    import torch
+    import oracle.optim as oo
+    import vamtoolbox_amd as vam
+    from vamtoolbox_amd.autograd import ResponseMap
+    if kind == "glf":
+        rm, ro = vam.response.ResponseModel(type="analytical", B=20, M=0.6), oo.Response("analytical", B=20, M=0.6)
+    elif kind == "glf_nu":
+        kw = dict(A=0.1, K=0.9, B=12, M=0.4, nu=2.5)
+        rm, ro = vam.response.ResponseModel(type="analytical", **kw), oo.Response("analytical", **kw)
+    elif kind == "linear":
+        rm = vam.response.ResponseModel(type="analytical", form="linear", M=1.7, C=-0.2)
+        ro = None
+    elif kind == "identity":
+        rm = vam.response.ResponseModel(type="analytical", form="identity")
+        ro = None
+    else:
+        rm, ro = vam.response.ResponseModel(B=25), oo.Response("interpolation", B=25)
+    rng = np.random.default_rng(9)
+    x = np.concatenate([rng.uniform(-0.5, 1.5, 50000), np.linspace(0, 1, 512), [0.0, 1.0, -3.0, 4.0]])
+    x = x.astype(np.float32).astype(np.float64)
+    if ro is not None:
+        m_ref, d_ref = ro.map(x), ro.dmap(x)
+    elif kind == "linear":
+        m_ref, d_ref = 1.7 * x - 0.2, np.full_like(x, 1.7)
+    else:
+        m_ref, d_ref = x.copy(), np.ones_like(x)
+    xt = torch.tensor(x, dtype=torch.float32, device=gpu, requires_grad=True)
+    m = ResponseMap.apply(xt, rm)
+    w = rng.uniform(-1, 1, x.size)
+    (m * torch.tensor(w, dtype=torch.float32, device=gpu)).sum().backward()
+    m_np, g_np = m.detach().cpu().numpy(), xt.grad.cpu().numpy()
+    e_m = np.max(np.abs(m_np - m_ref) / np.maximum(np.abs(m_ref), 1e-3))
+    e_g = np.max(np.abs(g_np - w * d_ref) / np.maximum(np.abs(w * d_ref), 1e-3))
+    print("%s: map max rel %.2e, grad max rel %.2e" % (kind, e_m, e_g))
+    # float64 evaluation, float32 storage: one rounding of the result
+    assert e_m <= 2e-7 and e_g <= 5e-7

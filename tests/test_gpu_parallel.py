@@ -427,6 +427,47 @@ def test_zero_dose_constraint(gpu):
     assert e <= TOL and zs.sum() > 0
 
 
+
+def test_radial_absorption_mask(gpu):
+    """The radial absorption mask of geometry.py:161-193 multiplies the
+    volume before the forward projection and the reconstruction after the
+    back projection (Projector3DParallelCUDA.py:43-44, 66-67); optimize()
+    builds it whenever absorption_coeff is set (optimize.py:199-200)."""
+    import oracle.d1 as d1
+    import oracle.optim as oo
+    import vamtoolbox_amd as vam
+    n, nz = 36, 5
+    ang = np.arange(0.0, 360.0, 8.0)
+    rng = np.random.default_rng(22)
+    kw = dict(absorption_coeff=0.08, container_radius=n / 2 * 0.1 + 0.9, projector_pixel_size=0.1)
+    tg = vam.geometry.TargetGeometry(target=rng.uniform(0, 1, (n, n, nz)))
+    pg = vam.geometry.ProjectionGeometry(ang, "parallel", CUDA=True, **kw)
+    pg.calcAbsorptionMask(tg)
+    mask = np.asarray(pg.absorption_mask)
+    assert mask.shape == (n, n, nz) and mask.max() < 1.0 and np.count_nonzero(mask == 0) > 0
+    A = vam.projectorconstructor.projectorconstructor(tg, pg)
+    x = rng.uniform(0, 1, (n, n, nz))
+    sino = rng.uniform(0, 1, (n, len(ang), nz))
+    e_f = rel_l2(A.forward(x), d1.forward(mask * x, ang))
+    e_b = rel_l2(A.backward(sino), mask * d1.backward(sino, ang))
+    print("radial mask: forward rel L2 %.2e backward %.2e" % (e_f, e_b))
+    assert e_f <= TOL and e_b <= TOL
+    # through optimize(): OSMO with the mask against the oracle loop on masked operators
+    t = np.zeros((n, n, nz))
+    yy, xx = np.mgrid[:n, :n] - (n - 1) / 2
+    t[(xx / 9) ** 2 + (yy / 6) ** 2 <= 1] = 1.0
+    tg = vam.geometry.TargetGeometry(target=t)
+    pg = vam.geometry.ProjectionGeometry(ang, "parallel", CUDA=True, **kw)
+    opts = vam.optimize.Options(method="OSMO", n_iter=4, d_h=0.85, d_l=0.6)
+    s_gpu, r_gpu, _ = vam.optimize.optimize(tg, pg, opts)
+    assert pg.absorption_mask is not None
+    b_ref, x_ref, _ = oo.osmo(t, lambda v: d1.forward(mask * v, ang), lambda s: mask * d1.backward(s, ang), 4,
+                              d_h=0.85, d_l=0.6)
+    e_s, e_r = rel_l2(s_gpu.array, b_ref), rel_l2(r_gpu.array, x_ref)
+    print("OSMO with the radial mask: rel L2 sinogram %.2e recon %.2e" % (e_s, e_r))
+    assert e_s <= 1e-4 and e_r <= 1e-4
+
+
 ADJ_WAVE_CASES = [
     # n, nz, angles, kwargs: z widths 4 / 2 / 1, partial 64-z chunks, odd n,
     # absorption and offset, angle counts around the 64-angle lane batches

@@ -42,6 +42,7 @@ LOOPS = "test_oracle_pin_loops"
 RAYS = "test_oracle_pin_raytrace"
 D1 = "test_oracle_d1"
 OPT = "test_oracle_optim"
+IMG = "test_oracle_imageseq"
 
 MUTATIONS = [
     # --- OSMO (OSMO.py:36-162) ---
@@ -68,7 +69,7 @@ MUTATIONS = [
      "check_bclp_closed_form_p2_q2"),
     ("oracle.optim", "bclp", "G = (q * loss ** ((q - p) / q)) * G", "G = (q * loss ** ((q - p) / p)) * G", LOOPS,
      "check_bclp_closed_form_p2_q1"),
-    ("oracle.optim", "bclp", "g = np.clip(g - lr * G, glb, gub)", "g = np.clip(g + lr * G, glb, gub)", LOOPS,
+    ("oracle.optim", "bclp", "g = constrain(g - lr * G)", "g = constrain(g + lr * G)", LOOPS,
      "check_bclp_closed_form_p2_q2"),
     ("oracle.optim", "bclp_state", "** (q / p)", "** (1 / p)", LOOPS, "check_bclp_closed_form_p2_q2"),
     ("oracle.optim", "bclp_state", "* np.sign(eT)", "", LOOPS, "check_bclp_closed_form_p2_q1"),
@@ -140,6 +141,31 @@ MUTATIONS = [
     ("oracle.optim", "filter_target_osmo", "f = np.sqrt(fx ** 2 + fy ** 2)", "f = fx ** 2 + fy ** 2", OPT,
      "test_filter_target_osmo_single_mode"),
     ("oracle.optim", "Response._dmap", "c2 = self.B / self.nu", "c2 = self.B", OPT, "test_response_closed_forms"),
+    # --- image-sequence export (imagesequence.py:85-399) ---
+    ("oracle.imageseq", "place", "extra = int(array_offset * d + v_offset)", "extra = int(array_offset * d)", IMG,
+     "check_place_known_answer"),
+    ("oracle.imageseq", "place", "d = int(np.ceil((k + 1) / 2) * (-1) ** k)", "d = int(np.ceil((k + 1) / 2))", IMG,
+     "check_place_known_answer"),
+    ("oracle.imageseq", "resize_linear", "f = (np.arange(n_dst) + 0.5) * (n_src / n_dst) - 0.5",
+     "f = np.arange(n_dst) * (n_src / n_dst)", IMG, "check_resize_closed_forms"),
+    ("oracle.imageseq", "frames", "m = to_unsigned(np.minimum(m, maxv), bit_depth)", "m = to_unsigned(m, bit_depth)",
+     IMG, "check_frames_known_answer"),
+    ("oracle.imageseq", "frames", "m = np.flip(m, axis=0)", "m = np.flip(m, axis=2)", IMG,
+     "check_frames_known_answer"),
+    ("oracle.imageseq", "frames", "img = m[:, j, :].T", "img = m[:, j, :]", IMG, "check_frames_known_answer"),
+    # --- bit depth (util/data.py:495-523 and its call sites) ---
+    ("oracle.optim", "discretize", "np.digitize(x, bins=levels) - 1", "np.digitize(x, bins=levels, right=True) - 1",
+     LOOPS, "check_discretize_known_answer"),
+    ("oracle.optim", "discretize", "2 ** bit_depth", "2 ** bit_depth - 1", LOOPS, "check_discretize_known_answer"),
+    ("oracle.optim", "osmo", "b = discretize(b, bit_depth, np.min(b), np.max(b))",
+     "b = discretize(b, bit_depth, 0.0, np.max(b))", LOOPS, "check_osmo_bit_depth"),
+    ("oracle.optim", "osmo", "b = discretize(b, bit_depth, 0.0, np.max(b))", "b = b", LOOPS, "check_osmo_bit_depth"),
+    ("oracle.optim", "cal", "b = discretize(b, bit_depth, 0.0, np.max(b))", "b = b", LOOPS,
+     "check_cal_fbp_bclp_bit_depth"),
+    ("oracle.optim", "fbp", "b = discretize(b, bit_depth, 0.0, np.max(b))", "b = b", LOOPS,
+     "check_cal_fbp_bclp_bit_depth"),
+    ("oracle.optim", "bclp", "g = discretize(g, bit_depth, 0.0, np.amax(g))", "g = g", LOOPS,
+     "check_cal_fbp_bclp_bit_depth"),
 ]
 
 

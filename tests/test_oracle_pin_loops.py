@@ -419,3 +419,95 @@ def test_fbp_reconstructs():
 
 def test_pm_known_answer():
     check_pm_known_answer()
+
+
+# --------------------------------------------------------------------------
+# bit depth (util/data.py:495-523 and its call sites in the loops)
+# --------------------------------------------------------------------------
+def check_discretize_known_answer():
+    """Two bits over [0, 3] give the levels 0, 1, 2, 3.  A value takes the
+    largest level not above it; values on or above the top level take it; a
+    value below the bottom level gets index -1, which NumPy indexing wraps to
+    the top level."""
+    x = np.array([-0.5, 0.0, 0.999, 1.0, 1.5, 2.9999, 3.0, 7.0])
+    np.testing.assert_array_equal(oo.discretize(x, 2, 0.0, 3.0), [3.0, 0.0, 0.0, 1.0, 1.0, 2.0, 3.0, 3.0])
+    # brute force over the 2**6 levels of a shifted range
+    rng = np.random.default_rng(11)
+    lo, hi = -0.3, 1.7
+    levels = lo + (hi - lo) * np.arange(64) / 63
+    x = np.concatenate([rng.uniform(lo, hi, 200), levels[::5]])
+    out = oo.discretize(x, 6, lo, hi)
+    for xi, oi in zip(x, out):
+        below = levels[levels <= xi + 1e-15 * abs(xi)]
+        assert abs(oi - below.max()) <= 1e-14, (xi, oi)
+        assert 0 <= xi - oi < (hi - lo) / 63 + 1e-14
+
+
+def _levels_of(v, top, bits):
+    """True when every value of v is k * top / (2**bits - 1) for an integer k."""
+    k = np.asarray(v) * (2 ** bits - 1) / top
+    return np.allclose(k, np.round(k), rtol=0, atol=1e-9)
+
+
+def check_osmo_bit_depth():
+    """OSMO.py:61-64, 99-102, 164-165.  With one bit, each normalised
+    projection inside the loop becomes two-valued: the entries equal to the
+    maximum stay 1, every other entry falls to the projection's minimum.  The
+    returned sinogram lies on the levels of [0, max].  Twenty bits reproduce the
+    unquantised loop."""
+    t, F, _ = _tiny_problem(4)
+    fwd, bwd = _pair(F, t.shape)
+    oo.osmo(t, fwd, bwd, 3, filt=None, bit_depth=1)
+    for xm, b_in in zip(fwd.calls[1:], bwd.calls[1:]):
+        raw = F @ xm.reshape(-1)
+        raw = raw / raw.max()
+        assert raw.min() > 0, "set-up: projections must stay positive"
+        expect = np.where(raw == 1.0, 1.0, raw.min())
+        np.testing.assert_array_equal(b_in, expect)
+    fwd, bwd = _pair(F, t.shape)
+    b3, _, _ = oo.osmo(t, fwd, bwd, 3, filt=None, bit_depth=3)
+    assert len(np.unique(b3)) <= 8 and _levels_of(b3, b3.max(), 3)
+    fwd, bwd = _pair(F, t.shape)
+    bq, xq, _ = oo.osmo(t, fwd, bwd, 3, filt=None, bit_depth=20)
+    fwd, bwd = _pair(F, t.shape)
+    b0, x0, _ = oo.osmo(t, fwd, bwd, 3, filt=None)
+    np.testing.assert_allclose(bq, b0, rtol=0, atol=2e-6)
+    np.testing.assert_allclose(xq, x0, rtol=0, atol=2e-6)
+
+
+def check_cal_fbp_bclp_bit_depth():
+    """CAL.py:171-173, FBP.py:56-57, BCLP.py:528-533: the sinogram is
+    quantised over [0, max] (levels k max / (2**bits - 1)) after each update
+    and never moves up by quantisation."""
+    t, F, _ = _tiny_problem(5)
+    fwd, bwd = _pair(F, t.shape)
+    b, _, _ = oo.cal(t, fwd, bwd, 3, filt=None, lr=0.01, bit_depth=3)
+    # the returned sinogram is normalised after the last quantisation
+    assert _levels_of(b, 1.0, 3)
+    for b_in in bwd.calls[1:]:
+        assert _levels_of(b_in, b_in.max(), 3) and len(np.unique(b_in)) <= 8
+    fwd, bwd = _pair(F, t.shape)
+    b2, _, _ = oo.fbp(t, fwd, bwd, filt=None, bit_depth=2)
+    fwd, bwd = _pair(F, t.shape)
+    b0, _, _ = oo.fbp(t, fwd, bwd, filt=None)
+    assert _levels_of(b2, 1.0, 2)
+    assert np.all(b2 <= b0 + 1e-15) and np.all(b0 - b2 < 1 / 3)
+    T, g0 = np.full((4, 3, 2), 0.5), np.linspace(0.1, 0.9, 24).reshape(4, 3, 2)
+    ident = lambda a: np.asarray(a, dtype=np.float64)
+    g, _, _, _ = oo.bclp(T, ident, ident, IdentityResponse(), 2, g0, eps=0.0, p=2.0, q=2.0, lr=0.1, bit_depth=2)
+    assert _levels_of(g, g.max(), 2) and len(np.unique(g)) <= 4
+    gq, _, _, _ = oo.bclp(T, ident, ident, IdentityResponse(), 2, g0, eps=0.0, p=2.0, q=2.0, lr=0.1, bit_depth=20)
+    gn, _, _, _ = oo.bclp(T, ident, ident, IdentityResponse(), 2, g0, eps=0.0, p=2.0, q=2.0, lr=0.1)
+    np.testing.assert_allclose(gq, gn, rtol=0, atol=2e-6)
+
+
+def test_discretize_known_answer():
+    check_discretize_known_answer()
+
+
+def test_osmo_bit_depth():
+    check_osmo_bit_depth()
+
+
+def test_cal_fbp_bclp_bit_depth():
+    check_cal_fbp_bclp_bit_depth()
