@@ -172,6 +172,13 @@ int vamx_bclp_eval(vamx_plan* plan, const vamx_response* resp, const vamx_bclp_p
  * the iteration counter and loss log. */
 int vamx_bclp_update(vamx_plan* plan, const vamx_bclp_params* prm, float* g, void* workspace, void* stream);
 
+/* Element-wise material response (ResponseModel.map and .dmapdf,
+ * vamtoolbox/response.py:191-291): mapped[i] = map(x[i]) and
+ * dmap[i] = map'(x[i]) for device float arrays of length n, evaluated in
+ * float64; either output may be NULL.  No host synchronisation. */
+int vamx_response_map(const vamx_response* resp, const float* x, float* mapped, float* dmap, int64_t n,
+                      void* stream);
+
 /* vamx_bclp_step without a workspace argument and without any allocation:
  * no counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
@@ -271,6 +278,17 @@ int vamx_rt_bclp_step(vamx_plan* plan, const vamx_response* resp, const vamx_bcl
                       float* g, const float* target, const float* eps_arr, const float* weight_arr,
                       float* dose, double* loss_log, int32_t* iter_counter, void* workspace, void* stream);
 
+/* Loss, dose and operand of the current ray sinogram g on the ray tracer,
+ * without updating g (BCLPNorm.updateVariables / computeLoss with
+ * PyTorchRayTracingPropagator, BCLP.py:277-351): the first half of
+ * vamx_rt_bclp_step.  operand (device [V], or NULL to leave it in the
+ * workspace) holds v w |e - eps|^(p-1) sign(e) map'(dose) without alpha_abs,
+ * so that q L^((q-p)/q) vamx_rt_integrate(operand) is the gradient of L(g);
+ * loss_out is a device double.  No host synchronisation. */
+int vamx_rt_bclp_eval(vamx_plan* plan, const vamx_response* resp, const vamx_bclp_params* prm,
+                      const float* g, const float* target, const float* eps_arr, const float* weight_arr,
+                      float* dose, float* operand, double* loss_out, void* workspace, void* stream);
+
 /* Propagation matrix of the ray tracer (PyTorchRayTracingPropagator.buildPropagationMatrix,
  * pyTorchRayTrace.py:168-197 -> recordEnergyUntilExit 573-690), in two passes
  * over the same traces:
@@ -283,6 +301,51 @@ int vamx_rt_bclp_step(vamx_plan* plan, const vamx_response* resp, const vamx_bcl
  * matrix P with P f = vamx_rt_integrate(f) and P^T g = vamx_rt_deposit(g). */
 int vamx_rt_record_counts(vamx_plan* plan, int64_t* counts, void* stream);
 int vamx_rt_record(vamx_plan* plan, const int64_t* offsets, int32_t* voxel, float* weight, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Image-sequence export (vamtoolbox/imagesequence.py:85-399)
+ * ------------------------------------------------------------------------- */
+#define VAMX_IMGSEQ_MAX_COPIES 64
+
+/* A sinogram-shaped float64 device array (nu, na, nv), v fastest, rotated in
+ * the (u, v) plane like imagesequence.py:397-399 (scipy.ndimage.rotate,
+ * axes (0, 2), order 1, constant 0 outside the input): out (mu, na, mv) at
+ * (i, j, k) samples the input at (matrix[0] i + matrix[1] k + offset[0],
+ * matrix[2] i + matrix[3] k + offset[1]); the caller computes mu, mv, matrix
+ * and offset as scipy does for reshape=True.  flip_u / flip_v reverse the
+ * input's u / v axes first (_invertU / _invertV, 363-370). */
+int vamx_imgseq_rotate(const double* in, int64_t nu, int64_t na, int64_t nv, int32_t flip_u, int32_t flip_v,
+                       const double* matrix, const double* offset, double* out, int64_t mu, int64_t mv,
+                       void* stream);
+
+/* Size scaling of each angle's (u, v) image (_scaleSize, imagesequence.py:
+ * 373-384: OpenCV resize with INTER_LINEAR) from (nu, na, nv) to (mu, na, mv),
+ * float64; flips as in vamx_imgseq_rotate. */
+int vamx_imgseq_resize(const double* in, int64_t nu, int64_t na, int64_t nv, int32_t flip_u, int32_t flip_v,
+                       double* out, int64_t mu, int64_t mv, void* stream);
+
+typedef struct vamx_imgseq_params {
+  int32_t n_u, n_v;           /* frame width and height (multiples of 4)             */
+  int32_t u1;                 /* first frame column of every copy                    */
+  int32_t n_copies;           /* 1 .. VAMX_IMGSEQ_MAX_COPIES                          */
+  int32_t v1[VAMX_IMGSEQ_MAX_COPIES];  /* first frame row of each copy, insertion order */
+  int32_t bit_depth;          /* 1 .. 32                                             */
+  int32_t normalize;          /* 1: x / norm_value * (2^bit_depth - 1)               */
+  double norm_value;          /* the normalisation percentile of the image           */
+  double intensity_scale;     /* applied when != 1                                   */
+  int32_t flip_u, flip_v;     /* reverse the image's u / v axes on read              */
+} vamx_imgseq_params;
+
+/* ImageSeq.__init__ (imagesequence.py:85-182) from the (possibly rotated and
+ * scaled) float64 image array (mu, na, mv): frames (na, n_v, n_u) uint8,
+ * frame j holding copies of the transposed (v, u) image of angle j at rows
+ * [v1[c], v1[c] + mv) and columns [u1, u1 + mu), a later copy overwriting an
+ * earlier one, zero elsewhere.  Each sample is normalised, scaled, truncated at
+ * 2^bit_depth - 1, cast toward zero to the bit depth's unsigned type and kept
+ * modulo 256 (NumPy's assignment into the reference's uint8 frames); NaN and
+ * negative samples give 0.  frames must be 4-byte aligned. */
+int vamx_imgseq_compose(const double* img, int64_t mu, int64_t na, int64_t mv, const vamx_imgseq_params* prm,
+                        uint8_t* frames, void* stream);
 
 /* ---------------------------------------------------------------------------
  * Algebraic (sparse-matrix) propagation

@@ -182,8 +182,10 @@ def test_media_variants(gpu, medium):
 
 
 def _rt_bclp_problem():
-    """16 x 16 x 4 graded medium, 12 angles, RK4 canonical, cylinder
-    attenuation and absorption, a smooth target in [0, 1]."""
+    """16 x 16 x 4 graded medium, 12 angles, RK4 canonical, attenuation and
+    absorption in a cylinder of radius 7.5 voxels (the default radius, the
+    smallest half-span, would be the 1.5-voxel half-height here), a smooth
+    target in [0, 1] inside it."""
     import oracle.raytrace as ort
     import vamtoolbox_amd as vam
     n, nz = 16, 4
@@ -193,11 +195,12 @@ def _rt_bclp_problem():
     vox = cv[0][1] - cv[0][0]
     X, Y, Z = np.meshgrid(*cv, indexing="ij")
     nf = 1.5 + 0.02 * np.exp(-((X - 2 * vox) ** 2 + Y ** 2 + Z ** 2) / (2 * (4 * vox) ** 2))
-    T = np.clip(1.2 * np.exp(-(X ** 2 + (Y - vox) ** 2) / (2 * (4 * vox) ** 2)) - 0.1, 0, 1)
+    T = np.clip(1.2 * np.exp(-(X ** 2 + (Y - vox) ** 2) / (2 * (2.5 * vox) ** 2)) - 0.1, 0, 1)
     alpha = 0.005 / vox
+    cyl = dict(alpha_internal=alpha, R=7.5 * vox, length_unit_of_R="physical_coordinate")
     im = vam.medium.IndexModel(cv, type="interpolation", form="freeform", n_x=nf)
-    am = vam.medium.AttenuationModel(cv, type="analytical", form="homogeneous_cylinder", alpha_internal=alpha)
-    ab = vam.medium.AbsorptionModel(cv, type="analytical", form="homogeneous_cylinder", alpha_internal=alpha)
+    am = vam.medium.AttenuationModel(cv, type="analytical", form="homogeneous_cylinder", **cyl)
+    ab = vam.medium.AbsorptionModel(cv, type="analytical", form="homogeneous_cylinder", **cyl)
     pg = vam.geometry.ProjectionGeometry(ang, "ray_trace", CUDA=True, index_model=im, attenuation_model=am,
                                          absorption_model=ab, ray_trace_method="eikonal",
                                          eikonal_parametrization="canonical", ray_trace_ode_solver="rk4",
@@ -247,11 +250,15 @@ def test_raytrace_bclp_fused_matches_oracle(gpu):
     with np.errstate(divide="ignore", invalid="ignore"):
         f = f / ab ** 2
     f[~np.isfinite(f)] = 0.0
-    g0 = np.clip(oo.filter_sinogram(fwd(f).reshape(shape), "ram-lak").ravel() * (np.pi / (2 * len(ang))), 0, None)
+    # 0.7 x the inverseBackward sinogram: the dose falls short of the band and
+    # the descent does real work (loss 7.27 -> 5.55, g moves by 19 %)
+    g0 = 0.7 * np.clip(oo.filter_sinogram(fwd(f).reshape(shape), "ram-lak").ravel() * (np.pi / (2 * len(ang))), 0,
+                       None)
     lr, n_iter = 1e-4, 20
     ro = oo.Response("analytical", B=20, M=0.6)
     g_ref, dose_ref, _, loss_ref = oo.bclp(T, fwd, bwd, ro, n_iter, g0, eps=0.05, lr=lr)
     assert np.all(np.diff(loss_ref) < 0), "test problem must descend"
+    assert loss_ref[-1] < 0.85 * loss_ref[0] and rel_l2(g_ref, g0) > 0.1, "the update must move g"
     rm = vam.response.ResponseModel(type="analytical", B=20, M=0.6)
     res = {}
     for graph in (False, True):
@@ -267,3 +274,83 @@ def test_raytrace_bclp_fused_matches_oracle(gpu):
     assert e_g <= 1e-4 and e_d <= 1e-4 and e_l <= 1e-4
     np.testing.assert_array_equal(res[False][0], res[True][0])
     np.testing.assert_array_equal(res[False][2], res[True][2])
+
+
+def test_autograd_ray_trace_and_deposit(gpu):
+    """RayTrace / RayDeposit: each one's backward is the other (the exact
+    transpose); gradients through autograd equal the oracle's deposit and
+    integrate, and the dot-product identity <W f, w> = <f, W^T w> holds."""
+    import torch
+    from vamtoolbox_amd.autograd import RayDeposit, RayTrace
+    from vamtoolbox_amd.projector.pyTorchRayTrace import PyTorchRayTracingPropagator
+    tg, pg, tr, (x0, v0, shape, e0), ab, ang = _rt_bclp_problem()
+    P = PyTorchRayTracingPropagator(tg, pg, output_torch_tensor=True)
+    rng = np.random.default_rng(5)
+    f = rng.uniform(0, 1, P.vol_shape)
+    w = rng.uniform(-1, 1, P.num_rays)
+    ft = torch.tensor(f, dtype=torch.float32, device=gpu, requires_grad=True)
+    wt = torch.tensor(w, dtype=torch.float32, device=gpu)
+    y = RayTrace.apply(ft, P)
+    (y * wt).sum().backward()
+    e_fwd = rel_l2(y.detach().cpu().numpy(), tr.run(x0, v0, e0, f=f))
+    e_grad = rel_l2(ft.grad.cpu().numpy(), tr.run(x0, v0, e0, g=w))
+    gt = torch.tensor(np.abs(w), dtype=torch.float32, device=gpu, requires_grad=True)
+    u = rng.uniform(-1, 1, P.vol_shape)
+    d = RayDeposit.apply(gt, P)
+    (d * torch.tensor(u, dtype=torch.float32, device=gpu)).sum().backward()
+    e_dep = rel_l2(d.detach().cpu().numpy(), tr.run(x0, v0, e0, g=np.abs(w)))
+    e_grad2 = rel_l2(gt.grad.cpu().numpy(), tr.run(x0, v0, e0, f=u))
+    lhs = float(np.dot(y.detach().cpu().numpy().astype(np.float64), w))
+    rhs = float(np.sum(f * RayDeposit.apply(wt, P).cpu().numpy()))
+    print("RayTrace fwd %.2e grad %.2e; RayDeposit fwd %.2e grad %.2e; dot %.6e vs %.6e" % (
+        e_fwd, e_grad, e_dep, e_grad2, lhs, rhs))
+    assert max(e_fwd, e_grad, e_dep, e_grad2) <= TOL
+    assert abs(lhs - rhs) <= 1e-5 * max(abs(lhs), abs(rhs))
+
+
+def test_autograd_raytrace_bclp_loss(gpu):
+    """RayTraceBCLPLoss (vamx_rt_bclp_eval forward, integrate backward)
+    against the oracle's BCLP loss and its gradient q L^((q-p)/q) W (alpha
+    operand) on the float64 tracer (BCLP.py:277-386), for p = 2, q = 1 and
+    p = 3, q = 2; and against a central difference of the GPU loss."""
+    import torch
+    import oracle.optim as oo
+    import vamtoolbox_amd as vam
+    from vamtoolbox_amd.autograd import RayTraceBCLPLoss
+    from vamtoolbox_amd.projector.pyTorchRayTrace import PyTorchRayTracingPropagator
+    tg, pg, tr, (x0, v0, shape, e0), ab, ang = _rt_bclp_problem()
+    P = PyTorchRayTracingPropagator(tg, pg, output_torch_tensor=True)
+    T = np.asarray(tg.array, dtype=np.float64)
+    fwd = lambda f: tr.run(x0, v0, e0, f=f)      # noqa: E731
+    bwd = lambda g: tr.run(x0, v0, e0, g=g)      # noqa: E731
+    ri = oo.Response("interpolation", B=20, M=0.6)
+    f = ri.map_inv(T)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        f = f / ab ** 2
+    f[~np.isfinite(f)] = 0.0
+    g0 = 0.7 * np.clip(oo.filter_sinogram(fwd(f).reshape(shape), "ram-lak").ravel() * (np.pi / (2 * len(ang))), 0,
+                       None)
+    rm = vam.response.ResponseModel(type="analytical", B=20, M=0.6)
+    ro = oo.Response("analytical", B=20, M=0.6)
+    Tt = torch.tensor(T, dtype=torch.float32, device=gpu)
+    rng = np.random.default_rng(8)
+    direction = rng.uniform(-1, 1, g0.shape) * g0
+    for p, q in ((2.0, 1.0), (3.0, 2.0)):
+        gt = torch.tensor(g0, dtype=torch.float32, device=gpu, requires_grad=True)
+        L = RayTraceBCLPLoss.apply(gt, P, rm, Tt, 0.05, 1.0, p, q)
+        L.backward()
+        _, _, L_ref, operand = oo.bclp_state(g0, T, bwd, ro, 0.05, 1.0, p, q)
+        G_ref = q * L_ref ** ((q - p) / q) * fwd(operand)
+        e_L = abs(float(L) - L_ref) / L_ref
+        e_G = rel_l2(gt.grad.cpu().numpy(), G_ref)
+        h = 5e-2   # float64 oracle: the central difference is within 2.4e-3 of the gradient here
+        with torch.no_grad():
+            dt = torch.tensor(direction, dtype=torch.float32, device=gpu)
+            Lp = float(RayTraceBCLPLoss.apply(gt + h * dt, P, rm, Tt, 0.05, 1.0, p, q))
+            Lm = float(RayTraceBCLPLoss.apply(gt - h * dt, P, rm, Tt, 0.05, 1.0, p, q))
+        fd = (Lp - Lm) / (2 * h)
+        an = float(np.dot(gt.grad.cpu().numpy().astype(np.float64), direction))
+        print("p=%g q=%g: loss %.6e vs %.6e (rel %.1e), grad rel L2 %.2e, central difference %.5e vs %.5e" % (
+            p, q, float(L), L_ref, e_L, e_G, fd, an))
+        assert e_L <= 1e-5 and e_G <= 1e-4
+        assert abs(fd - an) <= 1e-2 * abs(an)

@@ -81,6 +81,25 @@ class BclpParams(ctypes.Structure):
     ]
 
 
+IMGSEQ_MAX_COPIES = 64
+
+
+class ImgSeqParams(ctypes.Structure):
+    _fields_ = [
+        ("n_u", c_int32),
+        ("n_v", c_int32),
+        ("u1", c_int32),
+        ("n_copies", c_int32),
+        ("v1", c_int32 * IMGSEQ_MAX_COPIES),
+        ("bit_depth", c_int32),
+        ("normalize", c_int32),
+        ("norm_value", c_double),
+        ("intensity_scale", c_double),
+        ("flip_u", c_int32),
+        ("flip_v", c_int32),
+    ]
+
+
 class RtGeom(ctypes.Structure):
     _fields_ = [
         ("nx", c_int32), ("ny", c_int32), ("nz", c_int32),
@@ -136,6 +155,8 @@ SIGNATURES = {
                                  c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_void_p, c_void_p]),
     "vamx_bclp_update": (c_int32, [_PLAN_P, ctypes.POINTER(BclpParams), c_void_p, c_void_p, c_void_p]),
+    "vamx_response_map": (c_int32, [ctypes.POINTER(Response), c_void_p, c_void_p, c_void_p, ctypes.c_int64,
+                                    c_void_p]),
     "vamx_bclp_step_noalloc": (c_int32, [_PLAN_P, ctypes.POINTER(Response), ctypes.POINTER(BclpParams),
                                          c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                          c_void_p, c_void_p, c_void_p]),
@@ -147,6 +168,15 @@ SIGNATURES = {
     "vamx_rt_bclp_step": (c_int32, [_PLAN_P, ctypes.POINTER(Response), ctypes.POINTER(BclpParams),
                                     c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                     c_void_p, c_void_p, c_void_p]),
+    "vamx_rt_bclp_eval": (c_int32, [_PLAN_P, ctypes.POINTER(Response), ctypes.POINTER(BclpParams),
+                                    c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_void_p, c_void_p]),
+    "vamx_imgseq_rotate": (c_int32, [c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, c_int32, c_int32,
+                                     c_void_p, c_void_p, c_void_p, ctypes.c_int64, ctypes.c_int64, c_void_p]),
+    "vamx_imgseq_resize": (c_int32, [c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, c_int32, c_int32,
+                                     c_void_p, ctypes.c_int64, ctypes.c_int64, c_void_p]),
+    "vamx_imgseq_compose": (c_int32, [c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                      ctypes.POINTER(ImgSeqParams), c_void_p, c_void_p]),
     "vamx_rt_record_counts": (c_int32, [_PLAN_P, c_void_p, c_void_p]),
     "vamx_rt_record": (c_int32, [_PLAN_P, c_void_p, c_void_p, c_void_p, c_void_p]),
     "vamx_sino_resample": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p,

@@ -238,19 +238,26 @@ __global__ __launch_bounds__(256) void bclp_bpT_update(ParDev g, int clipc, cons
   vstore<ZV>(gp, gv);
 }
 
-int bclp_fill_devs(const vamx_response* resp, const vamx_bclp_params* prm, const float* eps_arr, const float* w_arr,
-                   RespDev& r, BclpDev& b) {
-  if (!resp || !prm) return set_error(VAMX_ERR_ARG, "bclp: null response or params");
+int resp_fill_dev(const vamx_response* resp, RespDev& r) {
+  if (!resp) return set_error(VAMX_ERR_ARG, "response: null model");
   if (resp->kind < VAMX_RESP_GLF || resp->kind > VAMX_RESP_TABLE)
-    return set_error(VAMX_ERR_ARG, "bclp: unknown response kind");
+    return set_error(VAMX_ERR_ARG, "response: unknown response kind");
   if (resp->kind == VAMX_RESP_TABLE && (resp->n_pts < 2 || !resp->table_f || !resp->table_map || !resp->table_dmap))
-    return set_error(VAMX_ERR_ARG, "bclp: interpolation response needs n_pts >= 2 and three device tables");
-  if (!(prm->p > 0.0) || !(prm->q > 0.0)) return set_error(VAMX_ERR_ARG, "bclp: p and q must be > 0");
+    return set_error(VAMX_ERR_ARG, "response: interpolation response needs n_pts >= 2 and three device tables");
   r.kind = resp->kind;
   r.A = resp->A; r.K = resp->K; r.B = resp->B; r.M = resp->M; r.nu = resp->nu;
   r.lin_M = resp->lin_M; r.lin_C = resp->lin_C;
   r.n_pts = resp->n_pts;
   r.tf = resp->table_f; r.tm = resp->table_map; r.td = resp->table_dmap;
+  return VAMX_OK;
+}
+
+int bclp_fill_devs(const vamx_response* resp, const vamx_bclp_params* prm, const float* eps_arr, const float* w_arr,
+                   RespDev& r, BclpDev& b) {
+  if (!resp || !prm) return set_error(VAMX_ERR_ARG, "bclp: null response or params");
+  const int rc = resp_fill_dev(resp, r);
+  if (rc) return rc;
+  if (!(prm->p > 0.0) || !(prm->q > 0.0)) return set_error(VAMX_ERR_ARG, "bclp: p and q must be > 0");
   b.p = prm->p; b.q = prm->q; b.eps = prm->eps; b.weight = prm->weight; b.dvol = prm->dvol;
   b.eps_arr = eps_arr;
   b.w_arr = w_arr;
@@ -509,6 +516,40 @@ extern "C" int vamx_bclp_eval(vamx_plan* plan, const vamx_response* resp, const 
   if (rc) return rc;
   hipLaunchKernelGGL(bclp_reduce, dim3(1), dim3(256), 0, s, partials, nparts, b.dvol, b.p, b.q, loss_out,
                      (int32_t*)nullptr, (double*)nullptr, scale);
+  VAMX_HIP(hipGetLastError());
+  return VAMX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Element-wise material response: mapped = map(x), dmap = map'(x), in float64
+// from float32 input (response.py:191-291), the forward and the saved
+// derivative of a response autograd node.
+// ---------------------------------------------------------------------------
+namespace {
+
+__global__ __launch_bounds__(256) void response_map_kernel(RespDev r, const float* __restrict__ x,
+                                                           float* __restrict__ mapped, float* __restrict__ dmap,
+                                                           long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double m, dm;
+  response_eval(r, (double)x[i], m, dm);
+  if (mapped) mapped[i] = (float)m;
+  if (dmap) dmap[i] = (float)dm;
+}
+
+}  // namespace
+
+extern "C" int vamx_response_map(const vamx_response* resp, const float* x, float* mapped, float* dmap, int64_t n,
+                                 void* stream) {
+  if (!x || (!mapped && !dmap)) return set_error(VAMX_ERR_ARG, "vamx_response_map: null buffer");
+  if (n < 0) return set_error(VAMX_ERR_ARG, "vamx_response_map: negative length");
+  RespDev r;
+  const int rc = resp_fill_dev(resp, r);
+  if (rc) return rc;
+  if (n == 0) return VAMX_OK;
+  hipLaunchKernelGGL(response_map_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, r, x, mapped,
+                     dmap, (long long)n);
   VAMX_HIP(hipGetLastError());
   return VAMX_OK;
 }

@@ -109,6 +109,8 @@ __device__ __forceinline__ double block_sum_256(double v, double* sh) {
 // (one block, fixed order; defined in bclp.hip)
 int bclp_reduce_launch(const double* partials, int nparts, const BclpDev& b, double* loss_dst, int32_t* counter,
                        double* loss_log, double* scale_out, hipStream_t s);
+// the response model of a C-ABI call in device form
+int resp_fill_dev(const vamx_response* resp, RespDev& r);
 // the response and loss parameters of a C-ABI call in device form
 int bclp_fill_devs(const vamx_response* resp, const vamx_bclp_params* prm, const float* eps_arr, const float* w_arr,
                    RespDev& r, BclpDev& b);

@@ -603,10 +603,11 @@ __global__ __launch_bounds__(256) void rt_finalize_kernel(const unsigned long lo
 // Deposit finalisation fused with the BCLP epilogue (the ray tracer's
 // backward followed by BCLPNorm's loss and operand, BCLP.py:277-367):
 // dose = alpha_abs acc / scale, the response, band and Lp weighting per
-// voxel, the operand times alpha_abs (the f_eff that the gradient's
-// integrate pass reads) and per-block loss partials.
+// voxel, the operand (times ab_op when given: alpha_abs makes it the f_eff
+// that the gradient's integrate pass reads) and per-block loss partials.
 __global__ __launch_bounds__(256) void rt_bclp_finalize_kernel(const unsigned long long* __restrict__ acc,
                                                                const float* __restrict__ ab,
+                                                               const float* __restrict__ ab_op,
                                                                const double* __restrict__ fx_scale, RespDev r,
                                                                BclpDev b, const float* __restrict__ target,
                                                                float* __restrict__ dose,
@@ -621,7 +622,7 @@ __global__ __launch_bounds__(256) void rt_bclp_finalize_kernel(const unsigned lo
     dose[i] = d;
     float o;
     part = bclp_voxel(r, b, d, target[i], (size_t)i, o);
-    operand_eff[i] = ab ? o * ab[i] : o;
+    operand_eff[i] = ab_op ? o * ab_op[i] : o;
   }
   const double sum = block_sum_256(part, sh);
   if (threadIdx.x == 0) partials[blockIdx.x] = sum;
@@ -920,7 +921,7 @@ extern "C" int vamx_rt_bclp_step(vamx_plan* plan, const vamx_response* resp, con
   if (rc) return rc;
   // response, band, Lp weighting, operand and loss partials; loss and scale
   hipLaunchKernelGGL(rt_bclp_finalize_kernel, dim3(w.nparts), dim3(256), 0, s, plan->d_accum, gd.abs_grid,
-                     (const double*)fx_scale, r, b, target, dose, operand_eff, partials, nv);
+                     gd.abs_grid, (const double*)fx_scale, r, b, target, dose, operand_eff, partials, nv);
   VAMX_HIP(hipGetLastError());
   rc = bclp_reduce_launch(partials, w.nparts, b, nullptr, iter_counter, loss_log, scale, s);
   if (rc) return rc;
@@ -937,6 +938,48 @@ extern "C" int vamx_rt_bclp_step(vamx_plan* plan, const vamx_response* resp, con
   rc = launch_trace<RT_INTEGRATE>(gd, dim3(blocks_for(nrays, 256)), s, iu);
   if (rc) return rc;
   VAMX_HIP(hipGetLastError());
+  plan->last_stream = s;
+  return VAMX_OK;
+}
+
+// The loss half of vamx_rt_bclp_step for an autograd node: dose, operand
+// (without alpha_abs: vamx_rt_integrate applies it) and L(g); g is not changed.
+extern "C" int vamx_rt_bclp_eval(vamx_plan* plan, const vamx_response* resp, const vamx_bclp_params* prm,
+                                 const float* g, const float* target, const float* eps_arr, const float* weight_arr,
+                                 float* dose, float* operand, double* loss_out, void* workspace, void* stream) {
+  if (!check_plan(plan, PLAN_RAYTRACE)) return set_error(VAMX_ERR_STATE, "vamx_rt_bclp_eval: not a ray-trace plan");
+  if (!g || !target || !dose || !loss_out || !workspace)
+    return set_error(VAMX_ERR_ARG, "vamx_rt_bclp_eval: null buffer");
+  RespDev r;
+  BclpDev b;
+  int rc = bclp_fill_devs(resp, prm, eps_arr, weight_arr, r, b);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const RtDev& gd = plan->rt;
+  const long long nv = (long long)gd.nx * gd.ny * gd.nz;
+  const long long nrays = (long long)gd.nt * gd.na * gd.nzr;
+  const RtBclpWs w = rt_bclp_ws(plan);
+  char* ws = (char*)workspace;
+  double* partials = (double*)(ws + w.partials_off);
+  double* scale = (double*)(ws + w.scale_off);
+  float* op = operand ? operand : (float*)(ws + w.operand_off);
+  double* fx_scale = (double*)plan->d_gmax;
+  float* gmax_part = (float*)((char*)plan->d_gmax + 64);
+  hipLaunchKernelGGL(rt_gmax_part, dim3(kGmaxParts), dim3(256), 0, s, g, nrays, gmax_part);
+  hipLaunchKernelGGL(rt_gmax_final, dim3(1), dim3(256), 0, s, gmax_part, (double)gd.na, fx_scale, plan->d_stats);
+  hipLaunchKernelGGL(rt_zero_u64, dim3(blocks_for(nv, 256)), dim3(256), 0, s, plan->d_accum, nv);
+  RtIo io{};
+  io.gin = g;
+  io.acc = plan->d_accum;
+  io.fx_scale = fx_scale;
+  io.stats = plan->d_stats;
+  rc = launch_trace<RT_DEPOSIT>(gd, dim3(blocks_for(nrays, 256)), s, io);
+  if (rc) return rc;
+  hipLaunchKernelGGL(rt_bclp_finalize_kernel, dim3(w.nparts), dim3(256), 0, s, plan->d_accum, gd.abs_grid,
+                     (const float*)nullptr, (const double*)fx_scale, r, b, target, dose, op, partials, nv);
+  VAMX_HIP(hipGetLastError());
+  rc = bclp_reduce_launch(partials, w.nparts, b, loss_out, nullptr, nullptr, scale, s);
+  if (rc) return rc;
   plan->last_stream = s;
   return VAMX_OK;
 }
