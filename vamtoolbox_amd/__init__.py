@@ -2,7 +2,7 @@
 
 Drop-in for the reference's projector / optimiser path: ``geometry``,
 ``projectorconstructor``, ``optimize`` (CAL, OSMO, BCLP), ``response``,
-``medium``, ``metrics`` and ``util.data``, with the propagators running as
+``medium``, ``metrics``, ``util.data`` and ``imagesequence``, with the propagators running as
 hand-written gfx950 HIP kernels (libvamx_hip.so, C ABI in include/vamx.h).
 """
 
@@ -11,5 +11,6 @@ from . import util  # noqa: F401
 from . import medium  # noqa: F401
 from . import projectorconstructor  # noqa: F401
 from . import optimize  # noqa: F401
+from . import imagesequence  # noqa: F401
 
 __version__ = "0.1.0"

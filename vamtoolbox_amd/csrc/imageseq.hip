@@ -1,0 +1,246 @@
+// Image-sequence export for gfx950: the frames a DLP projector shows, one per
+// angle, composed from the sinogram on the device.
+//
+// Reference: vamtoolbox/imagesequence.py, ImageSeq.__init__ (85-182) and its
+// helpers (300-399):
+//   flips of the u / v axes, rotation in the (u, v) plane
+//   (scipy.ndimage.rotate, order 1, reshape=True, constant 0 outside),
+//   size scaling (OpenCV resize, INTER_LINEAR), percentile normalisation,
+//   intensity scale, truncation at 2^bit_depth - 1, cast to the bit depth's
+//   unsigned type, and insertion of one or more copies of each angle's
+//   (v, u) image into a zeroed uint8 frame (a later copy overwrites an earlier
+//   one; wider types wrap modulo 256 on insertion, as NumPy's assignment does).
+//
+// Three kernels, all float64 like the reference's NumPy arrays:
+//   imgseq_rotate   one thread per output sample (v fastest), flips fused into
+//                   the read;
+//   imgseq_resize   one thread per output sample, flips fused into the read;
+//   imgseq_compose  one 256-thread block per 64 x 64 tile of one frame: the
+//                   tile's samples are read along v (the sinogram's fastest
+//                   axis, coalesced), converted to bytes, transposed through
+//                   LDS and written along u (the frame's fastest axis) as
+//                   dwords.
+#include "vamx_internal.h"
+
+namespace vamx {
+
+namespace {
+
+// scipy's order-1 affine transform in 'constant' mode: a sample whose
+// coordinate lies outside [0, n - 1] on either axis is 0; inside, the four
+// neighbours are summed as ((v * wy) * wx) in (0,0), (0,1), (1,0), (1,1)
+// order, a neighbour beyond the last row / column counting as 0.
+__global__ __launch_bounds__(256) void imgseq_rotate(const double* __restrict__ in, long long nu, long long na,
+                                                     long long nv, int flip_u, int flip_v, double m00, double m01,
+                                                     double m10, double m11, double off0, double off1,
+                                                     double* __restrict__ out, long long mu, long long mv) {
+#pragma clang fp contract(off)
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= mu * na * mv) return;
+  const long long k = tid % mv;
+  const long long j = (tid / mv) % na;
+  const long long i = tid / (mv * na);
+  const double fi = (double)i, fk = (double)k;
+  const double cy = m00 * fi + m01 * fk + off0;
+  const double cx = m10 * fi + m11 * fk + off1;
+  double v = 0.0;
+  if (cy >= 0.0 && cy <= (double)(nu - 1) && cx >= 0.0 && cx <= (double)(nv - 1)) {
+    const double y0f = floor(cy), x0f = floor(cx);
+    const double ty = cy - y0f, tx = cx - x0f;
+    const long long y0 = (long long)y0f, x0 = (long long)x0f;
+    const double wy[2] = {1.0 - ty, ty}, wx[2] = {1.0 - tx, tx};
+    double t = 0.0;
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const long long yy = y0 + a, xx = x0 + b;
+        double s = 0.0;
+        if (yy < nu && xx < nv) {
+          const long long ys = flip_u ? nu - 1 - yy : yy, xs = flip_v ? nv - 1 - xx : xx;
+          s = in[(ys * na + j) * nv + xs];
+        }
+        t += (s * wy[a]) * wx[b];
+      }
+    }
+    v = t;
+  }
+  out[tid] = v;
+}
+
+// OpenCV INTER_LINEAR: destination index d samples (d + 0.5) * (src / dst) -
+// 0.5; the integer part is clamped to [0, src - 1] with a zero fraction at
+// either edge.
+__device__ __forceinline__ void linear_tap(long long d, long long n_src, long long n_dst, long long& i0,
+                                           long long& i1, double& t) {
+#pragma clang fp contract(off)
+  const double f = ((double)d + 0.5) * ((double)n_src / (double)n_dst) - 0.5;
+  const double f0 = floor(f);
+  i0 = (long long)f0;
+  t = f - f0;
+  if (i0 < 0) {
+    t = 0.0;
+    i0 = 0;
+  }
+  if (i0 >= n_src - 1) {
+    t = 0.0;
+    i0 = n_src - 1;
+  }
+  i1 = i0 + 1 < n_src ? i0 + 1 : n_src - 1;
+}
+
+__global__ __launch_bounds__(256) void imgseq_resize(const double* __restrict__ in, long long nu, long long na,
+                                                     long long nv, int flip_u, int flip_v, double* __restrict__ out,
+                                                     long long mu, long long mv) {
+#pragma clang fp contract(off)
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= mu * na * mv) return;
+  const long long k = tid % mv;
+  const long long j = (tid / mv) % na;
+  const long long i = tid / (mv * na);
+  long long y0, y1, x0, x1;
+  double ty, tx;
+  linear_tap(i, nu, mu, y0, y1, ty);
+  linear_tap(k, nv, mv, x0, x1, tx);
+  auto at = [&](long long y, long long x) {
+    const long long ys = flip_u ? nu - 1 - y : y, xs = flip_v ? nv - 1 - x : x;
+    return in[(ys * na + j) * nv + xs];
+  };
+  const double top = at(y0, x0) * (1.0 - tx) + at(y0, x1) * tx;
+  const double bot = at(y1, x0) * (1.0 - tx) + at(y1, x1) * tx;
+  out[tid] = top * (1.0 - ty) + bot * ty;
+}
+
+constexpr int kTile = 64;
+constexpr int kPitch = 68;   // bytes per LDS row: 17 dwords, so 64 lanes writing one column hit 64 banks
+
+struct ComposeDev {
+  int n_u, n_v, u1, n_copies, bit_depth, normalize, flip_u, flip_v;
+  double norm_value, max_value, intensity_scale;
+  int v1[VAMX_IMGSEQ_MAX_COPIES];
+};
+
+// truncate at max_value (NaN passes, as np.minimum), cast toward zero to the
+// unsigned type (NaN and negatives give 0), keep the low byte
+__device__ __forceinline__ unsigned char to_byte(const ComposeDev& d, double x) {
+#pragma clang fp contract(off)
+  if (d.normalize) x = x / d.norm_value * d.max_value;
+  if (d.intensity_scale != 1.0) x = x * d.intensity_scale;
+  if (x > d.max_value) x = d.max_value;
+  if (!(x >= 0.0)) return 0;
+  return (unsigned char)((unsigned)x & 0xffu);
+}
+
+__global__ __launch_bounds__(256) void imgseq_compose(ComposeDev d, const double* __restrict__ img, long long mu,
+                                                      long long na, long long mv, unsigned char* __restrict__ frames) {
+  __shared__ __align__(16) unsigned char tile[kTile * kPitch];
+  const int u0 = blockIdx.x * kTile, v0 = blockIdx.y * kTile;
+  const long long j = blockIdx.z;
+  const int t = threadIdx.x;
+  // load: lanes along v (the image's contiguous axis), 4 columns per pass
+  {
+    const int vl = t & 63;
+    const int v = v0 + vl;
+    int k = -1;
+    if (v < d.n_v) {
+      for (int c = 0; c < d.n_copies; ++c)
+        if (v >= d.v1[c] && v < d.v1[c] + (int)mv) k = c;   // the last copy covering the row wins
+    }
+    const long long r = k >= 0 ? (long long)(v - d.v1[k]) : 0;
+    const long long rs = d.flip_v ? mv - 1 - r : r;
+    for (int ul = t >> 6; ul < kTile; ul += 4) {
+      const int u = u0 + ul;
+      unsigned char b = 0;
+      if (k >= 0 && u >= d.u1 && u < d.u1 + (int)mu) {
+        const long long c = u - d.u1;
+        const long long cs = d.flip_u ? mu - 1 - c : c;
+        b = to_byte(d, img[(cs * na + j) * mv + rs]);
+      }
+      tile[vl * kPitch + ul] = b;
+    }
+  }
+  __syncthreads();
+  // store: 4 threads per frame row, 16 bytes each as 4 dwords
+  {
+    const int vl = t >> 2, ch = t & 3;
+    const int v = v0 + vl;
+    if (v >= d.n_v) return;
+    unsigned char* row = frames + ((size_t)j * d.n_v + v) * d.n_u;
+    const unsigned* src = reinterpret_cast<const unsigned*>(tile + vl * kPitch + ch * 16);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int u = u0 + ch * 16 + q * 4;
+      if (u + 4 <= d.n_u) *reinterpret_cast<unsigned*>(row + u) = src[q];
+    }
+  }
+}
+
+}  // namespace
+
+}  // namespace vamx
+
+using namespace vamx;
+
+extern "C" int vamx_imgseq_rotate(const double* in, int64_t nu, int64_t na, int64_t nv, int32_t flip_u,
+                                  int32_t flip_v, const double* matrix, const double* offset, double* out,
+                                  int64_t mu, int64_t mv, void* stream) {
+  if (!in || !out || !matrix || !offset) return set_error(VAMX_ERR_ARG, "vamx_imgseq_rotate: null argument");
+  if (nu < 1 || na < 1 || nv < 1 || mu < 1 || mv < 1)
+    return set_error(VAMX_ERR_ARG, "vamx_imgseq_rotate: empty array");
+  hipLaunchKernelGGL(imgseq_rotate, dim3(blocks_for(mu * na * mv, 256)), dim3(256), 0, (hipStream_t)stream, in,
+                     (long long)nu, (long long)na, (long long)nv, flip_u ? 1 : 0, flip_v ? 1 : 0, matrix[0],
+                     matrix[1], matrix[2], matrix[3], offset[0], offset[1], out, (long long)mu, (long long)mv);
+  VAMX_HIP(hipGetLastError());
+  return VAMX_OK;
+}
+
+extern "C" int vamx_imgseq_resize(const double* in, int64_t nu, int64_t na, int64_t nv, int32_t flip_u,
+                                  int32_t flip_v, double* out, int64_t mu, int64_t mv, void* stream) {
+  if (!in || !out) return set_error(VAMX_ERR_ARG, "vamx_imgseq_resize: null argument");
+  if (nu < 1 || na < 1 || nv < 1 || mu < 1 || mv < 1)
+    return set_error(VAMX_ERR_ARG, "vamx_imgseq_resize: empty array");
+  hipLaunchKernelGGL(imgseq_resize, dim3(blocks_for(mu * na * mv, 256)), dim3(256), 0, (hipStream_t)stream, in,
+                     (long long)nu, (long long)na, (long long)nv, flip_u ? 1 : 0, flip_v ? 1 : 0, out,
+                     (long long)mu, (long long)mv);
+  VAMX_HIP(hipGetLastError());
+  return VAMX_OK;
+}
+
+extern "C" int vamx_imgseq_compose(const double* img, int64_t mu, int64_t na, int64_t mv,
+                                   const vamx_imgseq_params* prm, uint8_t* frames, void* stream) {
+  if (!img || !prm || !frames) return set_error(VAMX_ERR_ARG, "vamx_imgseq_compose: null argument");
+  if (mu < 1 || na < 1 || mv < 1) return set_error(VAMX_ERR_ARG, "vamx_imgseq_compose: empty image");
+  if (na > 65535) return set_error(VAMX_ERR_ARG, "vamx_imgseq_compose: at most 65535 frames per call");
+  if (prm->n_u < 4 || prm->n_v < 4 || prm->n_u % 4 || prm->n_v % 4)
+    return set_error(VAMX_ERR_ARG, "vamx_imgseq_compose: frame dimensions must be positive multiples of 4");
+  if (prm->n_copies < 1 || prm->n_copies > VAMX_IMGSEQ_MAX_COPIES)
+    return set_error(VAMX_ERR_ARG, "vamx_imgseq_compose: n_copies out of range");
+  if (prm->bit_depth < 1 || prm->bit_depth > 32)
+    return set_error(VAMX_ERR_ARG, "vamx_imgseq_compose: bit_depth must be in 1..32");
+  if (prm->u1 < 0 || (int64_t)prm->u1 + mu > prm->n_u)
+    return set_error(VAMX_ERR_ARG, "vamx_imgseq_compose: image exceeds the frame in u");
+  for (int c = 0; c < prm->n_copies; ++c)
+    if (prm->v1[c] < 0 || (int64_t)prm->v1[c] + mv > prm->n_v)
+      return set_error(VAMX_ERR_ARG, "vamx_imgseq_compose: image copy exceeds the frame in v");
+  if ((uintptr_t)frames % 4) return set_error(VAMX_ERR_ARG, "vamx_imgseq_compose: frames must be 4-byte aligned");
+  // a NaN normalisation value (a percentile of data holding NaN) makes every
+  // sample NaN and every byte 0, as the reference's cast of NaN does
+  ComposeDev d;
+  d.n_u = prm->n_u;
+  d.n_v = prm->n_v;
+  d.u1 = prm->u1;
+  d.n_copies = prm->n_copies;
+  d.bit_depth = prm->bit_depth;
+  d.normalize = prm->normalize ? 1 : 0;
+  d.flip_u = prm->flip_u ? 1 : 0;
+  d.flip_v = prm->flip_v ? 1 : 0;
+  d.norm_value = prm->norm_value;
+  d.max_value = prm->bit_depth == 32 ? 4294967295.0 : (double)((1ull << prm->bit_depth) - 1ull);
+  d.intensity_scale = prm->intensity_scale;
+  for (int c = 0; c < VAMX_IMGSEQ_MAX_COPIES; ++c) d.v1[c] = c < prm->n_copies ? prm->v1[c] : 0;
+  const dim3 grid((prm->n_u + kTile - 1) / kTile, (prm->n_v + kTile - 1) / kTile, (unsigned)na);
+  hipLaunchKernelGGL(imgseq_compose, grid, dim3(256), 0, (hipStream_t)stream, d, img, (long long)mu, (long long)na,
+                     (long long)mv, frames);
+  VAMX_HIP(hipGetLastError());
+  return VAMX_OK;
+}
