@@ -44,12 +44,9 @@ def test_frames_match_oracle(gpu, case):
     diff = got.astype(int) - ref.astype(int)
     off = np.count_nonzero(diff)
     print("case %d %s: %d of %d pixels differ (max %d)" % (case, kw, off, got.size, np.abs(diff).max()))
-    if "rotate_angle" in kw or "size_scale" in kw:
-        # float64 interpolation in another summation order: a sample within
-        # ~1e-13 of an integer may truncate to the neighbouring value
-        assert off <= 1e-4 * got.size and np.abs(diff).max() <= 1
-    else:
-        assert off == 0
+    # the kernels evaluate every float64 expression in the oracle's (and
+    # scipy's) operation order, so even rotated and scaled frames agree bit for bit
+    assert off == 0
     assert len(seq.images) == s.shape[1] and np.array_equal(seq.images[3], ref[3])
 
 

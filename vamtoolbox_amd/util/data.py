@@ -153,7 +153,7 @@ def discretize(x, bit_depth: int, range: list, output_dtype=float):
         r0 = float(range[0]) if not _is_t(range[0]) else range[0].item()
         r1 = float(range[1]) if not _is_t(range[1]) else range[1].item()
         bins = torch.as_tensor(np.linspace(r0, r1, 2 ** bit_depth, endpoint=True), device=x.device, dtype=torch.float64)
-        inds = torch.bucketize(x.to(torch.float64), bins, right=True) - 1
+        inds = torch.bucketize(x.to(torch.float64).contiguous(), bins, right=True) - 1
         inds = torch.remainder(inds, bins.numel())
         return bins[inds].to(x.dtype)
     bins = np.linspace(range[0], range[1], 2 ** bit_depth, endpoint=True)
