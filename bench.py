@@ -199,21 +199,22 @@ def _cpu_slice_iteration(job):
     eT = resp.map(dose) - target_sl
     eb = np.abs(eT) - 0.05
     operand = (eb > 0) * np.abs(eb) * np.sign(eT) * resp.dmap(dose)
-    d1.forward(operand, angles, det_offset=OFFSET, mu=MU)
-    return 0
+    return dose, d1.forward(operand, angles, det_offset=OFFSET, mu=MU)
 
 
 def cpu_baseline_all_cores(pool, workers, target_np, g_np, angles):
     """The z-slab variant: the iteration is independent per z-slice at zero
     inclination, so `workers` processes each run one slice of the port at
     once; the wall time of those slices is scaled to all NZ slices."""
-    jobs = [(target_np[:, :, z:z + 1], g_np[:, :, z:z + 1], angles) for z in range(workers)]
+    zs = [k * NZ // workers + NZ // (2 * workers) for k in range(workers)]   # spread over the volume
+    jobs = [(target_np[:, :, z:z + 1], g_np[:, :, z:z + 1], angles) for z in zs]
     pool.map(_cpu_slice_iteration, jobs[:1])          # warm the workers' imports
     t0 = time.perf_counter()
-    pool.map(_cpu_slice_iteration, jobs, chunksize=1)
+    res = pool.map(_cpu_slice_iteration, jobs, chunksize=1)
     dt = time.perf_counter() - t0
     per_iter = dt * (NZ / workers)
-    return {"value": 1.0 / per_iter, "unit": "iters/s", "cores": workers, "kind": "port",
+    return {"value": 1.0 / per_iter, "unit": "iters/s", "cores": workers, "kind": "port", "slices": zs,
+            "results": res,
             "sample": "float64 NumPy port of Projector3DParallelPython back+forward projection and the BCLP "
                       "epilogue, %d worker processes (one z-slice each, OMP_NUM_THREADS=1) over %d of %d z-slices "
                       "(all %d angles), %.1f s wall, scaled x%.0f to one iteration" % (
@@ -235,7 +236,7 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="run only the forward / back projection kernels of the workload (5 launches each) "
                          "and a 1 GiB device copy, for hardware-counter profiling")
-    ap.add_argument("--workload", choices=("bclp", "osmo", "batched", "raytrace"), default="bclp",
+    ap.add_argument("--workload", choices=("bclp", "osmo", "batched", "raytrace", "imageseq"), default="bclp",
                     help="bclp (default): fused BCLP at 256^3 x 360; osmo: plain forward/back projection and "
                          "OSMO at 256^3 x 360; batched: OSMO on 64 uneven problems, one batched call per projection; "
                          "raytrace: curved-ray tracer at 192^3 x 360 angles (RK4) and its BCLP iteration")
@@ -261,7 +262,8 @@ def main():
             else:
                 os.environ["OMP_NUM_THREADS"] = omp
     try:
-        {"bclp": run_bclp, "osmo": run_osmo, "batched": run_batched, "raytrace": run_raytrace}[args.workload](args)
+        {"bclp": run_bclp, "osmo": run_osmo, "batched": run_batched, "raytrace": run_raytrace,
+         "imageseq": run_imageseq}[args.workload](args)
     finally:
         if args.cpu_pool is not None:
             args.cpu_pool.terminate()
@@ -450,6 +452,22 @@ def run_bclp(args):
                 "limiter": "texture data path (TD busy %.0f %% of the kernel)" % (100 * tr["td_busy"])
                 if tr.get("td_busy") else None}
 
+    # ---- the same iteration through autograd: BCLPLoss forward (the fused
+    # epilogue kernel) and backward (the exact transpose of the back
+    # projection, wavefront-uniform kernel), then the clipped update ---------
+    from vamtoolbox_amd.autograd import BCLPLoss
+    g_ag = g.clone().requires_grad_(True)
+
+    def autograd_step():
+        g_ag.grad = None
+        L = BCLPLoss.apply(g_ag, bc.P, rm, bc.T, 0.05, 1.0, 2.0, 1.0)
+        L.backward()
+        with torch.no_grad():
+            g_ag.sub_(lr * g_ag.grad).clamp_(min=0.0)
+
+    autograd_ms = time_kernel(autograd_step)
+    log("autograd BCLP iteration (BCLPLoss forward + exact-adjoint backward + update): %.3f ms" % autograd_ms)
+
     roof_c = roof("bclp_fp_update_pair", upd_ms, upd_alg)
     roof_a = roof("bclp_bp_epilogue_wave", epi_ms, epi_alg)
     log("kernel C (FP + update) %.3f ms: %s GB/s measured traffic; kernel A (BP + epilogue) %.3f ms" % (
@@ -485,6 +503,7 @@ def run_bclp(args):
 
     # ---- CPU baseline ----------------------------------------------------------
     cpu = None
+    oracle_check = None
     if not args.no_cpu:
         log("CPU baseline (NumPy port, bounded sample)")
         tnp, gnp = bc.T.cpu().numpy().astype(np.float64), g.cpu().numpy()
@@ -493,6 +512,23 @@ def run_bclp(args):
         if args.cpu_pool is not None:
             cpu = cpu_baseline_all_cores(args.cpu_pool, args.cpu_workers, tnp, gnp, angles)
             cpu["one_core"] = {"value": one["value"], "sample": one["sample"]}
+            # the port's dose and unscaled gradient F(operand) on its slices
+            # against the GPU's at the same g (the oracle check of the timed
+            # iteration's two projections and epilogue at this geometry)
+            zs, res = cpu.pop("slices"), cpu.pop("results")
+            op_t = torch.empty(plan.vol_shape, device=dev)
+            dose_t = torch.empty(plan.vol_shape, device=dev)
+            _lib.check(lib.vamx_bclp_eval(plan.handle, ctypes.byref(resp), ctypes.byref(prm), g.data_ptr(),
+                                          bc.T.data_ptr(), 0, 0, dose_t.data_ptr(), op_t.data_ptr(),
+                                          l_out.data_ptr(), ws_t.data_ptr(), st), "vamx_bclp_eval")
+            fwd_t = plan.forward(op_t)
+            dg, fg = dose_t.cpu().numpy(), fwd_t.cpu().numpy()
+            dref = np.concatenate([r[0] for r in res], axis=2)
+            fref = np.concatenate([r[1] for r in res], axis=2)
+            e_d = float(np.linalg.norm(dg[:, :, zs] - dref) / np.linalg.norm(dref))
+            e_f = float(np.linalg.norm(fg[:, :, zs] - fref) / np.linalg.norm(fref))
+            oracle_check = {"rel_l2_dose": e_d, "rel_l2_gradient": e_f, "z_slices": zs}
+            log("GPU vs float64 port on %d slices: rel L2 dose %.2e, gradient %.2e" % (len(zs), e_d, e_f))
         else:
             cpu = one
         cpu["host"] = "%d host cores visible (%s)" % (os.cpu_count() or 0, _cpu_model())
@@ -527,6 +563,8 @@ def run_bclp(args):
         "backward_ms": bp_ms,
         "bp_epilogue_ms": epi_ms,
         "fp_update_ms": upd_ms,
+        "autograd_ms_per_iter": autograd_ms,
+        "rel_err_vs_oracle": oracle_check,
         "forward_gbs": fp_bytes / fp_ms / 1e6,
         "backward_gbs": bp_bytes / bp_ms / 1e6,
         "unfused_ms_per_iter": unf_ms,
@@ -540,6 +578,52 @@ def run_bclp(args):
         "tuning": tuning,
     }
     print(json.dumps(out), flush=True)
+
+
+def run_imageseq(args):
+    """Frames for a DLP projector: 360 frames of 1920 x 1080 from a 256 x 360 x
+    256 float64 sinogram, two arrayed copies, 99.5th-percentile normalisation
+    (ImageSeq, imagesequence.py:85-182).  The compose kernel writes every frame
+    byte once and reads each image sample once per copy."""
+    import torch
+    from vamtoolbox_amd.imagesequence import ImageConfig, ImageSeq
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    rng = np.random.default_rng(0)
+    s = torch.as_tensor(rng.uniform(0, 1, (N, NA, NZ)), device=dev)
+    cfg = ImageConfig((1920, 1080), array_num=2, array_offset=300, normalization_percentile=99.5)
+    ImageSeq(cfg, s)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        seq = ImageSeq(cfg, s)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / args.steps * 1e3
+    # the compose kernel alone, through the C ABI, on torch's stream
+    import ctypes
+    from vamtoolbox_amd import _lib
+    from vamtoolbox_amd.imagesequence import _windows
+    lib = _lib.load()
+    prm = _lib.ImgSeqParams()
+    prm.n_u, prm.n_v = 1920, 1080
+    wins = _windows(cfg, N, NZ)
+    prm.u1, prm.n_copies = wins[0][1], len(wins)
+    for k, (v1, _) in enumerate(wins):
+        prm.v1[k] = v1
+    prm.bit_depth, prm.normalize, prm.norm_value, prm.intensity_scale = 8, 1, 0.995, 1.0
+    frames = torch.empty((NA, 1080, 1920), device=dev, dtype=torch.uint8)
+    comp_ms = _time_events(lambda: _lib.check(lib.vamx_imgseq_compose(
+        s.data_ptr(), N, NA, NZ, ctypes.byref(prm), frames.data_ptr(), _lib.stream_ptr(dev)), "compose"), 10)
+    alg = NA * 1080 * 1920 + 8.0 * N * NA * NZ * len(wins)
+    log("ImageSeq: %.2f ms per sequence (percentile + compose); compose %.3f ms, %.0f GB/s algorithmic" % (
+        ms, comp_ms, alg / comp_ms / 1e6))
+    roof = {"bound": "hbm", "achieved": alg / comp_ms / 1e6, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": alg / comp_ms / 1e6 / HBM_PEAK_GBS, "traffic": None, "kernel": "imgseq_compose",
+            "kernel_ms": comp_ms, "algorithmic_bytes": alg}
+    _line("image sequences/s (360 frames of 1920x1080 from a 256x360x256 sinogram)", 1e3 / ms, "sequences/s", ms,
+          args, {"workload": "ImageSeq: 360 frames 1920x1080 uint8, 2 arrayed copies, 99.5th-percentile "
+                             "normalisation, float64 sinogram 256x360x256 resident on the GPU"}, roof, None,
+          compose_ms=comp_ms, frames_bytes=int(seq.frames.numel()))
 
 
 def _profile_table():

@@ -378,14 +378,50 @@ struct TapCell {
   float q[8];        // deposit: summed tap weights
 };
 
-__device__ __forceinline__ void tap_flush(const RtDev& g, const RtIo& io, const TapCell& tc, double scale) {
+// writes out the taps of `mask` (bit c = tap c) and clears them
+__device__ __forceinline__ void tap_flush(const RtDev& g, const RtIo& io, TapCell& tc, double scale,
+                                          unsigned mask = 0xffu) {
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
+    if (!((mask >> c) & 1u)) continue;
     const long long q = llrint((double)tc.q[c] * scale);
+    tc.q[c] = 0.0f;
     if (q == 0) continue;   // taps outside the grid carry weight 0
     const int xi = tc.x0 + (c & 1), yi = tc.y0 + ((c >> 1) & 1), zi = tc.z0 + ((c >> 2) & 1);
     atomicAdd(io.acc + ((size_t)xi * g.ny + yi) * g.nz + zi, (unsigned long long)q);
   }
+}
+
+// Moves the held cell by d (-1, 0 or +1) along axis AX (bit AX of a tap
+// index): the face the new cell shares with the old one keeps its sums, the
+// other face is written out.  |d| > 1: everything is written out.
+template <int AX>
+__device__ __forceinline__ void tap_shift(const RtDev& g, const RtIo& io, TapCell& tc, double scale, int d) {
+  constexpr unsigned lo = AX == 0 ? 0x55u : (AX == 1 ? 0x33u : 0x0fu);   // taps with bit AX = 0
+  constexpr int bit = 1 << AX;
+  if (d == 0) return;
+  if (d == 1) {
+    tap_flush(g, io, tc, scale, lo);
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+      if (!(c & bit)) {
+        tc.q[c] = tc.q[c | bit];
+        tc.q[c | bit] = 0.0f;
+      }
+  } else if (d == -1) {
+    tap_flush(g, io, tc, scale, lo ^ 0xffu);
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+      if (!(c & bit)) {
+        tc.q[c | bit] = tc.q[c];
+        tc.q[c] = 0.0f;
+      }
+  } else {
+    tap_flush(g, io, tc, scale);
+  }
+  if (AX == 0) tc.x0 += d;
+  else if (AX == 1) tc.y0 += d;
+  else tc.z0 += d;
 }
 
 // at most 128 VGPRs: 4 wavefronts per SIMD
@@ -468,13 +504,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void r
           integral += base * (double)tri_eval(tc.f, wx, wy, wz);
         } else if (MODE == RT_DEPOSIT) {
           if (key != tc.key) {
-            if (tc.key >= 0) tap_flush(g, io, tc, scale);
+            // the taps shared with the previous cell keep accumulating: about
+            // half of the atomics of a flush per cell change
+            if (tc.key >= 0) {
+              tap_shift<0>(g, io, tc, scale, tp.x0 - tc.x0);
+              tap_shift<1>(g, io, tc, scale, tp.y0 - tc.y0);
+              tap_shift<2>(g, io, tc, scale, tp.z0 - tc.z0);
+            }
             tc.key = key;
             tc.x0 = tp.x0;
             tc.y0 = tp.y0;
             tc.z0 = tp.z0;
-#pragma unroll
-            for (int c = 0; c < 8; ++c) tc.q[c] = 0.0f;
           }
           const float bf = (float)base;
 #pragma unroll
