@@ -431,11 +431,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void r
   const long long rid = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   // every lane runs to the end (no early return): the step statistics are
   // summed across the wavefront before one atomic per wavefront
-  bool live = rid < nrays;
-  const long long r = live ? rid : 0;
-  const int zr = (int)(r % g.nzr);
-  const long long ta = r / g.nzr;
-  const int a = (int)(ta % g.na), t = (int)(ta / g.na);
+  const bool valid = rid < nrays;
+  bool live = valid;
+  // launch order: z fastest, then t, then the angle, so that the rays in
+  // flight on one XCD (a contiguous range of blocks) are sheets of a few
+  // angles sweeping the volume together (their field reads and deposit
+  // atomics stay in a slab of cells); r is the ray's index in the sinogram
+  // (t, angle, z)
+  const long long q = valid ? rid : 0;
+  const int zr = (int)(q % g.nzr);
+  const long long qa = q / g.nzr;
+  const int t = (int)(qa % g.nt), a = (int)(qa / g.nt);
+  const long long r = ((long long)t * g.na + a) * g.nzr + zr;
   double E0 = (double)g.energy0;
   if (MODE == RT_DEPOSIT && live) {
     E0 = (double)io.gin[r];
@@ -542,16 +549,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void r
     }
     if (MODE == RT_DEPOSIT && tc.key >= 0) tap_flush(g, io, tc, scale);
   }
-  if (MODE == RT_INTEGRATE && rid < nrays) {
+  if (MODE == RT_INTEGRATE && valid) {
     if (io.upd_g) {
       // BCLP.py:486-487, 532: g <- clip(g - lr * scale * gradient)
       const float grad = (float)(*io.upd_scale) * (float)integral;
-      io.upd_g[rid] = fminf(fmaxf(io.upd_g[rid] - io.upd_lr * grad, io.upd_glb), io.upd_gub);
+      io.upd_g[r] = fminf(fmaxf(io.upd_g[r] - io.upd_lr * grad, io.upd_glb), io.upd_gub);
     } else {
-      io.gout[rid] = (float)integral;
+      io.gout[r] = (float)integral;
     }
   }
-  if (MODE == RT_COUNT && rid < nrays) io.counts[rid] = ntaps;
+  if (MODE == RT_COUNT && valid) io.counts[r] = ntaps;
   // wavefront sums of the step count and of the rays still active at max_steps
   unsigned long long st = live ? (unsigned long long)step : 0ull;
   unsigned long long un = active ? 1ull : 0ull;

@@ -114,8 +114,16 @@ def _percentile(t, q):
     if vi < 0:
         return float(torch.min(flat))
     lo = int(np.floor(vi))
-    a = float(torch.kthvalue(flat, lo + 1).values)
-    b = float(torch.kthvalue(flat, lo + 2).values)
+    # the order statistics lo and lo + 1 from the shorter tail (a multi-block
+    # radix selection), or from a full sort when neither tail is short
+    if lo + 2 <= 1 << 22:
+        a, b = (float(v) for v in torch.topk(flat, lo + 2, largest=False, sorted=True).values[lo:lo + 2])
+    elif n - lo <= 1 << 22:
+        top = torch.topk(flat, n - lo, largest=True, sorted=True).values
+        a, b = float(top[-1]), float(top[-2])
+    else:
+        srt = torch.sort(flat).values
+        a, b = float(srt[lo]), float(srt[lo + 1])
     g = vi - lo
     d = b - a
     return b - d * (1 - g) if g >= 0.5 else a + d * g
