@@ -938,6 +938,8 @@ def run_raytrace(args):
     losses = [float(fl[0]), float(fl[-1])]
     bytes_per_step = 72.0
     achieved = in_box * bytes_per_step / (fwd_ms * 1e-3) / 1e9
+    rt_tr = _profile_table().get("rt_trace_integrate", {})
+    traffic = rt_tr.get("bytes")
     cpu = None
     if not args.no_cpu:
         cpu = _raytrace_cpu_baseline(tg, pg, cv, nf, am, ab, na)
@@ -948,8 +950,10 @@ def run_raytrace(args):
                        "integrate fused with the update, replayed from a HIP graph",
            "n": n, "n_angles": na, "rays": int(192 * 192 * na)},
           {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-           "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "rt_trace_kernel<false>",
-           "kernel_ms": fwd_ms, "algorithmic_bytes": in_box * bytes_per_step},
+           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "rt_trace_kernel<0, RK4, canonical>",
+           "kernel_ms": fwd_ms, "algorithmic_bytes": in_box * bytes_per_step,
+           "traffic_gbs": traffic / (fwd_ms * 1e-3) / 1e9 if traffic else None,
+           "td_busy": rt_tr.get("td_busy"), "ta_busy": rt_tr.get("ta_busy")},
           cpu, integrate_ms=fwd_ms, deposit_ms=bwd_ms, ray_steps=in_box, ray_steps_per_s=in_box / (fwd_ms * 1e-3),
           unfinished_rays=int(steps_f[1]), loss_first=losses[0], loss_last=losses[-1], learning_rate=lr,
           unfused_ms_per_iter=unf_ms, speedup_vs_unfused=unf_ms / it_ms)
