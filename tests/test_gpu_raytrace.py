@@ -341,7 +341,7 @@ def test_autograd_raytrace_bclp_loss(gpu):
         L.backward()
         _, _, L_ref, operand = oo.bclp_state(g0, T, bwd, ro, 0.05, 1.0, p, q)
         G_ref = q * L_ref ** ((q - p) / q) * fwd(operand)
-        e_L = abs(float(L) - L_ref) / L_ref
+        e_L = abs(float(L.detach()) - L_ref) / L_ref
         e_G = rel_l2(gt.grad.cpu().numpy(), G_ref)
         h = 5e-2   # float64 oracle: the central difference is within 2.4e-3 of the gradient here
         with torch.no_grad():
@@ -351,6 +351,6 @@ def test_autograd_raytrace_bclp_loss(gpu):
         fd = (Lp - Lm) / (2 * h)
         an = float(np.dot(gt.grad.cpu().numpy().astype(np.float64), direction))
         print("p=%g q=%g: loss %.6e vs %.6e (rel %.1e), grad rel L2 %.2e, central difference %.5e vs %.5e" % (
-            p, q, float(L), L_ref, e_L, e_G, fd, an))
+            p, q, float(L.detach()), L_ref, e_L, e_G, fd, an))
         assert e_L <= 1e-5 and e_G <= 1e-4
         assert abs(fd - an) <= 1e-2 * abs(an)
