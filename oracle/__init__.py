@@ -64,6 +64,8 @@ PINNING = {
     "imageseq.frames / to_unsigned / percentile": "hand-computed bytes for truncation, clipping, 10-bit "
         "wrap modulo 256, 100th-percentile normalisation, flips (check_frames_known_answer); percentile is "
         "np.percentile",
+    "optim.zero_rays and the zero_sino branches of osmo / cal / fbp / bclp": "every back-projected and every "
+        "returned sinogram is zero on the masked rays; an empty mask changes nothing (check_zero_dose_rays)",
     "optim.discretize": "hand-computed levels incl. the index -1 wrap, brute force over 64 levels "
         "(check_discretize_known_answer)",
     "optim.osmo / cal / fbp / bclp with bit_depth": "one-bit two-level projections inside OSMO, returned "
@@ -104,5 +106,5 @@ PINNING = {
                                        "test_fan_beam_snell_and_centre_ray)",
 }
 """Every entry's check fails under a planted one-line mistake in the oracle:
-``tests/test_oracle_mutations.py`` plants 69 such mutations and asserts that
+``tests/test_oracle_mutations.py`` plants 76 such mutations and asserts that
 the named check catches each one."""

@@ -5,6 +5,8 @@ TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
 * ``fourier_filter`` / ``filter_sinogram``  vamtoolbox/util/data.py:302-438
 * ``filter_target_osmo``                    vamtoolbox/util/data.py:43-113
 * ``discretize``                            vamtoolbox/util/data.py:495-523
+* ``zero_rays``                             the in-place zeroing of the back projections
+                                            (Projector3DParallelCUDA.py:57-58)
 * ``Response``                              vamtoolbox/response.py:12-291
 * ``osmo``                                  vamtoolbox/optimizer/OSMO.py:9-178
 * ``cal``                                   vamtoolbox/optimizer/CAL.py:10-218
@@ -123,6 +125,15 @@ def discretize(x, bit_depth, lo, hi):
     return levels[k]
 
 
+def zero_rays(b, zero_sino):
+    """The sinogram as the reference's back projection leaves it: the rays
+    through the zero-dose body set to 0, in place in the reference
+    (Projector3DParallelCUDA.py:57-58, Projector3DParallel.py:43-44); a new
+    array here.  The loops below zero their sinogram before each back
+    projection, so the back projection and the rest of the loop see it zeroed."""
+    return b if zero_sino is None else np.where(zero_sino, 0.0, b)
+
+
 def circle_inds(target):
     n0, n1, n2 = target.shape
     cy, cx, _ = np.meshgrid(np.linspace(-1, 1, n0), np.linspace(-1, 1, n1), np.linspace(-1, 1, n2))
@@ -134,14 +145,15 @@ def ver(gel, void, x):
     return np.sum(x[void] >= np.min(x[gel])) / (np.sum(gel) + np.sum(void))
 
 
-def osmo(target, fwd, bwd, n_iter, d_h=0.8, d_l=0.7, filt="ram-lak", inhibition=0.0, bit_depth=None):
+def osmo(target, fwd, bwd, n_iter, d_h=0.8, d_l=0.7, filt="ram-lak", inhibition=0.0, bit_depth=None,
+         zero_sino=None):
     """OSMO.py:111-178 (no exit_param); ``bit_depth`` discretises each
     normalised projection over [min, max] (OSMO.py:61-64, 99-102) and the
     returned sinogram over [0, max] (OSMO.py:164-165)."""
     gel, void = circle_inds(target)
     err = np.full(n_iter, np.nan)
     xm = np.real(filter_target_osmo(target, filt))
-    b = np.clip(fwd(xm), 0, None)
+    b = zero_rays(np.clip(fwd(xm), 0, None), zero_sino)
     x = bwd(b)
     x = x / np.amax(x)
     err[0] = ver(gel, void, x)
@@ -152,6 +164,7 @@ def osmo(target, fwd, bwd, n_iter, d_h=0.8, d_l=0.7, filt="ram-lak", inhibition=
         if bit_depth is not None:
             b = discretize(b, bit_depth, np.min(b), np.max(b))
         b[b < inhibition] = inhibition
+        b = zero_rays(b, zero_sino)
         xu = bwd(b)
         xu = xu / np.amax(xu)
         xu[xu < 0] = 0
@@ -186,13 +199,14 @@ def cal_threshold(x, gel, void, sigmoid):
     return 1 / (1 + np.exp(-(float(sigmoid) * (x - th)))), th
 
 
-def cal(target, fwd, bwd, n_iter, d_h=0.8, filt="ram-lak", lr=0.01, momentum=0.0, sigmoid=0.01, bit_depth=None):
+def cal(target, fwd, bwd, n_iter, d_h=0.8, filt="ram-lak", lr=0.01, momentum=0.0, sigmoid=0.01, bit_depth=None,
+        zero_sino=None):
     """CAL.py:116-218: sigmoid threshold at d_h, or (d_h None) the score-maximising
     threshold of cal_threshold; ``bit_depth`` discretises each updated
     sinogram over [0, max] (CAL.py:171-173)."""
     gel, void = circle_inds(target)
     err = np.full(n_iter, np.nan)
-    b = np.clip(filter_sinogram(fwd(target), filt), 0, None)
+    b = zero_rays(np.clip(filter_sinogram(fwd(target), filt), 0, None), zero_sino)
     db_prev = np.zeros_like(b)
     x = bwd(b)
     x = x / np.max(x)
@@ -208,13 +222,14 @@ def cal(target, fwd, bwd, n_iter, d_h=0.8, filt="ram-lak", lr=0.01, momentum=0.0
         b = np.clip(b - lr * grad, 0, None)
         if bit_depth is not None:
             b = discretize(b, bit_depth, 0.0, np.max(b))
+        b = zero_rays(b, zero_sino)
         x = bwd(b)
         x = x / np.amax(x)
         db_prev = db
         b_opt = b
         err[it] = ver(gel, void, x)
     b_opt = np.clip(b_opt, 0, None)
-    b_opt = b_opt / np.amax(b_opt)
+    b_opt = b_opt / np.amax(b_opt)          # already zero on the zero-dose rays
     x_opt = bwd(b_opt)
     return b_opt, x_opt / np.amax(x_opt), err
 
@@ -232,7 +247,7 @@ def bclp_state(g, target, bwd, resp, eps, weight, p, q, dvol=1.0):
 
 
 def bclp(target, fwd, bwd, resp, n_iter, g0, eps=0.1, weight=1.0, p=2.0, q=1.0, lr=0.01, glb=0.0, gub=None,
-         grad_op=None, bit_depth=None):
+         grad_op=None, bit_depth=None, zero_sino=None):
     """BCLPNorm.gradientDescent (BCLP.py:477-526) from a given initial sinogram.
 
     ``grad_op`` (default ``fwd``) maps the operand to sinogram space.
@@ -247,12 +262,14 @@ def bclp(target, fwd, bwd, resp, n_iter, g0, eps=0.1, weight=1.0, p=2.0, q=1.0, 
     g = constrain(np.asarray(g0, dtype=np.float64))
     losses = np.full(n_iter + 1, np.nan)
     for k in range(n_iter):
+        g = zero_rays(g, zero_sino)
         dose, m, loss, operand = bclp_state(g, target, bwd, resp, eps, weight, p, q)
         losses[k] = loss
         G = grad_op(operand)
         if loss != 0.0:
             G = (q * loss ** ((q - p) / q)) * G
         g = constrain(g - lr * G)
+    g = zero_rays(g, zero_sino)
     dose, m, loss, _ = bclp_state(g, target, bwd, resp, eps, weight, p, q)
     losses[n_iter] = loss
     return g, dose, m, losses
@@ -265,7 +282,7 @@ def bclp_init(target, fwd, resp, n_angles, filt="ram-lak", glb=0.0, gub=None):
     return np.clip(g0, glb, gub)
 
 
-def fbp(target, fwd, bwd, filt="ram-lak", offset=False, bit_depth=None):
+def fbp(target, fwd, bwd, filt="ram-lak", offset=False, bit_depth=None, zero_sino=None):
     """FBP.py:42-63: filtered forward projection, non-negative, normalised
     (``bit_depth``: discretised over [0, max], FBP.py:56-57); dose normalised."""
     gel, void = circle_inds(target)
@@ -277,6 +294,7 @@ def fbp(target, fwd, bwd, filt="ram-lak", offset=False, bit_depth=None):
     b = b / np.max(b)
     if bit_depth is not None:
         b = discretize(b, bit_depth, 0.0, np.max(b))
+    b = zero_rays(b, zero_sino)
     x = bwd(b)
     x = x / np.max(x)
     return b, x, ver(gel, void, x)

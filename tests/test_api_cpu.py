@@ -199,3 +199,47 @@ def test_absorption_mask_known_answer():
                                             container_radius=2.0, projector_pixel_size=1.0)
     with pytest.raises(Exception):
         small.calcAbsorptionMask(tg)
+
+
+def test_volume_show_saves_an_image(tmp_path):
+    """Volume.show (geometry.py:276-306) draws the middle z slice and saves
+    it when given a path, as the reference's examples call it."""
+    import matplotlib
+    matplotlib.use("Agg")
+    vol = vam.geometry.Reconstruction(np.random.default_rng(0).uniform(size=(16, 16, 5)),
+                                      vam.geometry.ProjectionGeometry(np.arange(4.0), "parallel"))
+    out = tmp_path / "recon.png"
+    fig = vol.show(savepath=str(out), dpi=50)
+    assert out.exists() and out.stat().st_size > 0
+    assert np.array_equal(fig.axes[0].images[0].get_array(), vol.array[:, :, 2])
+
+
+def test_plot_helpers():
+    """The reference examples' plotting calls (response.py:294-361,
+    BCLP.py:94-139, medium.py:570-599, 918-947, geometry.py:578-628) run and
+    draw what they are asked for."""
+    import matplotlib
+    matplotlib.use("Agg")
+    import matplotlib.pyplot as plt
+    rm = vam.response.ResponseModel(form="linear")
+    fig, ax = plt.subplots()
+    rm.plotMap(fig, ax, label="linear", block=False)
+    rm2 = vam.response.ResponseModel(form="gen_log_fun", B=10)
+    rm2.plotMap(fig, ax, label="B=10", block=False)
+    x, y = ax.lines[1].get_data()
+    np.testing.assert_allclose(y, rm2.map(np.linspace(0, 1, 512)))
+    rm2.plotDmapDf(block=False)
+    rm2.plotMapInv(block=False)
+    tg = vam.geometry.TargetGeometry(target=np.ones((12, 12, 4)), zero_dose=np.zeros((12, 12, 4)))
+    cv = tg.constructCoordVec()
+    vam.medium.IndexModel(cv, type="interpolation", form="homogeneous", n_sur=1.5).plotIndex(block=False)
+    am = vam.medium.AttenuationModel(cv, type="analytical", form="homogeneous_cylinder", R=1, alpha_internal=0.2)
+    f, a = am.plotAlpha(block=False)
+    assert np.isclose(np.asarray(a.images[0].get_array()).max(), 0.2)
+    tg.show(show_bodies=True)
+    from vamtoolbox_amd.optimizer.BCLP import LogPerf
+    logs = LogPerf(vam.optimize.Options(method="BCLP", n_iter=4))
+    logs.loss[:] = [5, 4, 3, 2, 1]
+    fig, axs = logs.show()
+    assert len(axs) == 3
+    plt.close("all")

@@ -496,3 +496,53 @@ def test_response_map_autograd(gpu, kind):
     print("%s: map max rel %.2e, grad max rel %.2e" % (kind, e_m, e_g))
     # float64 evaluation, float32 storage: one rounding of the result
     assert e_m <= 2e-7 and e_g <= 5e-7
+
+
+@pytest.mark.parametrize("method", ["OSMO", "CAL", "FBP", "BCLP"])
+def test_zero_dose_loops_match_oracle(gpu, method):
+    """A zero-dose body (projectorconstructor.py:162-163): the reference's
+    back projection zeroes the sinogram's rays through it in place, so its
+    loops iterate on, and return, the zeroed sinogram.  The optimisers here
+    against the oracle loops that zero those rays before each back
+    projection; the returned sinogram is exactly zero on them (BCLP runs its
+    unfused loop with the constraint)."""
+    import oracle.d1 as d1
+    import oracle.optim as oo
+    import vamtoolbox_amd as vam
+    n, nz, na = 32, 4, 60
+    ang = np.linspace(0, 360, na, endpoint=False)
+    t = _ellipsoids(n, nz, 7, gray=(method == "BCLP"))
+    yy, xx = np.mgrid[:n, :n] - (n - 1) / 2
+    zd = np.repeat(((xx - 3) ** 2 + (yy + 2) ** 2 <= 9)[:, :, None], nz, axis=2).astype(float)
+    t = np.where(zd > 0, 0.0, t)
+    tg = vam.geometry.TargetGeometry(target=t, zero_dose=zd)
+    pg = vam.geometry.ProjectionGeometry(ang, "parallel", CUDA=True)
+    fwd, bwd = _oracle_ops(ang)
+    mask = d1.forward(zd, ang) != 0
+    if method == "BCLP":
+        rm = vam.response.ResponseModel(type="analytical", B=20, M=0.6)
+        ro = oo.Response("analytical", B=20, M=0.6)
+        g0 = 0.8 * oo.bclp_init(tg.array, fwd, oo.Response("interpolation", B=20, M=0.6), na)
+        opts = vam.optimize.Options(method="BCLP", n_iter=10, response_model=rm, eps=0.05, p=2, q=1,
+                                    learning_rate=1e-4, g0=g0)
+        g, dose, _, logs = vam.optimize.optimize(tg, pg, opts, output="components")
+        g_ref, dose_ref, _, _ = oo.bclp(tg.array, fwd, bwd, ro, 10, g0, eps=0.05, lr=1e-4, zero_sino=mask)
+        s_gpu, s_ref, x_gpu, x_ref = np.asarray(g).reshape(g_ref.shape), g_ref, dose, dose_ref
+    else:
+        if method == "OSMO":
+            opts = vam.optimize.Options(method="OSMO", n_iter=6, d_h=0.85, d_l=0.6)
+            s_ref, x_ref, _ = oo.osmo(tg.array, fwd, bwd, 6, d_h=0.85, d_l=0.6, zero_sino=mask)
+        elif method == "CAL":
+            opts = vam.optimize.Options(method="CAL", n_iter=6, learning_rate=0.002, sigmoid=0.01)
+            s_ref, x_ref, _ = oo.cal(tg.array, fwd, bwd, 6, lr=0.002, sigmoid=0.01, zero_sino=mask)
+        else:
+            opts = vam.optimize.Options(method="FBP")
+            s_ref, x_ref, _ = oo.fbp(tg.array, fwd, bwd, zero_sino=mask)
+        sino, recon, _ = vam.optimize.optimize(tg, pg, opts)
+        s_gpu, x_gpu = sino.array, recon.array
+    e_s, e_x = rel_l2(s_gpu, s_ref), rel_l2(x_gpu, x_ref)
+    print("%s with a zero-dose body: rel L2 sinogram %.2e dose %.2e; %d of %d rays zeroed" % (
+        method, e_s, e_x, mask.sum(), mask.size))
+    assert np.array_equal(np.asarray(pg.zero_dose_sino), mask)
+    assert mask.sum() > 0 and np.all(np.asarray(s_gpu)[mask] == 0)
+    assert e_s <= 1e-4 and e_x <= 1e-4

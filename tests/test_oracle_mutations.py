@@ -141,6 +141,18 @@ MUTATIONS = [
     ("oracle.optim", "filter_target_osmo", "f = np.sqrt(fx ** 2 + fy ** 2)", "f = fx ** 2 + fy ** 2", OPT,
      "test_filter_target_osmo_single_mode"),
     ("oracle.optim", "Response._dmap", "c2 = self.B / self.nu", "c2 = self.B", OPT, "test_response_closed_forms"),
+    # --- zero-dose rays (Projector3DParallelCUDA.py:57-58) ---
+    ("oracle.optim", "osmo", "        b = zero_rays(b, zero_sino)\n", "", LOOPS, "check_zero_dose_rays"),
+    ("oracle.optim", "osmo", "b = zero_rays(np.clip(fwd(xm), 0, None), zero_sino)", "b = np.clip(fwd(xm), 0, None)",
+     LOOPS, "check_zero_dose_rays"),
+    ("oracle.optim", "cal", "        b = zero_rays(b, zero_sino)\n", "", LOOPS, "check_zero_dose_rays"),
+    ("oracle.optim", "fbp", "b = zero_rays(b, zero_sino)", "pass", LOOPS, "check_zero_dose_rays"),
+    ("oracle.optim", "bclp", "        g = zero_rays(g, zero_sino)\n        dose, m, loss, operand",
+     "        dose, m, loss, operand", LOOPS, "check_zero_dose_rays"),
+    ("oracle.optim", "bclp", "    g = zero_rays(g, zero_sino)\n    dose, m, loss, _", "    dose, m, loss, _", LOOPS,
+     "check_zero_dose_rays"),
+    ("oracle.optim", "zero_rays", "np.where(zero_sino, 0.0, b)", "np.where(zero_sino, b, 0.0)", LOOPS,
+     "check_zero_dose_rays"),
     # --- image-sequence export (imagesequence.py:85-399) ---
     ("oracle.imageseq", "place", "extra = int(array_offset * d + v_offset)", "extra = int(array_offset * d)", IMG,
      "check_place_known_answer"),

@@ -511,3 +511,48 @@ def test_osmo_bit_depth():
 
 def test_cal_fbp_bclp_bit_depth():
     check_cal_fbp_bclp_bit_depth()
+
+
+# --------------------------------------------------------------------------
+# zero-dose rays (the reference's back projection zeroes them in its input)
+# --------------------------------------------------------------------------
+def check_zero_dose_rays():
+    """Every sinogram the loops back project is zero on the masked rays, and
+    so is the sinogram they return; an empty mask changes nothing."""
+    t, F, rng = _tiny_problem(8)
+    mask = rng.uniform(size=F.shape[0]) < 0.3
+    none = np.zeros_like(mask)
+    ident = lambda a: np.asarray(a, dtype=np.float64)   # noqa: E731
+
+    def runs(fn):
+        fwd, bwd = _pair(F, t.shape)
+        out = fn(fwd, bwd, mask)
+        for b_in in bwd.calls:
+            assert np.all(b_in[mask] == 0), "a back-projected sinogram is not zero on the zero-dose rays"
+        return out, fn(*_pair(F, t.shape), none), fn(*_pair(F, t.shape), None)
+
+    for name, fn in (("osmo", lambda f, b, m: oo.osmo(t, f, b, 3, filt=None, zero_sino=m)),
+                     # d_h near 1: the thresholded dose is ~0, the gradient F(-T) <= 0 and
+                     # the update raises the zeroed rays again before each back projection
+                     ("cal", lambda f, b, m: oo.cal(t, f, b, 3, d_h=0.99, filt=None, lr=0.01, zero_sino=m)),
+                     ("fbp", lambda f, b, m: oo.fbp(t, f, b, filt=None, zero_sino=m))):
+        out, empty, plain = runs(fn)
+        assert np.all(out[0][mask] == 0), name
+        np.testing.assert_array_equal(empty[0], plain[0])
+        np.testing.assert_array_equal(empty[1], plain[1])
+    T = rng.uniform(0.2, 0.8, F.shape[0])
+    g0 = T + 0.2
+    calls = []
+
+    def bwd(a):
+        calls.append(np.array(a))
+        return ident(a)
+    g, _, _, _ = oo.bclp(T, ident, bwd, IdentityResponse(), 3, g0, eps=0.0, p=2.0, q=2.0, lr=0.1, zero_sino=mask)
+    assert np.all(g[mask] == 0) and all(np.all(c[mask] == 0) for c in calls)
+    g_e, _, _, _ = oo.bclp(T, ident, ident, IdentityResponse(), 3, g0, eps=0.0, p=2.0, q=2.0, lr=0.1, zero_sino=none)
+    g_n, _, _, _ = oo.bclp(T, ident, ident, IdentityResponse(), 3, g0, eps=0.0, p=2.0, q=2.0, lr=0.1)
+    np.testing.assert_array_equal(g_e, g_n)
+
+
+def test_zero_dose_rays():
+    check_zero_dose_rays()

@@ -119,6 +119,25 @@ class Volume:
             self.nZ = slices[1] - slices[0] + 1
             self.resolution = self.nZ
 
+    def show(self, savepath=None, dpi="figure", transparent=False, **kwargs):
+        """A still image of the volume (geometry.py:276-306): the middle z
+        slice with matplotlib's imshow (keyword arguments passed on; colour
+        map "CMRmap" by default), saved to ``savepath`` when given.  The
+        reference's interactive slice viewers are display code outside this
+        package; this draws the one slice they open on."""
+        import matplotlib.pyplot as plt
+        kwargs.setdefault("cmap", "CMRmap")
+        kwargs.setdefault("interpolation", "antialiased")
+        a = np.asarray(self.array)
+        img = a if a.ndim == 2 else a[:, :, a.shape[2] // 2]
+        fig, ax = plt.subplots()
+        ax.imshow(img, **kwargs)
+        ax.set_title("%s%s" % (self.vol_type or "volume",
+                               "" if a.ndim == 2 else " (z = %d of %d)" % (a.shape[2] // 2, a.shape[2])))
+        if savepath is not None:
+            fig.savefig(savepath, dpi=dpi, transparent=transparent)
+        return fig
+
     def save(self, name: str):
         """Save the array, its type and the projection geometry's plain fields
         (angles, ray type, numbers and strings; model objects are not saved)
@@ -196,6 +215,20 @@ class TargetGeometry(Volume):
             from .util.data import clipToCircle
             array = clipToCircle(array)
         super().__init__(array=array, options=options, file_extension=".target", vol_type="target")
+
+    def show(self, show_bodies=False, savepath=None, dpi="figure", transparent=False, **kwargs):
+        """geometry.py:578-628: the target in gray (the insert and zero-dose
+        bodies, with show_bodies, as values 2 and 3 of the same image)."""
+        kwargs.setdefault("cmap", "gray")
+        kwargs.setdefault("interpolation", "none")
+        if not show_bodies or (self.insert is None and self.zero_dose is None):
+            return super().show(savepath=savepath, dpi=dpi, transparent=transparent, **kwargs)
+        a = np.array(self.array, dtype=np.float64)
+        for body, value in ((self.insert, 2.0), (self.zero_dose, 3.0)):
+            if body is not None:
+                a = np.where(np.broadcast_to(np.atleast_3d(np.asarray(body)), a.shape) > 0, value, a)
+        shown = Volume(a, vol_type="target")
+        return shown.show(savepath=savepath, dpi=dpi, transparent=transparent, **kwargs)
 
 
 class Sinogram(Volume):

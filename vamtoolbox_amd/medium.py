@@ -37,6 +37,21 @@ class MediumModel:
         self.shape = (self.xv.size, self.yv.size, self.zv.size)
         self.interpolation_padding_mode = "border"
 
+    def _plot_mid_slice(self, values, title, fig, ax, block):
+        """Middle z slice of a field on the grid (medium.py:570-599, 918-947)."""
+        import matplotlib.pyplot as plt
+        sl = np.asarray(values).reshape(self.shape)[:, :, self.shape[2] // 2]
+        if ax is None:
+            fig, ax = plt.subplots()
+        mappable = ax.imshow(sl, cmap="gray", vmin=np.amin(sl), vmax=np.amax(sl))
+        ax.set_title(title)
+        ax.set_xlabel("x")
+        ax.set_ylabel("y")
+        plt.colorbar(mappable)
+        if block:
+            plt.show(block=True)
+        return fig, ax
+
     def getPositionVectorsAtGridPoints(self):
         """(num_points, 3) positions of the grid points (medium.py:218-222)."""
         xg, yg, zg = np.meshgrid(self.xv, self.yv, self.zv, indexing="ij")
@@ -142,6 +157,10 @@ class IndexModel(MediumModel):
             return self._n_homo(x) if self.form == "homogeneous" else self._n_lune(x)
         return _grid_sample(self, self.presampled_scalar_field_3D[..., None], x, "border")[:, 0]
 
+    def plotIndex(self, fig=None, ax=None, block=False):
+        return self._plot_mid_slice(self.n(self.getPositionVectorsAtGridPoints()), "Index distribution", fig, ax,
+                                    block)
+
     def grad_n(self, x):
         x = np.asarray(x, dtype=np.float64)
         if self.type == "analytical":
@@ -216,6 +235,9 @@ class AttenuationModel(MediumModel):
         if self.type == "analytical":
             return self._alpha_homo_cylinder(x) if self.form == "homogeneous_cylinder" else self._alpha_homo_ball(x)
         return _grid_sample(self, self.presampled_scalar_field_3D[..., None], x, "zeros")[:, 0]
+
+    def plotAlpha(self, fig=None, ax=None, block=False):
+        return self._plot_mid_slice(self.alpha_at_grid(), "Attenuation coefficient", fig, ax, block)
 
     def alpha_at_grid(self):
         """alpha at the grid points, (nx, ny, nz) float64."""

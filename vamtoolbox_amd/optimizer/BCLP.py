@@ -60,6 +60,28 @@ class LogPerf:
                    header="n,compute time,loss,L0 norm,L1 norm,L2 norm,Linf norm,L0 norm eps=0,L1 norm eps=0,"
                           "L2 norm eps=0,Linf norm eps=0,")
 
+    def show(self, normalation_flags=(True, True, True, True)):
+        """The loss and the other norms over the iterations (BCLP.py:94-139),
+        three matplotlib panels; a norm is divided by its largest value when
+        its flag is set."""
+        import matplotlib.pyplot as plt
+        fig, axs = plt.subplots(1, 3)
+        fig.set_size_inches(12, 8)
+        axs[0].plot(self.loss)
+        axs[0].set(title="Weighted $L_{p}$ norm cost function", xlabel="Iteration", ylabel="Cost")
+        names = [r"$L_{0}$", r"$L_{1}$", r"$L_{2}$", r"$L_{\infty}$"]
+        for ax, keys, title in ((axs[1], ("l0", "l1", "l2", "lf"), r"Other $L_{p}$ norms with original tolerance"),
+                                (axs[2], ("l0_eps0", "l1_eps0", "l2_eps0", "lf_eps0"),
+                                 r"Other $L_{p}$ norms with zero tolerance")):
+            for k, name, norm in zip(keys, names, normalation_flags):
+                v = np.asarray(getattr(self, k), dtype=np.float64)
+                top = np.nanmax(np.abs(v)) if np.any(np.isfinite(v)) else 0.0
+                ax.plot(v / top if norm and top > 0 else v, label=name)
+            ax.set(title=title, xlabel="Iteration", ylabel="Normalized norm value")
+            ax.legend()
+        plt.show()
+        return fig, axs
+
     def startTiming(self):
         self.t0 = time.perf_counter()
 
@@ -103,7 +125,11 @@ class BCLPNorm:
             self.gradient = "reference"
         # fused iteration on the GPU: vamx_bclp_step (parallel beam) or
         # vamx_rt_bclp_step (ray tracer)
-        self.fused = bool(_opt(options, "fused", True)) and self.bit_depth is None
+        # (a zero-dose constraint changes the iterate between the two
+        # projections, see _zero_dose: the unfused loop carries it)
+        zero_dose = getattr(self.P, "imposeZeroDose", None) is not None and \
+            getattr(self.proj_geo, "zero_dose_sino", None) is not None
+        self.fused = bool(_opt(options, "fused", True)) and self.bit_depth is None and not zero_dose
         self.hip_graph = bool(_opt(options, "hip_graph", True))
         self.log_metrics = bool(_opt(options, "log_metrics", not self.fused))
         dev = self.device
@@ -142,6 +168,13 @@ class BCLPNorm:
 
     def _cyl(self, g):
         return g.reshape(self.g0_shape)
+
+    def _zero_dose(self, g):
+        """g with the zero-dose rays set to 0: the reference's back projection
+        zeroes them in g itself (Projector3DParallelCUDA.py:57-58), so its
+        update and its result continue from the zeroed sinogram."""
+        f = getattr(self.P, "imposeZeroDose", None)
+        return g if f is None else f(self._cyl(g)).reshape(g.shape)
 
     def _eval_state(self, g):
         """dose, mapped dose, errors, v, loss and operand of sinogram g (updateVariables)."""
@@ -202,6 +235,7 @@ class BCLPNorm:
         n_iter = self.logs.options.n_iter
         for k in range(n_iter):
             dose, mapped, eT, eb, v, loss, operand = self._eval_state(g)
+            g = self._zero_dose(g)
             L = float(loss.item())                                 # host branch of BCLP.py:370-375
             self.logs.loss[k] = L
             G = self._grad(operand, L)
@@ -218,7 +252,7 @@ class BCLPNorm:
                 self.logs.is_converged = True
                 break
         self._finish(g, self.logs.curr_iter)
-        return g
+        return self._zero_dose(g)
 
     def _vamx_args(self):
         resp, tabs = self.response_model.vamx_struct(self.device)

@@ -10,6 +10,7 @@ import torch
 from .. import geometry, metrics
 from ..projectorconstructor import projectorconstructor
 from ..util import data
+from .FBP import _zero_dose
 
 
 def _threshold_reconstruction(x, target_geo, sigmoid_strength):
@@ -43,6 +44,7 @@ def minimizeCAL(target_geo, proj_geo, options):
     b0 = b
     delta_b_prev = torch.zeros_like(b)
     x0 = A.backward(b0)
+    b0 = b = _zero_dose(A, b0)
     x0 = x0 / torch.amax(x0)
     x = x0
     _error[0] = metrics.calcVER(target_geo, x0)
@@ -63,6 +65,7 @@ def minimizeCAL(target_geo, proj_geo, options):
         if options.bit_depth is not None:
             b = data.discretize(b, options.bit_depth, [0.0, torch.amax(b)])
         x = A.backward(b)
+        b = _zero_dose(A, b)
         x = x / torch.amax(x)
         delta_b_prev = delta_b
         b_opt = b
@@ -75,6 +78,7 @@ def minimizeCAL(target_geo, proj_geo, options):
     b_opt = torch.clamp(b_opt, min=0.0)                          # CAL.py:375-378
     b_opt = b_opt / torch.amax(b_opt)
     x_opt = A.backward(b_opt)
+    b_opt = _zero_dose(A, b_opt)
     x_opt = x_opt / torch.amax(x_opt)
     options.iter_times = iter_times
     return (geometry.Sinogram(b_opt.double().cpu().numpy(), proj_geo, options),

@@ -18,6 +18,13 @@ from ..projectorconstructor import projectorconstructor
 from ..util import data
 
 
+def _zero_dose(A, b):
+    """The sinogram as the reference's back projection leaves it: zero on the
+    rays through the zero-dose body (parallel-beam projectors only)."""
+    f = getattr(A, "imposeZeroDose", None)
+    return b if f is None else f(b)
+
+
 def minimizeFBP(target_geo, proj_geo, options):
     A = projectorconstructor(target_geo, proj_geo)
     dev = A.device
@@ -35,6 +42,7 @@ def minimizeFBP(target_geo, proj_geo, options):
     if options.bit_depth is not None:
         b = data.discretize(b, options.bit_depth, [0.0, torch.amax(b)])
     x = A.backward(b.float())
+    b = _zero_dose(A, b)
     x = x / torch.amax(x)
     _error[0] = metrics.calcVER(target_geo, x)
     iter_times[0] = time.perf_counter() - t0

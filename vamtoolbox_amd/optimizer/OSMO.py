@@ -16,6 +16,7 @@ import torch
 from .. import geometry, metrics
 from ..projectorconstructor import projectorconstructor
 from ..util import data
+from .FBP import _zero_dose
 
 
 def minimizeOSMO(target_geo, proj_geo, options):
@@ -48,6 +49,7 @@ def minimizeOSMO(target_geo, proj_geo, options):
     x_model = (x_model.real if torch.is_complex(x_model) else x_model).to(torch.float32)
     b = torch.clamp(A.forward(x_model), min=0.0)                 # OSMO.py:133-134
     x = A.backward(b).reshape(T.shape)
+    b = _zero_dose(A, b)
     x = x / torch.amax(x)
     _error[0] = metrics.calcVER(target_geo, x)
     iter_times[0] = time.perf_counter() - t0
@@ -58,11 +60,13 @@ def minimizeOSMO(target_geo, proj_geo, options):
         x_model = torch.where(void, x_model - void_diff, x_model)
         b = project(x_model)
         x = reconstruct(b)
+        b = _zero_dose(A, b)
         # gel step (OSMO.py:73-109)
         gel_diff = torch.clamp(options.d_h - x, min=0.0)
         x_model = torch.where(gel, x_model + gel_diff, x_model)
         b = project(x_model)
         x = reconstruct(b)
+        b = _zero_dose(A, b)
         _error[curr_iter] = metrics.calcVER(target_geo, x)
         iter_times[curr_iter] = time.perf_counter() - t0
         if options.verbose in ("time", "plot"):

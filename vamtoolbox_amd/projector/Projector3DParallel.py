@@ -83,6 +83,18 @@ class Projector3DParallelHIP:
             self._zd_t = torch.as_tensor(np.array(np.broadcast_to(zd, self.sino_shape)), device=self.device)
         return self._zd_t
 
+    def imposeZeroDose(self, b):
+        """b with the rays through the zero-dose body set to 0 (a new tensor).
+        The reference's back projection zeroes them in its input array, in
+        place (Projector3DParallelCUDA.py:57-58, Projector3DParallel.py:43-44),
+        so its optimisers carry on -- and return -- the zeroed sinogram; they
+        call this after every back projection of their own sinogram."""
+        zd = self._zero_dose()
+        if zd is None:
+            return b
+        t = b if isinstance(b, torch.Tensor) else torch.as_tensor(np.asarray(b), device=self.device)
+        return torch.where(zd.reshape(t.shape), torch.zeros((), device=t.device, dtype=t.dtype), t)
+
     # -- operators --------------------------------------------------------
     def forward(self, x):
         """Sinogram of ``x`` (Projector3DParallel.py:113-156)."""

@@ -192,6 +192,31 @@ class ResponseModel:
             validity = False
         return validity
 
+    # ---- plots (response.py:294-361) ---------------------------------------
+    def _plot(self, fn, ylabel, fig, ax, lb, ub, n_pts, block, plot_kwargs, xlabel="Optical dose"):
+        import matplotlib.pyplot as plt
+        f = np.linspace(lb, ub, n_pts)
+        if ax is None:
+            fig, ax = plt.subplots()
+        ax.plot(f, fn(f), **plot_kwargs)
+        ax.set_xlabel(xlabel)
+        ax.set_ylabel(ylabel)
+        if block:
+            if "label" in plot_kwargs:
+                ax.legend()
+            plt.show(block=True)
+        return fig, ax
+
+    def plotMap(self, fig=None, ax=None, lb=0, ub=1, n_pts=512, block=True, **plot_kwargs):
+        return self._plot(self.map, "Material response (mapped dose)", fig, ax, lb, ub, n_pts, block, plot_kwargs)
+
+    def plotDmapDf(self, fig=None, ax=None, lb=0, ub=1, n_pts=512, block=True, **plot_kwargs):
+        return self._plot(self.dmapdf, "Derivative of the response", fig, ax, lb, ub, n_pts, block, plot_kwargs)
+
+    def plotMapInv(self, fig=None, ax=None, lb=0, ub=1, n_pts=512, block=True, **plot_kwargs):
+        return self._plot(self.map_inv, "Optical dose", fig, ax, lb, ub, n_pts, block, plot_kwargs,
+                          xlabel="Material response (mapped dose)")
+
     def vamx_struct(self, device):
         """Pack the model for the fused BCLP kernels (``vamx_response``)."""
         from . import _lib
