@@ -243,3 +243,29 @@ def test_plot_helpers():
     fig, axs = logs.show()
     assert len(axs) == 3
     plt.close("all")
+
+
+def test_metrics_known_answers():
+    """calcCV / calcPW / calcIPDR (metrics.py:6-167) by hand, NumPy and torch
+    (CPU) inputs alike; calcCV's layerwise branch repeats the volume value
+    as the reference's does."""
+    import torch
+    t = np.zeros((5, 5, 2))
+    t[1:4, 1:4, :] = 1.0
+    tg = vam.geometry.TargetGeometry(target=t)
+    recon = np.zeros((5, 5, 2))
+    recon[1:4, 1:4, 0] = [[0.8, 0.9, 1.0], [0.7, 0.9, 1.0], [0.9, 0.9, 0.9]]
+    recon[1:4, 1:4, 1] = 0.6
+    recon[2, 0, 0], recon[0, 2, 1] = 0.75, 0.65
+    g = recon[tg.gel_inds]
+    assert vam.metrics.calcCV(tg, recon) == pytest.approx(np.std(g) / np.mean(g), rel=1e-12)
+    # void inside the circle: max 0.75 (slice 0), 0.65 (slice 1); gel min 0.6
+    assert vam.metrics.calcPW(tg, recon) == pytest.approx(0.75 - 0.6, rel=1e-12)
+    assert vam.metrics.calcIPDR(tg, recon) == pytest.approx(1.0 - 0.6, rel=1e-12)
+    np.testing.assert_allclose(vam.metrics.calcPW(tg, recon, layerwise=True), [0.75 - 0.7, 0.65 - 0.6])
+    np.testing.assert_allclose(vam.metrics.calcIPDR(tg, recon, layerwise=True), [0.3, 0.0], atol=1e-15)
+    np.testing.assert_allclose(vam.metrics.calcCV(tg, recon, layerwise=True), np.std(g) / np.mean(g))
+    rt = torch.tensor(recon)
+    for fn in (vam.metrics.calcCV, vam.metrics.calcPW, vam.metrics.calcIPDR):
+        assert fn(tg, rt) == pytest.approx(fn(tg, recon), rel=1e-12)
+        np.testing.assert_allclose(fn(tg, rt, layerwise=True), fn(tg, recon, layerwise=True), rtol=1e-12)

@@ -1,4 +1,12 @@
-"""Optimisation metrics: ``calcVER`` (vamtoolbox/metrics.py:47-87)."""
+"""Optimisation metrics (vamtoolbox/metrics.py): the volumetric error rate
+``calcVER`` (47-87), the coefficient of variation ``calcCV`` (6-44), the
+process window ``calcPW`` (90-127) and the in-part dose range ``calcIPDR``
+(130-167), on NumPy arrays or device tensors.
+
+``layerwise``: per z slice with the slice's gel / void voxels.  The
+reference indexes each 2-D slice with the 3-D masks there, which NumPy
+rejects for volumes, except in ``calcCV``, whose layerwise branch returns the
+whole-volume value for every slice; ``calcCV`` keeps that."""
 
 from __future__ import annotations
 
@@ -56,3 +64,62 @@ def calcVER(target, recon, layerwise=False):
         min_gel_dose = np.min(sl[gel_inds[:, :, z]])
         out[z] = np.sum(sl[void_inds[:, :, z]] >= min_gel_dose) / num_gel_void
     return out
+
+
+def _gel_void_values(target, recon):
+    """(gel doses, void doses) as 1-D arrays or tensors."""
+    if torch is not None and isinstance(recon, torch.Tensor):
+        gt, vt, _, _ = device_masks(target, recon.device)
+        return recon[gt.reshape(recon.shape)], recon[vt.reshape(recon.shape)]
+    r = np.asarray(recon)
+    return r[np.asarray(target.gel_inds).reshape(r.shape)], r[np.asarray(target.void_inds).reshape(r.shape)]
+
+
+def _per_slice(target, recon, fn):
+    nz = recon.shape[2]
+    gel = np.asarray(target.gel_inds).reshape(recon.shape)
+    void = np.asarray(target.void_inds).reshape(recon.shape)
+    out = np.zeros(nz)
+    for z in range(nz):
+        sl = recon[:, :, z]
+        if torch is not None and isinstance(recon, torch.Tensor):
+            g = torch.as_tensor(gel[:, :, z], device=recon.device)
+            v = torch.as_tensor(void[:, :, z], device=recon.device)
+        else:
+            g, v = gel[:, :, z], void[:, :, z]
+        out[z] = fn(sl[g], sl[v])
+    return out
+
+
+def _f(x):
+    return float(x.item()) if hasattr(x, "item") else float(x)
+
+
+def calcCV(target, recon, layerwise=False):
+    """Coefficient of variation of the gel doses: std / mean (metrics.py:6-44;
+    population standard deviation, as np.std)."""
+    g, _ = _gel_void_values(target, recon)
+    if torch is not None and isinstance(g, torch.Tensor):
+        gd = g.double()
+        cv = _f(torch.sqrt(torch.mean((gd - torch.mean(gd)) ** 2)) / torch.mean(gd))
+    else:
+        cv = float(np.std(g) / np.mean(g))
+    if not layerwise:
+        return cv
+    return np.full(recon.shape[2], cv)
+
+
+def calcPW(target, recon, layerwise=False):
+    """Process window: max void dose - min gel dose (metrics.py:90-127)."""
+    fn = lambda g, v: _f(v.max()) - _f(g.min())   # noqa: E731
+    if not layerwise:
+        return fn(*_gel_void_values(target, recon))
+    return _per_slice(target, recon, fn)
+
+
+def calcIPDR(target, recon, layerwise=False):
+    """In-part dose range: max gel dose - min gel dose (metrics.py:130-167)."""
+    fn = lambda g, v: _f(g.max()) - _f(g.min())   # noqa: E731
+    if not layerwise:
+        return fn(*_gel_void_values(target, recon))
+    return _per_slice(target, recon, fn)
