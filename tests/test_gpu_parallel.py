@@ -500,6 +500,10 @@ def test_adjoint_wave_kernels_bitwise(gpu, case):
     bw, bl = wave.backward_adjoint(tv), lane.backward_adjoint(tv)
     assert torch.equal(fw, fl)
     assert torch.equal(bw, bl)
+    # B^T pairs opposite bins like the forward (one pass over the shared
+    # strip); unpaired it runs one bin per wavefront: bitwise the same
+    wave.set_tuning(fp_pair=0)
+    assert torch.equal(wave.backward_adjoint(tv), bl)
     z = [0, nz // 2, nz - 1]
     e1 = rel_l2(fw[:, :, z].cpu().numpy(), d1.forward_adjoint(sino[:, :, z], ang, **kw))
     e2 = rel_l2(bw[:, :, z].cpu().numpy(), d1.backward_adjoint(vol[:, :, z], ang, **kw))

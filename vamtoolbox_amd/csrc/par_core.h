@@ -589,6 +589,13 @@ __device__ __forceinline__ void adj_put(float4& w, int4& o, int m, float wk, int
   else { w.w = wk; o.w = off; }
 }
 
+__device__ __forceinline__ void adj_putw(float4& w, int m, float wk) {
+  if (m == 0) w.x = wk;
+  else if (m == 1) w.y = wk;
+  else if (m == 2) w.z = wk;
+  else w.w = wk;
+}
+
 template <int ZV>
 __device__ __forceinline__ typename VecT<ZV>::T adj_taps(const float* __restrict__ base, size_t stride, int zl,
                                                          const AdjSlot& sl, int rows, typename VecT<ZV>::T acc) {
@@ -651,6 +658,114 @@ __device__ __forceinline__ typename VecT<ZV>::T bpT_bin_wave(const ParDev& g, co
     acc = adj_taps<ZV>(vol, (size_t)g.nz, zl, sl, n - a0 < 64 ? n - a0 : 64, acc);
   }
   return acc;
+}
+
+// weight of voxel (x, y) in B^T of bin k at the angle (co, sn): bpT_bin's
+// arithmetic, 0 when the voxel is outside the bin's strip
+__device__ __forceinline__ float bpT_weight(const ParDev& g, double co, double sn, int k, double x, double y) {
+  const double t = x * co - y * sn;
+  int j, j1;
+  float wj;
+  if (!interp_node(g, t, j, j1, wj)) return 0.0f;
+  float wk = 0.0f;
+  if (j == k) wk += 1.0f - wj;
+  if (j1 == k) wk += wj;
+  if (wk == 0.0f) return 0.0f;
+  if (g.mu != 0.0) wk *= abs_weight(g, t, x * sn + y * co);
+  return wk;
+}
+
+// B^T of bin (k, i) and of the opposite bin (k2, i2): angle i2 is angle
+// i + 180 deg and k2 = 2c + 2 det_offset - k, so t2 = -t and the two strips
+// hold the same voxels.  Each voxel column of the strip is loaded once and
+// feeds both sums.  The partner's weights use its own angle's cos / sin; both
+// sums walk the primary's rows (columns) in order, which is the partner's own
+// order whenever the two angles agree on the row direction (all but exact
+// 45-degree multiples), so each sum is bpT_bin_wave's to the bit there.  Per
+// row the two strips have at most 3 voxels together (|t - t_k| < 1 on a row
+// where t moves >= 1/sqrt(2) per voxel; the partner's strip differs by the
+// pairing tolerance only), within the 4 slots.
+struct AdjSlot2 {
+  float4* w;   // [64] primary tap weights
+  float4* w2;  // [64] partner tap weights
+  int4* o;     // [64] tap voxel column, -1 = none
+};
+__device__ __forceinline__ AdjSlot2 adj_lds_slot2() {
+  __shared__ float4 w[4][64];
+  __shared__ float4 w2[4][64];
+  __shared__ int4 o[4][64];
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) & 3;
+  return {w[wv], w2[wv], o[wv]};
+}
+
+template <int ZV>
+__device__ __forceinline__ void bpT_pair_wave(const ParDev& g, const float* __restrict__ vol, int k, int i, int k2,
+                                              int i2, int zl, int lane, int clipc, typename VecT<ZV>::T& acc,
+                                              typename VecT<ZV>::T& acc2) {
+  const int n = g.n;
+  const size_t stride = (size_t)g.nz;
+  const double co = g.cs[2 * i], sn = g.cs[2 * i + 1];
+  const double co2 = g.cs[2 * i2], sn2 = g.cs[2 * i2 + 1];
+  const double xk = (double)(k - g.c) - g.det_offset;
+  const bool rows_major = fabs(co) >= fabs(sn);
+  const AdjSlot2 sl = adj_lds_slot2();
+  acc = vzero<ZV>();
+  acc2 = vzero<ZV>();
+  for (int a0 = 0; a0 < n; a0 += 64) {
+    const int a = a0 + lane;
+    float4 w = make_float4(0.f, 0.f, 0.f, 0.f), w2 = w;
+    int4 o = make_int4(-1, -1, -1, -1);
+    if (a < n) {
+      int b0, b1;
+      bpT_range(g, co, sn, xk, a, rows_major, b0, b1);
+      int m = 0;
+      for (int bb = b0; bb <= b1 && m < 4; ++bb) {
+        const int row = rows_major ? a : bb;
+        const int col = rows_major ? bb : a;
+        const int pix = row * n + col;
+        if (clipc && g.outside[pix]) continue;
+        const double x = (double)(col - g.c), y = (double)(row - g.c);
+        const float wk = bpT_weight(g, co, sn, k, x, y);
+        const float wk2 = bpT_weight(g, co2, sn2, k2, x, y);
+        if (wk == 0.0f && wk2 == 0.0f) continue;
+        adj_put(w, o, m, wk, pix);
+        adj_putw(w2, m, wk2);
+        ++m;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    sl.w[lane] = w;
+    sl.w2[lane] = w2;
+    sl.o[lane] = o;
+    __builtin_amdgcn_wave_barrier();
+    const int rows = n - a0 < 64 ? n - a0 : 64;
+    for (int r = 0; r < rows; ++r) {
+      const float4 wr = sl.w[r], wr2 = sl.w2[r];
+      const int4 orr = sl.o[r];
+      const int o0 = __builtin_amdgcn_readfirstlane(orr.x), o1 = __builtin_amdgcn_readfirstlane(orr.y);
+      const int o2 = __builtin_amdgcn_readfirstlane(orr.z), o3 = __builtin_amdgcn_readfirstlane(orr.w);
+      if (o0 >= 0) {
+        const typename VecT<ZV>::T v = vload<ZV>(vol + (size_t)o0 * stride + zl);
+        acc = vfma(wr.x, v, acc);
+        acc2 = vfma(wr2.x, v, acc2);
+      }
+      if (o1 >= 0) {
+        const typename VecT<ZV>::T v = vload<ZV>(vol + (size_t)o1 * stride + zl);
+        acc = vfma(wr.y, v, acc);
+        acc2 = vfma(wr2.y, v, acc2);
+      }
+      if (o2 >= 0) {
+        const typename VecT<ZV>::T v = vload<ZV>(vol + (size_t)o2 * stride + zl);
+        acc = vfma(wr.z, v, acc);
+        acc2 = vfma(wr2.z, v, acc2);
+      }
+      if (o3 >= 0) {
+        const typename VecT<ZV>::T v = vload<ZV>(vol + (size_t)o3 * stride + zl);
+        acc = vfma(wr.w, v, acc);
+        acc2 = vfma(wr2.w, v, acc2);
+      }
+    }
+  }
 }
 
 // A^T of voxel column pix: the detector bins of each angle whose samples

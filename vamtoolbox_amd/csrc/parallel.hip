@@ -316,6 +316,30 @@ __global__ __launch_bounds__(256) void par_bpT_wave_kernel(ParDev g, const float
   wave_zstore<ZV, false>(sino + ((size_t)k * g.na + i) * g.nz, z, g.nz, acc);
 }
 
+// B^T paired like the forward projection: one wavefront = bin (k, i) and,
+// when it exists, the opposite bin (k2, i2) x 64*ZV z, from one pass over the
+// strip's voxel columns (the forward's work units and tile order)
+template <int ZV>
+__global__ __launch_bounds__(256) void par_bpT_pair_kernel(ParDev g, const float* __restrict__ vol,
+                                                           float* __restrict__ sino, int clipc) {
+  using V = typename VecT<ZV>::T;
+  const int lane = threadIdx.x & 63;
+  const long long wave = wave_index(xcd_remap(blockIdx.x, gridDim.x));
+  const PairItem it = fp_pair_item<ZV, false>(wave, g, lane);
+  if (!it.ok) return;
+  int zl, zl2;
+  wave_zload<ZV, false>(it.z, g.nz, zl, zl2);
+  if (it.k2 < 0) {
+    const V acc = bpT_bin_wave<ZV>(g, vol, it.k, it.i, zl, lane, clipc);
+    wave_zstore<ZV, false>(sino + ((size_t)it.k * g.na + it.i) * g.nz, it.z, g.nz, acc);
+    return;
+  }
+  V acc, acc2;
+  bpT_pair_wave<ZV>(g, vol, it.k, it.i, it.k2, it.i2, zl, lane, clipc, acc, acc2);
+  wave_zstore<ZV, false>(sino + ((size_t)it.k * g.na + it.i) * g.nz, it.z, g.nz, acc);
+  wave_zstore<ZV, false>(sino + ((size_t)it.k2 * g.na + it.i2) * g.nz, it.z, g.nz, acc2);
+}
+
 __global__ __launch_bounds__(256) void par_fpT_incl_kernel(ParDev g, const float* __restrict__ sino,
                                                            float* __restrict__ vol) {
   const long long total = (long long)g.n * g.n * g.nz;
@@ -766,7 +790,13 @@ extern "C" int vamx_par_backward_adjoint(vamx_plan* plan, const float* vol, floa
   hipStream_t s = (hipStream_t)stream;
   const int clipc = (plan->flags & VAMX_PAR_NO_CIRCLE_CLIP) ? 0 : 1;
   const int wz = (!g.incl && !(plan->flags & VAMX_PAR_PER_LANE_GEOMETRY)) ? par_wave_zv(plan, vol, sino, 4) : 0;
-  if (wz) {
+  if (wz && g.fp_pair && g.pair_ok) {
+    const long long threads = fp_pair_tiled(g) * wave_zchunks(g.nz, wz) * 64;
+    const dim3 grid(blocks_for(threads, 256)), block(256);
+    if (wz == 4) hipLaunchKernelGGL((par_bpT_pair_kernel<4>), grid, block, 0, s, g, vol, sino, clipc);
+    else if (wz == 2) hipLaunchKernelGGL((par_bpT_pair_kernel<2>), grid, block, 0, s, g, vol, sino, clipc);
+    else hipLaunchKernelGGL((par_bpT_pair_kernel<1>), grid, block, 0, s, g, vol, sino, clipc);
+  } else if (wz) {
     const long long threads = (long long)g.n * g.na * wave_zchunks(g.nz, wz) * 64;
     const dim3 grid(blocks_for(threads, 256)), block(256);
     if (wz == 4) hipLaunchKernelGGL((par_bpT_wave_kernel<4>), grid, block, 0, s, g, vol, sino, clipc);
