@@ -467,6 +467,18 @@ def run_bclp(args):
 
     autograd_ms = time_kernel(autograd_step)
     log("autograd BCLP iteration (BCLPLoss forward + exact-adjoint backward + update): %.3f ms" % autograd_ms)
+    # the fused iteration with the exact gradient (grad_op 1: the transpose of
+    # the back projection fused with the update instead of the forward
+    # projection), eager launches
+    prm_x = _lib.BclpParams.from_buffer_copy(prm)
+    prm_x.grad_op = 1
+    g_x = g.clone()
+    log_x = torch.zeros(64, device=dev, dtype=torch.float64)
+    cnt_x = torch.zeros(1, device=dev, dtype=torch.int32)
+    exact_ms = time_kernel(lambda: (cnt_x.zero_(), _lib.check(lib.vamx_bclp_step(  # noqa
+        plan.handle, ctypes.byref(resp), ctypes.byref(prm_x), g_x.data_ptr(), bc.T.data_ptr(), 0, 0,
+        vol_out.data_ptr(), log_x.data_ptr(), cnt_x.data_ptr(), ws_t.data_ptr(), st), "vamx_bclp_step")))
+    log("fused BCLP iteration with the exact gradient (grad_op 1): %.3f ms" % exact_ms)
 
     roof_c = roof("bclp_fp_update_pair", upd_ms, upd_alg)
     roof_a = roof("bclp_bp_epilogue_wave", epi_ms, epi_alg)
@@ -564,6 +576,7 @@ def run_bclp(args):
         "bp_epilogue_ms": epi_ms,
         "fp_update_ms": upd_ms,
         "autograd_ms_per_iter": autograd_ms,
+        "exact_gradient_ms_per_iter": exact_ms,
         "rel_err_vs_oracle": oracle_check,
         "forward_gbs": fp_bytes / fp_ms / 1e6,
         "backward_gbs": bp_bytes / bp_ms / 1e6,

@@ -147,8 +147,9 @@ int vamx_bclp_workspace_bytes(vamx_plan* plan, int64_t* bytes);
  *   L = (sum v w |b|^p * dvol)^(q/p)      -> loss_log[*iter_counter]
  *   operand = v w |b|^(p-1) sign(e) map'(dose)
  *   g <- clip(g - lr * s * A operand, glb, gub),  s = q L^((q-p)/q) (1 if L == 0)
- * (A = forward projector, or B^T when prm->grad_op == 1)
- * and ++*iter_counter.  dose (device [V]) receives B g of the input g.
+ * (A = forward projector, or B^T when prm->grad_op == 1; grad_op must be 0
+ * or 1, and 1 returns VAMX_ERR_STATE for a plan with an occlusion map, whose
+ * exact transpose is not available) and ++*iter_counter.  dose (device [V]) receives B g of the input g.
  * eps_arr / weight_arr are optional device arrays of the volume's size.
  * iter_counter is a device int32; workspace has vamx_bclp_workspace_bytes.
  * No host synchronisation: the step can be captured in a HIP graph. */

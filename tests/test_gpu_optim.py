@@ -372,6 +372,65 @@ def test_bclp_eval_then_update_equals_step(gpu):
     assert not torch.equal(g1, g0)
 
 
+@pytest.mark.parametrize("nz", [64, 40, 256])
+def test_bclp_exact_gradient_step_kernels_agree(gpu, nz):
+    """vamx_bclp_step with grad_op = 1 (the exact transpose of the back
+    projection fused with the update): the wavefront kernel pairing opposite
+    bins equals the unpaired wavefront kernel bit for bit, and both agree with
+    the per-lane kernels (whose loss partials are summed in another order) to
+    float32 rounding, over three iterations; an occlusion plan is rejected."""
+    import ctypes
+    import torch
+    from vamtoolbox_amd import _lib
+    from vamtoolbox_amd.parallel import ParallelPlan
+    from vamtoolbox_amd.response import ResponseModel
+    rng = np.random.default_rng(9)
+    n, na = 40, 90
+    ang = np.linspace(0, 360, na, endpoint=False)
+    T = torch.tensor(np.clip(rng.random((n, n, nz)) * 1.2 - 0.1, 0, 1), dtype=torch.float32, device=gpu)
+    g0 = torch.tensor(rng.random((n, na, nz)) * 0.05, dtype=torch.float32, device=gpu)
+    lib = _lib.load()
+    resp, _ = ResponseModel(type="analytical", B=20, M=0.6).vamx_struct(gpu)
+    prm = _lib.BclpParams(p=2.0, q=1.0, eps=0.05, weight=1.0, learning_rate=1e-3, glb=0.0, gub=np.inf, dvol=1.0,
+                          grad_op=1)
+    st = _lib.stream_ptr(gpu)
+
+    def run(plan):
+        nbytes = ctypes.c_int64()
+        _lib.check(lib.vamx_bclp_workspace_bytes(plan.handle, ctypes.byref(nbytes)), "workspace")
+        ws = torch.empty(int(nbytes.value), device=gpu, dtype=torch.uint8)
+        g = g0.clone()
+        d = torch.empty(plan.vol_shape, device=gpu)
+        log = torch.zeros(4, device=gpu, dtype=torch.float64)
+        cnt = torch.zeros(1, device=gpu, dtype=torch.int32)
+        for _ in range(3):
+            _lib.check(lib.vamx_bclp_step(plan.handle, ctypes.byref(resp), ctypes.byref(prm), g.data_ptr(),
+                                          T.data_ptr(), 0, 0, d.data_ptr(), log.data_ptr(), cnt.data_ptr(),
+                                          ws.data_ptr(), st), "step")
+        torch.cuda.synchronize()
+        return g, log[:3].cpu().numpy()
+
+    kw = dict(mu=0.01, det_offset=3.0, device=gpu)
+    paired = ParallelPlan(n, nz, ang, **kw)
+    unpaired = ParallelPlan(n, nz, ang, **kw)
+    unpaired.set_tuning(fp_pair=0)
+    lane = ParallelPlan(n, nz, ang, per_lane_geometry=True, **kw)
+    gp, lp = run(paired)
+    gu, lu = run(unpaired)
+    gl, ll = run(lane)
+    assert not torch.equal(gp, g0)
+    assert torch.equal(gp, gu) and np.array_equal(lp, lu)
+    e = rel_l2(gp.cpu().numpy(), gl.cpu().numpy())
+    print("exact-gradient step: wave vs per-lane rel L2 %.2e, losses" % e, lp)
+    assert e <= 1e-6 and np.allclose(lp, ll, rtol=1e-9, atol=0)
+    insert = np.zeros((n, n, nz))
+    insert[n // 2 - 2:n // 2 + 2, n // 2 - 2:n // 2 + 2, :] = 1.0
+    occ = ParallelPlan(n, nz, ang, occlusion=insert, device=gpu)
+    assert occ.occluded
+    with pytest.raises(RuntimeError, match="occlusion"):
+        run(occ)
+
+
 def _on_levels_close(a, ref, bits, top, max_frac, max_levels=1):
     """Quantised iterates: float32 rounding moves a value across a level
     boundary now and then, so at most max_frac of the entries may differ, and

@@ -214,6 +214,60 @@ __global__ __launch_bounds__(256) void bclp_fp_update(ParDev g, const float* __r
   vstore<ZV>(gp, gv);
 }
 
+// g <- clip(g - lr * scale * grad) on a lane's z values of one sinogram row
+template <int ZV>
+__device__ __forceinline__ void wave_update_store(float* row, int z, int nz, typename VecT<ZV>::T gr, float scale,
+                                                  float lr, float glb, float gub) {
+  if (z >= nz) return;
+  typename VecT<ZV>::T gv = vload<ZV>(row + z);
+#pragma unroll
+  for (int q = 0; q < ZV; ++q) vset(gv, q, clampf(vget(gv, q) - lr * (scale * vget(gr, q)), glb, gub));
+  vstore<ZV>(row + z, gv);
+}
+
+// C' on wavefronts: the exact transpose of the back projection (one bin, or a
+// bin and its opposite bin, x 64*ZV z; the cores of par_bpT_wave_kernel /
+// par_bpT_pair_kernel) fused with the sinogram update; the same values as
+// the per-lane kernel
+template <int ZV, bool PAIR>
+__global__ __launch_bounds__(256) void bclp_bpT_update_wave(ParDev g, int clipc, const float* __restrict__ operand,
+                                                            float* __restrict__ gsino,
+                                                            const double* __restrict__ scale_p, float lr, float glb,
+                                                            float gub) {
+  using V = typename VecT<ZV>::T;
+  const int lane = threadIdx.x & 63;
+  const long long wave = wave_index(xcd_remap(blockIdx.x, gridDim.x));
+  int k, i, k2 = -1, i2 = -1, z;
+  if constexpr (PAIR) {
+    const PairItem it = fp_pair_item<ZV, false>(wave, g, lane);
+    if (!it.ok) return;
+    k = it.k;
+    i = it.i;
+    k2 = it.k2;
+    i2 = it.i2;
+    z = it.z;
+  } else {
+    const int nch = wave_zchunks(g.nz, ZV);
+    if (wave >= (long long)g.n * g.na * nch) return;   // whole wavefronts
+    const long long bin = wave / nch;
+    k = (int)(bin % g.n);
+    i = (int)(bin / g.n);
+    z = (int)(wave % nch) * 64 * ZV + lane * ZV;
+  }
+  int zl, zl2;
+  wave_zload<ZV, false>(z, g.nz, zl, zl2);
+  const float scale = (float)(*scale_p);
+  if (k2 < 0) {
+    const V gr = bpT_bin_wave<ZV>(g, operand, k, i, zl, lane, clipc);
+    wave_update_store<ZV>(gsino + ((size_t)k * g.na + i) * g.nz, z, g.nz, gr, scale, lr, glb, gub);
+    return;
+  }
+  V gr, gr2;
+  bpT_pair_wave<ZV>(g, operand, k, i, k2, i2, zl, lane, clipc, gr, gr2);
+  wave_update_store<ZV>(gsino + ((size_t)k * g.na + i) * g.nz, z, g.nz, gr, scale, lr, glb, gub);
+  wave_update_store<ZV>(gsino + ((size_t)k2 * g.na + i2) * g.nz, z, g.nz, gr2, scale, lr, glb, gub);
+}
+
 // C': exact transpose of the back projection fused with the sinogram update
 template <int ZV, bool INCL>
 __global__ __launch_bounds__(256) void bclp_bpT_update(ParDev g, int clipc, const float* __restrict__ operand,
@@ -361,11 +415,22 @@ extern "C" int vamx_bclp_workspace_bytes(vamx_plan* plan, int64_t* bytes) {
 
 namespace {
 
+// grad_op 0 (forward projector) or 1 (exact transpose of the back projection,
+// which does not model occlusion: rejected for plans with an occlusion map,
+// like vamx_par_backward_adjoint)
+int check_grad_op(const vamx_plan* plan, const vamx_bclp_params* prm) {
+  if (prm->grad_op != 0 && prm->grad_op != 1) return set_error(VAMX_ERR_ARG, "bclp: grad_op must be 0 or 1");
+  if (prm->grad_op == 1 && plan->par.occ)
+    return set_error(VAMX_ERR_STATE, "bclp: the exact gradient (grad_op 1) is not available with occlusion");
+  return VAMX_OK;
+}
+
 // Kernel C (with the slice min/max pre-pass of the warp clip): the update
 // g <- clip(g - lr * scale * A operand) with the device scale
 int bclp_update_impl(vamx_plan* plan, const vamx_bclp_params* prm, float* g, const float* operand,
                      const double* scale, hipStream_t s) {
-  int rc = VAMX_OK;
+  int rc = check_grad_op(plan, prm);
+  if (rc) return rc;
   const ParDev& gd = plan->par;
   const float lr = (float)prm->learning_rate, glb = (float)prm->glb, gub = (float)prm->gub;
   const bool v4 = !gd.incl && gd.nz % 4 == 0 && ((uintptr_t)g | (uintptr_t)operand) % 16 == 0;
@@ -436,6 +501,26 @@ int bclp_update_impl(vamx_plan* plan, const vamx_bclp_params* prm, float* g, con
                          lr, glb, gub);
   } else {
     const int clipc = (plan->flags & VAMX_PAR_NO_CIRCLE_CLIP) ? 0 : 1;
+    const int wz = (!gd.incl && !(plan->flags & VAMX_PAR_PER_LANE_GEOMETRY)) ? par_wave_zv(plan, g, operand, 4) : 0;
+    if (wz) {
+      const bool pair = gd.fp_pair && gd.pair_ok;
+      const long long waves = (pair ? fp_pair_tiled(gd) : (long long)gd.n * gd.na) * wave_zchunks(gd.nz, wz);
+      const dim3 grid(blocks_for(waves * 64, 256)), block(256);
+#define VAMX_BTU(Z, P) \
+  hipLaunchKernelGGL((bclp_bpT_update_wave<Z, P>), grid, block, 0, s, gd, clipc, operand, g, scale, lr, glb, gub)
+      if (pair) {
+        if (wz == 4) VAMX_BTU(4, true);
+        else if (wz == 2) VAMX_BTU(2, true);
+        else VAMX_BTU(1, true);
+      } else {
+        if (wz == 4) VAMX_BTU(4, false);
+        else if (wz == 2) VAMX_BTU(2, false);
+        else VAMX_BTU(1, false);
+      }
+#undef VAMX_BTU
+      VAMX_HIP(hipGetLastError());
+      return VAMX_OK;
+    }
     const long long total = (long long)gd.n * gd.na * (gd.nz / zv);
     const dim3 grid(blocks_for(total, 256)), block(256);
     if (gd.incl)
@@ -454,7 +539,9 @@ int bclp_step_impl(vamx_plan* plan, const RespDev& r, const BclpDev& b, const va
                    const float* target, float* dose, float* operand, double* partials, int max_parts,
                    double* scale, double* loss_log, int32_t* iter_counter, hipStream_t s) {
   int nparts = 0;
-  int rc = launch_epilogue(plan, r, b, g, target, dose, operand, partials, max_parts, nparts, s);
+  int rc = check_grad_op(plan, prm);
+  if (rc) return rc;
+  rc = launch_epilogue(plan, r, b, g, target, dose, operand, partials, max_parts, nparts, s);
   if (rc) return rc;
   hipLaunchKernelGGL(bclp_reduce, dim3(1), dim3(256), 0, s, partials, nparts, b.dvol, b.p, b.q, (double*)nullptr,
                      iter_counter, loss_log, scale);
