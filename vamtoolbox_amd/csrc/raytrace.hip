@@ -176,9 +176,9 @@ __device__ __forceinline__ void field_sample(const RtDev& g, d3 x, FieldCell& fc
 // boundary |x| < R is tested as |x|^2 < R^2
 __device__ __forceinline__ double att_alpha(const RtDev& g, d3 x) {
   if (g.att_kind == VAMX_RT_ATT_CYLINDER)
-    return x.x * x.x + x.y * x.y < g.att_radius * g.att_radius ? g.att_alpha : 0.0;
+    return x.x * x.x + x.y * x.y < g.att_r2 ? g.att_alpha : 0.0;
   if (g.att_kind == VAMX_RT_ATT_BALL)
-    return x.x * x.x + x.y * x.y + x.z * x.z < g.att_radius * g.att_radius ? g.att_alpha : 0.0;
+    return x.x * x.x + x.y * x.y + x.z * x.z < g.att_r2 ? g.att_alpha : 0.0;
   if (g.att_kind == VAMX_RT_ATT_GRID) {
     // grid_sample(align_corners=True, padding_mode="zeros")
     const double ix = (x.x - g.xmin) * g.inv_dx, iy = (x.y - g.ymin) * g.inv_dy;
@@ -309,11 +309,10 @@ __device__ __forceinline__ void ray_init(const RtDev& g, int t, int a, int zr, d
 
 // exit test (pyTorchRayTrace.py:1391-1420)
 __device__ __forceinline__ bool exited(const RtDev& g, d3 x, d3 v) {
-  const double vz = g.nz > 1 ? g.dz : INFINITY;
-  const bool pos = (x.x > g.xmax + g.dx && v.x > 0.0) || (x.y > g.ymax + g.dy && v.y > 0.0) ||
-                   (x.z > g.zmax + vz && v.z > 0.0);
-  const bool neg = (x.x < g.xmin - g.dx && v.x < 0.0) || (x.y < g.ymin - g.dy && v.y < 0.0) ||
-                   (x.z < g.zmin - vz && v.z < 0.0);
+  const bool pos = (x.x > g.exit_hi_x && v.x > 0.0) || (x.y > g.exit_hi_y && v.y > 0.0) ||
+                   (x.z > g.exit_hi_z && v.z > 0.0);
+  const bool neg = (x.x < g.exit_lo_x && v.x < 0.0) || (x.y < g.exit_lo_y && v.y < 0.0) ||
+                   (x.z < g.exit_lo_z && v.z < 0.0);
   return pos || neg;
 }
 
@@ -461,10 +460,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void r
   for (int c = 0; c < 8; ++c) tc.q[c] = 0.0f;
   d3 x, v;
   ray_init<SOLVER, PARAM, LUNE>(g, t, a, zr, x, v, fc);
-  const double hw = g.step / g.dx;  // step_size_idx_unit (pyTorchRayTrace.py:840-842)
+  const double hw = g.hw;
   const float e0f = (float)E0;
   double A = 0.0, integral = 0.0;
-  int step = 0;
+  int step = 0, until_exit = 1;
   bool active = live;
   if (live) {
     for (; step < g.max_steps; ++step) {
@@ -541,10 +540,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void r
           }
         }
       }
-      if (step % g.exit_every == 0 && exited(g, x, v)) {
-        active = false;
-        ++step;
-        break;
+      // the exit test runs at steps 0, exit_every, 2 exit_every, ...
+      if (--until_exit == 0) {
+        until_exit = g.exit_every;
+        if (exited(g, x, v)) {
+          active = false;
+          ++step;
+          break;
+        }
       }
     }
     if (MODE == RT_DEPOSIT && tc.key >= 0) tap_flush(g, io, tc, scale);
@@ -768,6 +771,14 @@ extern "C" int vamx_rt_create(const vamx_rt_geom* geom, vamx_plan** out_plan) {
   g.att_kind = geom->att_kind;
   g.att_alpha = geom->att_alpha;
   g.att_radius = geom->att_radius;
+  g.att_r2 = g.att_radius * g.att_radius;
+  g.hw = g.step / g.dx;   // step_size_idx_unit (pyTorchRayTrace.py:840-842)
+  {
+    const double vz = g.nz > 1 ? g.dz : INFINITY;   // exitCheck (pyTorchRayTrace.py:1391-1420)
+    g.exit_lo_x = g.xmin - g.dx; g.exit_hi_x = g.xmax + g.dx;
+    g.exit_lo_y = g.ymin - g.dy; g.exit_hi_y = g.ymax + g.dy;
+    g.exit_lo_z = g.zmin - vz; g.exit_hi_z = g.zmax + vz;
+  }
   g.att_grid = geom->att_grid;
   g.abs_grid = geom->abs_grid;
   // ray coordinates (setupRaysParallel, pyTorchRayTrace.py:1545-1574):

@@ -57,6 +57,12 @@ struct RtDev {
   int nt, na, nzr;
   double incl;               // radians
   double step;
+  // derived on the host once (kernel arguments in scalar registers, not
+  // per-lane copies): step / dx, the exit bounds min - d and max + d per axis
+  // (+-inf on z in 2-D), att_radius^2
+  double hw;
+  double exit_lo_x, exit_lo_y, exit_lo_z, exit_hi_x, exit_hi_y, exit_hi_z;
+  double att_r2;
   int max_steps, exit_every, solver, param;
   float energy0;
   const float4* field;       // (n, gx, gy, gz)
