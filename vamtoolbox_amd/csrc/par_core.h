@@ -1365,6 +1365,115 @@ __device__ __forceinline__ void bp_quad_masks(const ParDev& g, int r0, int c0, i
   }
 }
 
+// ---------------------------------------------------------------------------
+// A^T on voxel quads.  Within one angle the detector coordinates of a 2x2
+// quad's voxels lie within |cos| + |sin| <= sqrt(2) of each other, so the
+// candidate bins of the four voxels (fpT_voxel: floor(k*) - 1 .. + 2 each)
+// span at most 6 consecutive bins.  Lane l evaluates angle i0 + l for all
+// four voxels with fpT_voxel's arithmetic and writes a 6-bin x 4-voxel weight
+// table to LDS; the wavefront then loads each of the angle's sinogram rows
+// with a nonzero weight once for the four voxels.  Each voxel sums the same
+// nonzero terms in the same order as fpT_voxel (a zero weight adds an exact
+// zero), so the results are bitwise equal.
+// ---------------------------------------------------------------------------
+struct FpTQuadSlot {
+  float4* w;   // [64][6]: bin kbase + j -> weights of the quad's voxels 0..3
+  int2* kb;    // [64] first candidate bin of the angle, bit j = bin kbase + j has a nonzero weight
+};
+__device__ __forceinline__ FpTQuadSlot fpT_quad_slot() {
+  __shared__ float4 w[4][64 * 6];
+  __shared__ int2 kb[4][64];
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) & 3;
+  return {w[wv], kb[wv]};
+}
+
+template <int ZV>
+__device__ __forceinline__ void fpT_quad_wave(const ParDev& g, const float* __restrict__ sino, int r0, int c0,
+                                              unsigned inrange, int zl, int lane, typename VecT<ZV>::T acc[4]) {
+  using V = typename VecT<ZV>::T;
+  const int n = g.n;
+  const size_t zstride = (size_t)g.nz;
+  const FpTQuadSlot sl = fpT_quad_slot();
+#pragma unroll
+  for (int q = 0; q < 4; ++q) acc[q] = vzero<ZV>();
+  for (int i0 = 0; i0 < g.na; i0 += 64) {
+    const int i = i0 + lane;
+    // the lane's own table row: cleared, then filled with the nonzero weights
+    __builtin_amdgcn_wave_barrier();
+    float4* const wrow = sl.w + lane * 6;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) wrow[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+    float* const wf = reinterpret_cast<float*>(wrow);
+    int kbase = 0, mask = 0;
+    if (i < g.na) {
+      const double* M = g.mats + 6 * i;
+      const double co = M[0], sn = M[1];
+      int kbq[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const double X = (double)(c0 + (q & 1) - g.c), Y = (double)(r0 + (q >> 1) - g.c);
+        const double kstar = co * X - sn * Y + (double)g.c + g.det_offset;
+        kbq[q] = (int)floor(kstar) - 1;
+      }
+      kbase = min(min(kbq[0], kbq[1]), min(kbq[2], kbq[3]));
+#pragma unroll 1
+      for (int q = 0; q < 4; ++q) {
+        if (!((inrange >> q) & 1u)) continue;
+        const int row = r0 + (q >> 1), col = c0 + (q & 1);
+        const double X = (double)(col - g.c), Y = (double)(row - g.c);
+        const double rstar = sn * X + co * Y + (double)g.c;
+        const int rb = (int)floor(rstar) - 1;
+        for (int d = 0; d < 4; ++d) {
+          const int kk = kbq[q] + d;
+          if (kk < 0 || kk >= n) continue;
+          const double kp = (double)kk - g.det_offset;
+          const double a0 = M[0] * kp, a1 = M[3] * kp;
+          const double t = (double)(kk - g.c) - g.det_offset;
+          float wk = 0.0f;
+          for (int r = rb; r < rb + 4; ++r) {
+            if (r < 0 || r >= n) continue;
+            const double rr = (double)r;
+            const double colS = (a0 + M[1] * rr) + M[2];
+            const double rowS = (a1 + M[4] * rr) + M[5];
+            if (!(fabs(rowS - (double)row) < 1.0 && fabs(colS - (double)col) < 1.0)) continue;
+            const float wr = tap_weight_axis(rowS, row);
+            const float wc = tap_weight_axis(colS, col);
+            wk = __builtin_fmaf(wr * wc, abs_weight(g, t, (double)(r - g.c)), wk);
+          }
+          // slot kk - kbase is in 0..5: the four voxels' first bins differ by
+          // <= 2 (their k* lie within |cos| + |sin| <= sqrt(2))
+          const int j = kk - kbase;
+          if (wk != 0.0f && j < 6) {
+            wf[j * 4 + q] = wk;
+            mask |= 1 << j;
+          }
+        }
+      }
+    }
+    sl.kb[lane] = make_int2(kbase, mask);
+    __builtin_amdgcn_wave_barrier();
+    const int rows = g.na - i0 < 64 ? g.na - i0 : 64;
+    for (int a = 0; a < rows; ++a) {
+      const int2 km = sl.kb[a];
+      const int kb0 = __builtin_amdgcn_readfirstlane(km.x);
+      const int msk = __builtin_amdgcn_readfirstlane(km.y);
+      const int ia = i0 + a;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        // rows without a nonzero weight are skipped, among them every bin
+        // outside 0..n-1 (its weights stay 0)
+        if (!((msk >> j) & 1)) continue;
+        const float4 w = sl.w[a * 6 + j];
+        const V v = vload<ZV>(sino + ((size_t)(kb0 + j) * g.na + ia) * zstride + zl);
+        acc[0] = vfma(w.x, v, acc[0]);
+        acc[1] = vfma(w.y, v, acc[1]);
+        acc[2] = vfma(w.z, v, acc[2]);
+        acc[3] = vfma(w.w, v, acc[3]);
+      }
+    }
+  }
+}
+
 // host helpers implemented in parallel.hip
 int par_slice_minmax(vamx_plan* p, const float* vol, hipStream_t s);
 int par_forward_launch(vamx_plan* p, const float* vol, float* sino, hipStream_t s, bool warp_clip);
