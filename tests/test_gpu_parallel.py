@@ -476,6 +476,9 @@ ADJ_WAVE_CASES = [
     (24, 77, np.arange(0, 180, 2.5), dict(det_offset=-1.5)),
     (40, 64, np.linspace(0, 360, 65, endpoint=False), dict(mu=0.03)),
     (16, 40, np.arange(0, 360, 3.0), {}),
+    # odd detector shift 2 det_offset = 1 (opposite bins k and 2c + 1 - k) and
+    # 45-degree multiples, where cos and sin tie
+    (24, 128, np.arange(0, 360, 15.0), dict(mu=0.02, det_offset=0.5)),
 ]
 
 
