@@ -348,6 +348,16 @@ typedef struct vamx_imgseq_params {
 int vamx_imgseq_compose(const double* img, int64_t mu, int64_t na, int64_t mv, const vamx_imgseq_params* prm,
                         uint8_t* frames, void* stream);
 
+/* The order statistics np.percentile (linear method) interpolates between,
+ * for ImageSeq's percentile normalisation (np.percentile(mod_sinogram, q),
+ * imagesequence.py:128-131): out[0] = sorted(x)[rank], out[1] =
+ * sorted(x)[rank + 1] (sorted(x)[rank] when rank = n - 1), both NaN when x
+ * holds a NaN.  x and out (2 doubles) are device memory; workspace has
+ * vamx_order_stats_workspace_bytes and is 8-byte aligned.  Exact (a radix
+ * selection on the values' bits), no host synchronisation. */
+int vamx_order_stats_workspace_bytes(int64_t* bytes);
+int vamx_order_stats(const double* x, int64_t n, int64_t rank, double* out, void* workspace, void* stream);
+
 /* ---------------------------------------------------------------------------
  * Algebraic (sparse-matrix) propagation
  * ------------------------------------------------------------------------- */

@@ -94,3 +94,48 @@ def test_full_size_sequence(gpu):
         sub = s[:, j:j + 1, :] / norm * 255   # the oracle with the normalisation applied by hand
         ref = oi.frames(sub, 1920, 1080, array_num=2, array_offset=300, u_offset=40)
         np.testing.assert_array_equal(seq.frames[j].cpu().numpy(), ref[0])
+
+
+def test_percentile_order_statistics_exact(gpu):
+    """The percentile normalisation (np.percentile, imagesequence.py:128-131)
+    from vamx_order_stats: equal to numpy's value for every case (ties, signed
+    zeros, infinities, NaN, one and two elements, the q extremes, the
+    benchmark's 256 x 360 x 256 sinogram), and each order statistic equal to
+    the sorted array's entry."""
+    import torch
+    from vamtoolbox_amd.imagesequence import _order_stats, _percentile
+    rng = np.random.default_rng(7)
+    cases = [
+        rng.standard_normal(1000),
+        np.repeat(np.arange(10.0), 1001)[rng.permutation(10010)],
+        np.full(777, 3.25),
+        np.array([0.0, -0.0, 1.0, -1.0, -0.0, 0.0, 2.0]),
+        np.array([np.inf, -np.inf, 1.0, 5.0, -3.0]),
+        np.array([4.5]),
+        np.array([2.0, -7.0]),
+        rng.uniform(-1e300, 1e300, 4097),
+        np.concatenate([rng.uniform(0, 1, 500), rng.uniform(0, 1e-300, 500), [5e-324, -5e-324]]),
+    ]
+    for x in cases:
+        t = torch.tensor(x, dtype=torch.float64, device=gpu)
+        for q in (0, 0.1, 25, 50, 99.5, 100):
+            with np.errstate(invalid="ignore"):
+                ref = float(np.percentile(x, q))
+            got = _percentile(t, q)
+            assert got == ref or (np.isnan(got) and np.isnan(ref)), (x[:5], q, got, ref)
+        srt = np.sort(x)
+        for r in sorted({0, len(x) // 3, len(x) - 1}):
+            a, b = _order_stats(t, r)
+            assert a == srt[r] and b == srt[min(r + 1, len(x) - 1)], (r, a, b)
+    # an array starting 8 bytes past a 16-byte boundary (one double per load)
+    x = rng.standard_normal(3001)
+    t = torch.tensor(x, dtype=torch.float64, device=gpu)[1:]
+    assert t.data_ptr() % 16 == 8
+    for q in (0, 37.5, 99.5, 100):
+        assert _percentile(t, q) == float(np.percentile(x[1:], q))
+    x = rng.uniform(0, 1, 1000)
+    x[123] = np.nan
+    assert np.isnan(_percentile(torch.tensor(x, device=gpu), 50)) and np.isnan(np.percentile(x, 50))
+    big = torch.rand((256, 360, 256), dtype=torch.float64, device=gpu, generator=torch.Generator(gpu).manual_seed(3))
+    ref = float(np.percentile(big.cpu().numpy(), 99.5))
+    assert _percentile(big, 99.5) == ref

@@ -177,6 +177,8 @@ SIGNATURES = {
                                      c_void_p, ctypes.c_int64, ctypes.c_int64, c_void_p]),
     "vamx_imgseq_compose": (c_int32, [c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                       ctypes.POINTER(ImgSeqParams), c_void_p, c_void_p]),
+    "vamx_order_stats_workspace_bytes": (c_int32, [ctypes.POINTER(ctypes.c_int64)]),
+    "vamx_order_stats": (c_int32, [c_void_p, ctypes.c_int64, ctypes.c_int64, c_void_p, c_void_p, c_void_p]),
     "vamx_rt_record_counts": (c_int32, [_PLAN_P, c_void_p, c_void_p]),
     "vamx_rt_record": (c_int32, [_PLAN_P, c_void_p, c_void_p, c_void_p, c_void_p]),
     "vamx_sino_resample": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p,

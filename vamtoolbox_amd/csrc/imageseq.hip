@@ -20,6 +20,8 @@
 //                   axis, coalesced), converted to bytes, transposed through
 //                   LDS and written along u (the frame's fastest axis) as
 //                   dwords.
+#include <algorithm>
+
 #include "vamx_internal.h"
 
 namespace vamx {
@@ -175,11 +177,203 @@ __global__ __launch_bounds__(256) void imgseq_compose(ComposeDev d, const double
   }
 }
 
+// ---------------------------------------------------------------------------
+// Order statistics for np.percentile (imagesequence.py:128-131): the values
+// of rank r and r + 1 of a float64 array, by an exact 8-pass radix selection
+// on 64-bit keys whose unsigned order is the numeric order.  Pass p
+// histograms digit p (8 bits, most significant first) of the elements whose
+// higher digits equal the ones selected so far; a one-block launch then picks
+// the bucket holding the rank and leaves the digits and the rank within them
+// for the next pass.  The counts are
+// integers, so the result is exact and does not depend on the launch.
+// ---------------------------------------------------------------------------
+constexpr int OS_PASSES = 8;
+struct OsWork {
+  unsigned long long counts[OS_PASSES][256];
+  unsigned long long prefix[OS_PASSES + 1];   // digits selected before pass p
+  unsigned long long rem[OS_PASSES + 1];      // rank within the elements matching them
+  unsigned long long bucket[OS_PASSES + 1];   // elements matching them
+  unsigned long long nans;     // NaN elements (counted in pass 0)
+  unsigned long long min_gt;   // least key above the rank-r key (tail pass)
+};
+
+// negative values bit-inverted, others with the sign bit set: -0 sorts just
+// below +0, NaN beyond -inf / +inf (a NaN anywhere makes the result NaN)
+__device__ __forceinline__ unsigned long long os_key(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double os_value(unsigned long long k) {
+  const unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+  return __longlong_as_double((long long)b);
+}
+
+__global__ __launch_bounds__(256) void os_init(OsWork* w, long long rank, long long n) {
+  unsigned long long* c = &w->counts[0][0];
+  for (int i = threadIdx.x; i < OS_PASSES * 256; i += blockDim.x) c[i] = 0ull;
+  if (threadIdx.x == 0) {
+    w->prefix[0] = 0ull;
+    w->rem[0] = (unsigned long long)rank;
+    w->bucket[0] = (unsigned long long)n;
+    w->nans = 0ull;
+    w->min_gt = ~0ull;
+  }
+}
+
+// adds value v's digit to the block histogram when its higher digits match;
+// lanes sharing the first active lane's digit add once, together
+__device__ __forceinline__ void os_count(unsigned* h, double v, int pass, int shift, unsigned long long prefix,
+                                         unsigned long long& nan_count) {
+  if (pass == 0 && v != v) ++nan_count;
+  const unsigned long long k = os_key(v);
+  const bool match = pass == 0 || (k >> (shift + 8)) == (prefix >> (shift + 8));
+  const unsigned d = (unsigned)(k >> shift) & 255u;
+  const unsigned lead = __builtin_amdgcn_readfirstlane(match ? d : 256u);
+  const unsigned long long same = __ballot(match && d == lead);
+  if (match && d == lead) {
+    if (__lane_id() == (unsigned)__ffsll((long long)same) - 1u) atomicAdd(&h[d], (unsigned)__popcll(same));
+  } else if (match) {
+    atomicAdd(&h[d], 1u);
+  }
+}
+
+template <bool VEC>
+__global__ __launch_bounds__(256) void os_hist(const double* __restrict__ x, long long n, OsWork* w, int pass) {
+  __shared__ unsigned h[256];
+  const int t = threadIdx.x;
+  h[t] = 0u;
+  __syncthreads();
+  const unsigned long long prefix = w->prefix[pass];
+  const int shift = 56 - 8 * pass;
+  unsigned long long nan_count = 0ull;
+  const long long gid = (long long)blockIdx.x * blockDim.x + t;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  if (VEC) {
+    const double2* x2 = reinterpret_cast<const double2*>(x);
+    for (long long i = gid; i < n / 2; i += stride) {
+      const double2 v = x2[i];
+      os_count(h, v.x, pass, shift, prefix, nan_count);
+      os_count(h, v.y, pass, shift, prefix, nan_count);
+    }
+    if (gid == 0 && (n & 1)) os_count(h, x[n - 1], pass, shift, prefix, nan_count);
+  } else {
+    for (long long i = gid; i < n; i += stride) os_count(h, x[i], pass, shift, prefix, nan_count);
+  }
+  if (nan_count) atomicAdd(&w->nans, nan_count);
+  __syncthreads();
+  const unsigned c = h[t];
+  if (c) atomicAdd(&w->counts[pass][t], (unsigned long long)c);
+}
+
+// one block: the bucket of pass `pass` holding the rank (an inclusive scan of
+// its complete histogram), leaving the digits and the rank within them for the
+// next pass
+__global__ __launch_bounds__(256) void os_select(OsWork* w, int pass) {
+  __shared__ unsigned long long sc[256];
+  const int t = threadIdx.x;
+  const int shift = 56 - 8 * pass;
+  const unsigned long long prefix = w->prefix[pass];
+  const unsigned long long cnt = w->counts[pass][t];
+  sc[t] = cnt;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {
+    const unsigned long long add = t >= o ? sc[t - o] : 0ull;
+    __syncthreads();
+    sc[t] += add;
+    __syncthreads();
+  }
+  const unsigned long long incl = sc[t], excl = incl - cnt, rem = w->rem[pass];
+  if (excl <= rem && rem < incl) {   // exactly one digit
+    w->prefix[pass + 1] = prefix | ((unsigned long long)t << shift);
+    w->rem[pass + 1] = rem - excl;
+    w->bucket[pass + 1] = cnt;
+  }
+}
+
+// rank r + 1 is an equal key when the rank-r key's bucket holds it
+__device__ __forceinline__ bool os_need_next(const OsWork* w, long long n, long long rank) {
+  return rank + 1 < n && w->rem[OS_PASSES] + 1 >= w->bucket[OS_PASSES];
+}
+
+// least key above the rank-r key, when rank r + 1 is not an equal key
+template <bool VEC>
+__global__ __launch_bounds__(256) void os_tail(const double* __restrict__ x, long long n, long long rank, OsWork* w) {
+  __shared__ unsigned long long s_min[4];
+  if (!os_need_next(w, n, rank)) return;   // grid-uniform
+  const unsigned long long key = w->prefix[OS_PASSES];
+  unsigned long long m = ~0ull;
+  auto take = [&](double v) {
+    const unsigned long long k = os_key(v);
+    if (k > key && k < m) m = k;
+  };
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  if (VEC) {
+    const double2* x2 = reinterpret_cast<const double2*>(x);
+    for (long long i = gid; i < n / 2; i += stride) {
+      const double2 v = x2[i];
+      take(v.x);
+      take(v.y);
+    }
+    if (gid == 0 && (n & 1)) take(x[n - 1]);
+  } else {
+    for (long long i = gid; i < n; i += stride) take(x[i]);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long q = __shfl_xor(m, o);
+    m = q < m ? q : m;
+  }
+  if (__lane_id() == 0) s_min[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long b = s_min[0];
+    for (int i = 1; i < 4; ++i) b = s_min[i] < b ? s_min[i] : b;
+    if (b != ~0ull) atomicMin(&w->min_gt, b);
+  }
+}
+
+__global__ void os_finish(long long n, long long rank, const OsWork* w, double* out) {
+  const double a = os_value(w->prefix[OS_PASSES]);
+  const double b = os_need_next(w, n, rank) ? os_value(w->min_gt) : a;
+  const double nan = __longlong_as_double(0x7ff8000000000000ll);
+  out[0] = w->nans ? nan : a;
+  out[1] = w->nans ? nan : b;
+}
+
 }  // namespace
 
 }  // namespace vamx
 
 using namespace vamx;
+
+extern "C" int vamx_order_stats_workspace_bytes(int64_t* bytes) {
+  if (!bytes) return set_error(VAMX_ERR_ARG, "vamx_order_stats_workspace_bytes: null argument");
+  *bytes = (int64_t)sizeof(OsWork);
+  return VAMX_OK;
+}
+
+extern "C" int vamx_order_stats(const double* x, int64_t n, int64_t rank, double* out, void* workspace,
+                                void* stream) {
+  if (!x || !out || !workspace) return set_error(VAMX_ERR_ARG, "vamx_order_stats: null argument");
+  if (n < 1) return set_error(VAMX_ERR_ARG, "vamx_order_stats: empty array");
+  if (rank < 0 || rank >= n) return set_error(VAMX_ERR_ARG, "vamx_order_stats: rank outside 0 .. n-1");
+  if ((uintptr_t)workspace % 8) return set_error(VAMX_ERR_ARG, "vamx_order_stats: workspace must be 8-byte aligned");
+  hipStream_t s = (hipStream_t)stream;
+  OsWork* w = (OsWork*)workspace;
+  const long long blocks = std::min<long long>(2048, std::max<long long>(1, (n + 511) / 512));
+  hipLaunchKernelGGL(os_init, dim3(1), dim3(256), 0, s, w, (long long)rank, (long long)n);
+  const bool vec = (uintptr_t)x % 16 == 0;
+  for (int p = 0; p < OS_PASSES; ++p) {
+    if (vec) hipLaunchKernelGGL(os_hist<true>, dim3((unsigned)blocks), dim3(256), 0, s, x, (long long)n, w, p);
+    else hipLaunchKernelGGL(os_hist<false>, dim3((unsigned)blocks), dim3(256), 0, s, x, (long long)n, w, p);
+    hipLaunchKernelGGL(os_select, dim3(1), dim3(256), 0, s, w, p);
+  }
+  if (vec) hipLaunchKernelGGL(os_tail<true>, dim3((unsigned)blocks), dim3(256), 0, s, x, (long long)n, (long long)rank, w);
+  else hipLaunchKernelGGL(os_tail<false>, dim3((unsigned)blocks), dim3(256), 0, s, x, (long long)n, (long long)rank, w);
+  hipLaunchKernelGGL(os_finish, dim3(1), dim3(1), 0, s, (long long)n, (long long)rank, w, out);
+  VAMX_HIP(hipGetLastError());
+  return VAMX_OK;
+}
 
 extern "C" int vamx_imgseq_rotate(const double* in, int64_t nu, int64_t na, int64_t nv, int32_t flip_u,
                                   int32_t flip_v, const double* matrix, const double* offset, double* out,

@@ -102,28 +102,43 @@ def _rotation(shape_uv, angle_deg):
     return (int(out_shape[0]), int(out_shape[1])), rot, in_center - out_center
 
 
+_OS_WORK = {}
+
+
+def _order_stats(flat, rank):
+    """sorted(flat)[rank], sorted(flat)[rank + 1] of a float64 device vector
+    (vamx_order_stats: an exact radix selection; NaN when it holds a NaN)."""
+    import ctypes
+    from vamtoolbox_amd import _lib
+    lib = _lib.load()
+    dev = flat.device
+    ws = _OS_WORK.get(dev)
+    if ws is None:
+        nb = ctypes.c_int64()
+        _lib.check(lib.vamx_order_stats_workspace_bytes(ctypes.byref(nb)), "vamx_order_stats_workspace_bytes")
+        ws = _OS_WORK[dev] = torch.empty(int(nb.value), dtype=torch.uint8, device=dev)
+    out = torch.empty(2, dtype=torch.float64, device=dev)
+    _lib.check(lib.vamx_order_stats(flat.data_ptr(), flat.numel(), int(rank), out.data_ptr(), ws.data_ptr(),
+                                    _lib.stream_ptr(dev)), "vamx_order_stats")
+    a, b = out.tolist()
+    return a, b
+
+
 def _percentile(t, q):
-    """np.percentile(a, q) (linear method) of a float64 device tensor."""
-    flat = t.reshape(-1)
+    """np.percentile(a, q) (linear method) of a float64 device tensor: the
+    order statistics lo and lo + 1 on the device, numpy's interpolation on
+    the host."""
+    flat = t.reshape(-1).contiguous()
     n = flat.numel()
-    if bool(torch.isnan(flat).any()):
-        return float("nan")
     vi = (n - 1) * (np.float64(q) / 100)
-    if vi >= n - 1:
-        return float(torch.max(flat))
     if vi < 0:
-        return float(torch.min(flat))
-    lo = int(np.floor(vi))
-    # the order statistics lo and lo + 1 from the shorter tail (a multi-block
-    # radix selection), or from a full sort when neither tail is short
-    if lo + 2 <= 1 << 22:
-        a, b = (float(v) for v in torch.topk(flat, lo + 2, largest=False, sorted=True).values[lo:lo + 2])
-    elif n - lo <= 1 << 22:
-        top = torch.topk(flat, n - lo, largest=True, sorted=True).values
-        a, b = float(top[-1]), float(top[-2])
-    else:
-        srt = torch.sort(flat).values
-        a, b = float(srt[lo]), float(srt[lo + 1])
+        return _order_stats(flat, 0)[0]
+    # at q = 100 numpy interpolates the last element with itself (an
+    # infinite one gives NaN)
+    lo = min(int(np.floor(vi)), n - 1)
+    a, b = _order_stats(flat, lo)
+    if a != a:
+        return float("nan")
     g = vi - lo
     d = b - a
     return b - d * (1 - g) if g >= 0.5 else a + d * g
