@@ -94,17 +94,31 @@ def test_rotate_is_scipy():
     check_rotate_is_scipy()
 
 
-def test_percentile_two_order_statistics():
-    """vamtoolbox_amd.imagesequence._percentile (the product's device
-    percentile, run here on a CPU tensor) equals np.percentile's linear method
-    on the lerp branches, the ends and ties."""
+def test_percentile_two_order_statistics(monkeypatch):
+    """The host half of vamtoolbox_amd.imagesequence._percentile (the rank it
+    asks the device for and NumPy's linear-method lerp between the two order
+    statistics), with the device selection replaced by a sort: equal to
+    np.percentile on the lerp branches, the ends, ties and infinities."""
+    import warnings
     import torch
-    from vamtoolbox_amd.imagesequence import _percentile
+    import vamtoolbox_amd.imagesequence as iseq
+
+    def order_stats(flat, rank):   # vamx_order_stats' contract: NaN when any is NaN
+        srt = np.sort(flat.numpy())
+        if np.isnan(srt).any():
+            return float("nan"), float("nan")
+        return float(srt[rank]), float(srt[min(rank + 1, srt.size - 1)])
+
+    monkeypatch.setattr(iseq, "_order_stats", order_stats)
     rng = np.random.default_rng(2)
-    for a in (rng.uniform(-3, 7, 1001), rng.integers(0, 5, 400).astype(float), np.array([2.5])):
+    for a in (rng.uniform(-3, 7, 1001), rng.integers(0, 5, 400).astype(float), np.array([2.5]),
+              np.array([np.inf, -np.inf, 1.0, 5.0, -3.0]), np.array([3.0, np.nan, 1.0])):
         t = torch.as_tensor(a)
         for q in (0, 0.1, 12.5, 50, 62.5, 99.5, 99.99, 100):
-            assert _percentile(t, q) == np.percentile(a, q), (a.size, q)
+            with warnings.catch_warnings():
+                warnings.simplefilter("ignore")
+                got, ref = iseq._percentile(t, q), float(np.percentile(a, q))
+            assert got == ref or (np.isnan(got) and np.isnan(ref)), (a.size, q, got, ref)
 
 
 def test_image_config_and_windows():
