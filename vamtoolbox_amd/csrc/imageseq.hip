@@ -118,6 +118,7 @@ constexpr int kPitch = 68;   // bytes per LDS row: 17 dwords, so 64 lanes writin
 
 struct ComposeDev {
   int n_u, n_v, u1, n_copies, bit_depth, normalize, flip_u, flip_v;
+  int tile_u0, tile_v0;   // first tile of the launch (the copies' bounding box)
   double norm_value, max_value, intensity_scale;
   int v1[VAMX_IMGSEQ_MAX_COPIES];
 };
@@ -133,12 +134,35 @@ __device__ __forceinline__ unsigned char to_byte(const ComposeDev& d, double x) 
   return (unsigned char)((unsigned)x & 0xffu);
 }
 
+// zero frames: 16-byte stores (WIDE) or dword stores, grid-stride
+template <bool WIDE>
+__global__ __launch_bounds__(256) void imgseq_clear(unsigned char* __restrict__ frames, long long nbytes) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  if (WIDE) {
+    uint4* f = reinterpret_cast<uint4*>(frames);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nbytes / 16; i += stride)
+      f[i] = make_uint4(0u, 0u, 0u, 0u);
+  } else {
+    unsigned* f = reinterpret_cast<unsigned*>(frames);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nbytes / 4; i += stride) f[i] = 0u;
+  }
+}
+
 __global__ __launch_bounds__(256) void imgseq_compose(ComposeDev d, const double* __restrict__ img, long long mu,
                                                       long long na, long long mv, unsigned char* __restrict__ frames) {
   __shared__ __align__(16) unsigned char tile[kTile * kPitch];
-  const int u0 = blockIdx.x * kTile, v0 = blockIdx.y * kTile;
+  const int u0 = (d.tile_u0 + (int)blockIdx.x) * kTile, v0 = (d.tile_v0 + (int)blockIdx.y) * kTile;
   const long long j = blockIdx.z;
   const int t = threadIdx.x;
+  // a tile outside every copy of the image stays as imgseq_clear left it
+  // (block-uniform test; the grid spans the copies' bounding box)
+  bool empty = u0 + kTile <= d.u1 || u0 >= d.u1 + (int)mu;
+  if (!empty) {
+    empty = true;
+    for (int c = 0; c < d.n_copies; ++c)
+      if (v0 < d.v1[c] + (int)mv && v0 + kTile > d.v1[c]) empty = false;
+  }
+  if (empty) return;
   // load: lanes along v (the image's contiguous axis), 4 columns per pass
   {
     const int vl = t & 63;
@@ -184,8 +208,8 @@ __global__ __launch_bounds__(256) void imgseq_compose(ComposeDev d, const double
 // histograms digit p (8 bits, most significant first) of the elements whose
 // higher digits equal the ones selected so far; a one-block launch then picks
 // the bucket holding the rank and leaves the digits and the rank within them
-// for the next pass.  The counts are
-// integers, so the result is exact and does not depend on the launch.
+// for the next pass.  The counts are integers, so the result is exact and
+// does not depend on the launch.
 // ---------------------------------------------------------------------------
 constexpr int OS_PASSES = 8;
 struct OsWork {
@@ -432,9 +456,25 @@ extern "C" int vamx_imgseq_compose(const double* img, int64_t mu, int64_t na, in
   d.max_value = prm->bit_depth == 32 ? 4294967295.0 : (double)((1ull << prm->bit_depth) - 1ull);
   d.intensity_scale = prm->intensity_scale;
   for (int c = 0; c < VAMX_IMGSEQ_MAX_COPIES; ++c) d.v1[c] = c < prm->n_copies ? prm->v1[c] : 0;
-  const dim3 grid((prm->n_u + kTile - 1) / kTile, (prm->n_v + kTile - 1) / kTile, (unsigned)na);
-  hipLaunchKernelGGL(imgseq_compose, grid, dim3(256), 0, (hipStream_t)stream, d, img, (long long)mu, (long long)na,
-                     (long long)mv, frames);
+  // every frame byte zeroed once, then the tiles of the copies' bounding box
+  hipStream_t s = (hipStream_t)stream;
+  const long long nbytes = (long long)na * prm->n_v * prm->n_u;   // a multiple of 16 (n_u, n_v multiples of 4)
+  const long long cblocks = std::min<long long>(8192, (nbytes / 16 + 255) / 256);
+  if ((uintptr_t)frames % 16 == 0)
+    hipLaunchKernelGGL(imgseq_clear<true>, dim3((unsigned)cblocks), dim3(256), 0, s, frames, nbytes);
+  else
+    hipLaunchKernelGGL(imgseq_clear<false>, dim3((unsigned)cblocks), dim3(256), 0, s, frames, nbytes);
+  int vlo = prm->v1[0], vhi = prm->v1[0] + (int)mv;
+  for (int c = 1; c < prm->n_copies; ++c) {
+    vlo = std::min(vlo, prm->v1[c]);
+    vhi = std::max(vhi, prm->v1[c] + (int)mv);
+  }
+  d.tile_u0 = prm->u1 / kTile;
+  d.tile_v0 = vlo / kTile;
+  const int tu1 = (prm->u1 + (int)mu - 1) / kTile, tv1 = (vhi - 1) / kTile;
+  const dim3 grid(tu1 - d.tile_u0 + 1, tv1 - d.tile_v0 + 1, (unsigned)na);
+  hipLaunchKernelGGL(imgseq_compose, grid, dim3(256), 0, s, d, img, (long long)mu, (long long)na, (long long)mv,
+                     frames);
   VAMX_HIP(hipGetLastError());
   return VAMX_OK;
 }
