@@ -51,3 +51,20 @@ def test_error_path_without_gpu_work():
     assert b"null" in lib.vamx_last_error()
     rc = lib.vamx_par_forward(None, None, None, None)
     assert rc == -3
+
+
+def test_order_stats_argument_errors_without_gpu_work():
+    """vamx_order_stats validates its arguments before any launch."""
+    import ctypes
+    from vamtoolbox_amd import _lib
+    lib = _lib.load()
+    nb = ctypes.c_int64()
+    assert lib.vamx_order_stats_workspace_bytes(ctypes.byref(nb)) == 0 and nb.value >= 8 * 256 * 8
+    assert lib.vamx_order_stats_workspace_bytes(None) == -1
+    fake = ctypes.c_void_p(256)   # never dereferenced: every call below fails validation first
+    assert lib.vamx_order_stats(None, 10, 0, fake, fake, None) == -1
+    assert lib.vamx_order_stats(fake, 0, 0, fake, fake, None) == -1 and b"empty" in lib.vamx_last_error()
+    assert lib.vamx_order_stats(fake, 10, 10, fake, fake, None) == -1 and b"rank" in lib.vamx_last_error()
+    assert lib.vamx_order_stats(fake, 10, -1, fake, fake, None) == -1
+    assert lib.vamx_order_stats(fake, 10, 3, fake, ctypes.c_void_p(260), None) == -1
+    assert b"aligned" in lib.vamx_last_error()
