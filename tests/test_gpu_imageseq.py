@@ -119,9 +119,9 @@ def test_percentile_order_statistics_exact(gpu):
     for x in cases:
         t = torch.tensor(x, dtype=torch.float64, device=gpu)
         for q in (0, 0.1, 25, 50, 99.5, 100):
-            with np.errstate(invalid="ignore"):
+            with np.errstate(invalid="ignore"):   # inf - inf in the interpolation, as NumPy's
                 ref = float(np.percentile(x, q))
-            got = _percentile(t, q)
+                got = _percentile(t, q)
             assert got == ref or (np.isnan(got) and np.isnan(ref)), (x[:5], q, got, ref)
         srt = np.sort(x)
         for r in sorted({0, len(x) // 3, len(x) - 1}):
