@@ -563,8 +563,9 @@ def test_zero_dose_loops_match_oracle(gpu, method):
     back projection zeroes the sinogram's rays through it in place, so its
     loops iterate on, and return, the zeroed sinogram.  The optimisers here
     against the oracle loops that zero those rays before each back
-    projection; the returned sinogram is exactly zero on them (BCLP runs its
-    unfused loop with the constraint)."""
+    projection; the returned sinogram is exactly zero on them.  BCLP runs
+    its fused step (the iterate zeroed on those rays after every step) and
+    agrees with its unfused loop."""
     import oracle.d1 as d1
     import oracle.optim as oo
     import vamtoolbox_amd as vam
@@ -586,6 +587,12 @@ def test_zero_dose_loops_match_oracle(gpu, method):
                                     learning_rate=1e-4, g0=g0)
         g, dose, _, logs = vam.optimize.optimize(tg, pg, opts, output="components")
         g_ref, dose_ref, _, _ = oo.bclp(tg.array, fwd, bwd, ro, 10, g0, eps=0.05, lr=1e-4, zero_sino=mask)
+        opts_u = vam.optimize.Options(method="BCLP", n_iter=10, response_model=rm, eps=0.05, p=2, q=1,
+                                      learning_rate=1e-4, g0=g0, fused=False)
+        g_u, dose_u, _, logs_u = vam.optimize.optimize(tg, pg, opts_u, output="components")
+        e_fu = rel_l2(np.asarray(g), np.asarray(g_u))
+        print("BCLP zero-dose fused vs unfused rel L2 g %.2e" % e_fu)
+        assert e_fu <= 1e-6 and np.allclose(logs.loss, logs_u.loss, rtol=1e-6, atol=0)
         s_gpu, s_ref, x_gpu, x_ref = np.asarray(g).reshape(g_ref.shape), g_ref, dose, dose_ref
     else:
         if method == "OSMO":

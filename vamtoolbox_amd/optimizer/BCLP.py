@@ -124,12 +124,13 @@ class BCLPNorm:
             # the ray tracer's forward is already the exact transpose of its backward
             self.gradient = "reference"
         # fused iteration on the GPU: vamx_bclp_step (parallel beam) or
-        # vamx_rt_bclp_step (ray tracer)
-        # (a zero-dose constraint changes the iterate between the two
-        # projections, see _zero_dose: the unfused loop carries it)
+        # vamx_rt_bclp_step (ray tracer); a zero-dose constraint zeroes the
+        # iterate's rays through the zero-dose body after each step (see
+        # _zero_dose and _gradient_descent_fused)
         zero_dose = getattr(self.P, "imposeZeroDose", None) is not None and \
             getattr(self.proj_geo, "zero_dose_sino", None) is not None
-        self.fused = bool(_opt(options, "fused", True)) and self.bit_depth is None and not zero_dose
+        self.fused = bool(_opt(options, "fused", True)) and self.bit_depth is None
+        self.zero_dose = zero_dose
         self.hip_graph = bool(_opt(options, "hip_graph", True))
         self.log_metrics = bool(_opt(options, "log_metrics", not self.fused))
         dev = self.device
@@ -291,11 +292,22 @@ class BCLPNorm:
         resp, prm = self._vamx_args()
         eps_p = 0 if self.eps_t is None else self.eps_t.data_ptr()
         w_p = 0 if self.w_t is None else self.w_t.data_ptr()
+        # zero-dose rays: the unfused loop back-projects, updates and returns
+        # the sinogram zeroed on them (see _zero_dose); here g is zeroed on
+        # them before the first step and after every step, in place (a fill
+        # kernel, so the step stays graph-capturable), which gives every back
+        # projection the same input and the same final g
+        zd = self.P._zero_dose() if (self.parallel and self.zero_dose) else None
+        if zd is not None:
+            zd = zd.reshape(g.shape)
+            g.masked_fill_(zd, 0.0)
 
         def step():
             _lib.check(step_fn(handle, ctypes.byref(resp), ctypes.byref(prm), g.data_ptr(), T.data_ptr(), eps_p, w_p,
                                dose.data_ptr(), loss_log.data_ptr(), counter.data_ptr(), ws.data_ptr(),
                                _lib.stream_ptr(dev)), step_name)
+            if zd is not None:
+                g.masked_fill_(zd, 0.0)
 
         per_iter_host = self.exit_param is not None or self.log_metrics or self.verbose in ("time", "plot")
         use_graph = self.hip_graph and not per_iter_host and n_iter > 1
