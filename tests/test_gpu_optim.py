@@ -491,6 +491,13 @@ def test_bit_depth_matches_oracle(gpu, method):
         print("BCLP bit depth 8: rel L2 g %.2e dose %.2e" % (rel_l2(g, g_ref), rel_l2(dose, dose_ref)))
         _on_levels_close(g, g_ref, bits, g_ref.max(), 0.005)
         assert rel_l2(dose, dose_ref) <= 1e-3
+        # the fused step followed by the constraints is the unfused loop: the
+        # same iterates bit for bit (the loss sums in another order)
+        opts_u = vam.optimize.Options(method="BCLP", n_iter=n_iter, response_model=rm, eps=0.05, p=2, q=1,
+                                      learning_rate=lr, g0=g0, bit_depth=bits, fused=False)
+        g_u, dose_u, _, logs_u = vam.optimize.optimize(tg, pg, opts_u, output="components")
+        assert np.array_equal(np.asarray(g), np.asarray(g_u))
+        np.testing.assert_allclose(logs.loss[:n_iter], logs_u.loss[:n_iter], rtol=1e-12, atol=0)
         return
     tg, pg, ang = _problem(seed=4)
     fwd, bwd = _oracle_ops(ang)

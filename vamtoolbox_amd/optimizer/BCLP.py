@@ -14,10 +14,11 @@ tensors:
   (deposit) fused with response, band and Lp weighting and the loss
   partials, a fixed-order loss reduction, and the forward projection
   (integrate) fused with the sinogram update -- optionally replayed from a
-  captured HIP graph; the loss log stays on the device.
-* unfused: projector calls plus torch element-wise operations with the
-  reference's host-side branches (used for ``bit_depth``, and as the plain
-  GPU loop).
+  captured HIP graph; the loss log stays on the device.  With ``bit_depth``
+  the step's update is followed by imposeSinogramConstraints (eager, no
+  graph); with a zero-dose body by zeroing the iterate's rays through it.
+* unfused (``fused=False``): projector calls plus torch element-wise
+  operations with the reference's host-side branches, the plain GPU loop.
 * ``gradient="exact"``: the gradient is taken through the exact transpose of
   the back projector (the true gradient of the loss) instead of the forward
   projector the reference uses in its place (BCLP.py:372).
@@ -129,7 +130,7 @@ class BCLPNorm:
         # _zero_dose and _gradient_descent_fused)
         zero_dose = getattr(self.P, "imposeZeroDose", None) is not None and \
             getattr(self.proj_geo, "zero_dose_sino", None) is not None
-        self.fused = bool(_opt(options, "fused", True)) and self.bit_depth is None
+        self.fused = bool(_opt(options, "fused", True))
         self.zero_dose = zero_dose
         self.hip_graph = bool(_opt(options, "hip_graph", True))
         self.log_metrics = bool(_opt(options, "log_metrics", not self.fused))
@@ -263,6 +264,10 @@ class BCLPNorm:
             weight=float(self.weight) if self.w_t is None else 0.0, learning_rate=self.learning_rate,
             glb=-np.inf if self.glb is None else float(self.glb), gub=np.inf if self.gub is None else float(self.gub),
             dvol=float(self.dvol), grad_op=1 if self.gradient == "exact" else 0)
+        if self.bit_depth is not None:
+            # the step's update stays unbounded; imposeSinogramConstraints
+            # (discretize, then the bounds) follows it, as in the unfused loop
+            prm.glb, prm.gub = -np.inf, np.inf
         return resp, prm
 
     def _gradient_descent_fused(self):
@@ -306,11 +311,14 @@ class BCLPNorm:
             _lib.check(step_fn(handle, ctypes.byref(resp), ctypes.byref(prm), g.data_ptr(), T.data_ptr(), eps_p, w_p,
                                dose.data_ptr(), loss_log.data_ptr(), counter.data_ptr(), ws.data_ptr(),
                                _lib.stream_ptr(dev)), step_name)
+            if self.bit_depth is not None:
+                g.copy_(self.imposeSinogramConstraints(g))
             if zd is not None:
                 g.masked_fill_(zd, 0.0)
 
         per_iter_host = self.exit_param is not None or self.log_metrics or self.verbose in ("time", "plot")
-        use_graph = self.hip_graph and not per_iter_host and n_iter > 1
+        # (discretize's reductions stay out of graphs)
+        use_graph = self.hip_graph and not per_iter_host and n_iter > 1 and self.bit_depth is None
         k = 0
         if use_graph:
             step()  # first iteration eagerly (also warms up the kernels)
