@@ -359,6 +359,20 @@ int vamx_order_stats_workspace_bytes(int64_t* bytes);
 int vamx_order_stats(const double* x, int64_t n, int64_t rank, double* out, void* workspace, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * OSMO element-wise steps (vamtoolbox/optimizer/OSMO.py:36-110)
+ * ------------------------------------------------------------------------- */
+/* The model update of minimizeOSMO's stepVoid / stepGel on n voxels (device
+ * float32 x and x_model, uint8 mask): gel = 0: x_model -= max(x - d, 0), gel =
+ * 1: x_model += max(d - x, 0), on the voxels where mask is nonzero
+ * (OSMO.py:54-56, 92-94; NaN passes the bound as the reference's masked
+ * assignment lets it). */
+int vamx_osmo_model_update(const float* x, float* x_model, const uint8_t* mask, int64_t n, float d, int32_t gel,
+                           void* stream);
+/* v = v / *m, then v[v < lo] = lo, in place (the max normalisations of
+ * OSMO.py:60-65 and 67-69 with the device scalar m = max(v)). */
+int vamx_scale_clamp(float* v, int64_t n, const float* m, float lo, void* stream);
+
+/* ---------------------------------------------------------------------------
  * Algebraic (sparse-matrix) propagation
  * ------------------------------------------------------------------------- */
 /* y = P x for a CSR matrix P (n_rows x n_cols; device row_ptr [n_rows+1] int64,

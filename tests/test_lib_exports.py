@@ -68,3 +68,18 @@ def test_order_stats_argument_errors_without_gpu_work():
     assert lib.vamx_order_stats(fake, 10, -1, fake, fake, None) == -1
     assert lib.vamx_order_stats(fake, 10, 3, fake, ctypes.c_void_p(260), None) == -1
     assert b"aligned" in lib.vamx_last_error()
+
+
+def test_osmo_elementwise_argument_errors_without_gpu_work():
+    """vamx_osmo_model_update / vamx_scale_clamp validate their arguments and
+    launch nothing for an empty array."""
+    import ctypes
+    from vamtoolbox_amd import _lib
+    lib = _lib.load()
+    fake = ctypes.c_void_p(256)   # never dereferenced
+    assert lib.vamx_osmo_model_update(None, fake, fake, 10, 0.5, 0, None) == -1
+    assert lib.vamx_osmo_model_update(fake, fake, fake, -1, 0.5, 0, None) == -1
+    assert lib.vamx_osmo_model_update(fake, fake, fake, 0, 0.5, 1, None) == 0
+    assert lib.vamx_scale_clamp(fake, 10, None, 0.0, None) == -1
+    assert lib.vamx_scale_clamp(fake, -3, fake, 0.0, None) == -1
+    assert lib.vamx_scale_clamp(fake, 0, fake, 0.0, None) == 0

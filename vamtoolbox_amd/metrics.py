@@ -66,6 +66,17 @@ def calcVER(target, recon, layerwise=False):
     return out
 
 
+def calcVER_device(target, recon):
+    """calcVER of a device tensor (not layerwise) as a 0-d float64 device
+    tensor, without a host synchronisation: the same counts and division as
+    calcVER."""
+    gt, vt, _, _ = device_masks(target, recon.device)
+    if target._n_gel == 0:
+        raise ValueError("zero-size array to reduction operation minimum which has no identity")
+    min_gel = torch.amin(torch.where(gt, recon, float("inf")))
+    return torch.count_nonzero(vt & (recon >= min_gel)).double() / target._num_gv
+
+
 def _gel_void_values(target, recon):
     """(gel doses, void doses) as 1-D arrays or tensors."""
     if torch is not None and isinstance(recon, torch.Tensor):
