@@ -619,3 +619,33 @@ def test_zero_dose_loops_match_oracle(gpu, method):
     assert np.array_equal(np.asarray(pg.zero_dose_sino), mask)
     assert mask.sum() > 0 and np.all(np.asarray(s_gpu)[mask] == 0)
     assert e_s <= 1e-4 and e_x <= 1e-4
+
+
+def test_osmo_elementwise_kernels_equal_torch(gpu):
+    """vamx_osmo_model_update and vamx_scale_clamp equal the torch expressions
+    of OSMO.py:54-56, 92-94 and 60-69 bit for bit, NaN included."""
+    import torch
+    from vamtoolbox_amd import _lib
+    lib = _lib.load()
+    st = _lib.stream_ptr(gpu)
+    g = torch.Generator(device=gpu).manual_seed(4)
+    n = 100003
+    x = torch.rand(n, device=gpu, generator=g) * 1.4 - 0.2
+    x[17] = float("nan")
+    xm0 = torch.rand(n, device=gpu, generator=g) - 0.5
+    mask = torch.rand(n, device=gpu, generator=g) > 0.4
+    for d, gel in ((0.7, 0), (0.8, 1)):
+        xm = xm0.clone()
+        _lib.check(lib.vamx_osmo_model_update(x.data_ptr(), xm.data_ptr(), mask.data_ptr(), n, d, gel, st), "update")
+        ref = (torch.where(mask, xm0 + torch.clamp(d - x, min=0.0), xm0) if gel else
+               torch.where(mask, xm0 - torch.clamp(x - d, min=0.0), xm0))
+        torch.cuda.synchronize()
+        assert torch.equal(torch.nan_to_num(xm, nan=7.0), torch.nan_to_num(ref, nan=7.0))
+        assert bool(torch.isnan(xm[17])) == bool(mask[17])
+    for lo in (0.0, 0.05):
+        v = x.clone()
+        m = torch.amax(torch.nan_to_num(x, nan=0.0))
+        _lib.check(lib.vamx_scale_clamp(v.data_ptr(), n, m.data_ptr(), lo, st), "scale_clamp")
+        ref = torch.clamp(x / m, min=lo)
+        torch.cuda.synchronize()
+        assert torch.equal(torch.nan_to_num(v, nan=7.0), torch.nan_to_num(ref, nan=7.0)) and bool(torch.isnan(v[17]))
