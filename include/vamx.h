@@ -372,6 +372,15 @@ int vamx_osmo_model_update(const float* x, float* x_model, const uint8_t* mask, 
  * OSMO.py:60-65 and 67-69 with the device scalar m = max(v)). */
 int vamx_scale_clamp(float* v, int64_t n, const float* m, float lo, void* stream);
 
+/* calcVER (vamtoolbox/metrics.py:47-87, not layerwise) of a device float32
+ * dose of n voxels with device uint8 gel / void masks: *ver (device double)
+ * = #(void voxels with dose >= the least gel dose) / num, num = #gel + #void;
+ * 0 when a gel dose is NaN.  workspace has vamx_ver_workspace_bytes, 8-byte
+ * aligned.  No host synchronisation. */
+int vamx_ver_workspace_bytes(int64_t* bytes);
+int vamx_ver(const float* dose, const uint8_t* gel, const uint8_t* void_mask, int64_t n, int64_t num, double* ver,
+             void* workspace, void* stream);
+
 /* ---------------------------------------------------------------------------
  * Algebraic (sparse-matrix) propagation
  * ------------------------------------------------------------------------- */

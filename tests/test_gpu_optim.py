@@ -649,3 +649,32 @@ def test_osmo_elementwise_kernels_equal_torch(gpu):
         ref = torch.clamp(x / m, min=lo)
         torch.cuda.synchronize()
         assert torch.equal(torch.nan_to_num(v, nan=7.0), torch.nan_to_num(ref, nan=7.0)) and bool(torch.isnan(v[17]))
+
+
+def test_ver_kernel_equals_numpy(gpu):
+    """calcVER on a device dose (vamx_ver: the least gel dose, then the count
+    of void voxels at or above it) equals the NumPy path (metrics.py:47-87),
+    with ties at the least gel dose, signed zeros and a NaN gel dose (VER 0,
+    as np.min and >= give)."""
+    import torch
+    import vamtoolbox_amd as vam
+    from tests.test_gpu_optim import _ellipsoids
+    t = _ellipsoids(40, 12, seed=9)
+    tg = vam.geometry.TargetGeometry(target=t)
+    rng = np.random.default_rng(3)
+    for case in range(4):
+        d = rng.random(t.shape).astype(np.float32)
+        gel, void = np.asarray(tg.gel_inds), np.asarray(tg.void_inds)
+        d[gel] = np.float32(0.5) + np.float32(0.5) * d[gel]   # about half of the void voxels count
+        if case == 1:   # ties: void voxels equal to the least gel dose, and -0 / +0
+            vi = np.argwhere(void)
+            d[tuple(vi[:50].T)] = d[gel].min()
+            d[tuple(vi[50:60].T)] = -0.0
+        if case == 2:
+            d[tuple(np.argwhere(gel)[3])] = np.nan
+        if case == 3:
+            d[gel] = -0.0
+        ref = vam.metrics.calcVER(tg, d)
+        got = vam.metrics.calcVER(tg, torch.tensor(d, device=gpu))
+        print("case %d: VER %.6f vs %.6f" % (case, got, ref))
+        assert got == ref

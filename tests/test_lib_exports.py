@@ -83,3 +83,17 @@ def test_osmo_elementwise_argument_errors_without_gpu_work():
     assert lib.vamx_scale_clamp(fake, 10, None, 0.0, None) == -1
     assert lib.vamx_scale_clamp(fake, -3, fake, 0.0, None) == -1
     assert lib.vamx_scale_clamp(fake, 0, fake, 0.0, None) == 0
+
+
+def test_ver_argument_errors_without_gpu_work():
+    """vamx_ver validates its arguments before any launch."""
+    import ctypes
+    from vamtoolbox_amd import _lib
+    lib = _lib.load()
+    nb = ctypes.c_int64()
+    assert lib.vamx_ver_workspace_bytes(ctypes.byref(nb)) == 0 and nb.value >= 16
+    fake = ctypes.c_void_p(256)   # never dereferenced
+    assert lib.vamx_ver(None, fake, fake, 10, 5, fake, fake, None) == -1
+    assert lib.vamx_ver(fake, fake, fake, 0, 5, fake, fake, None) == -1
+    assert lib.vamx_ver(fake, fake, fake, 10, 0, fake, fake, None) == -1
+    assert lib.vamx_ver(fake, fake, fake, 10, 5, fake, ctypes.c_void_p(260), None) == -1

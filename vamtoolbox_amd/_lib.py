@@ -180,6 +180,9 @@ SIGNATURES = {
     "vamx_osmo_model_update": (c_int32, [c_void_p, c_void_p, c_void_p, ctypes.c_int64, ctypes.c_float, c_int32,
                                          c_void_p]),
     "vamx_scale_clamp": (c_int32, [c_void_p, ctypes.c_int64, c_void_p, ctypes.c_float, c_void_p]),
+    "vamx_ver_workspace_bytes": (c_int32, [ctypes.POINTER(ctypes.c_int64)]),
+    "vamx_ver": (c_int32, [c_void_p, c_void_p, c_void_p, ctypes.c_int64, ctypes.c_int64, c_void_p, c_void_p,
+                           c_void_p]),
     "vamx_order_stats_workspace_bytes": (c_int32, [ctypes.POINTER(ctypes.c_int64)]),
     "vamx_order_stats": (c_int32, [c_void_p, ctypes.c_int64, ctypes.c_int64, c_void_p, c_void_p, c_void_p]),
     "vamx_rt_record_counts": (c_int32, [_PLAN_P, c_void_p, c_void_p]),

@@ -38,6 +38,85 @@ __global__ __launch_bounds__(256) void scale_clamp(float* __restrict__ v, long l
 
 long long ew_blocks(long long n) { return std::min<long long>(4096, std::max<long long>(1, (n + 255) / 256)); }
 
+// ---------------------------------------------------------------------------
+// The volumetric error rate (metrics.py:47-87): #(void voxels with dose >=
+// the least gel dose) / (#gel + #void), as two grid passes over the dose with
+// integer atomics (order-independent, so the value does not depend on the
+// launch).  A NaN gel dose makes the least gel dose NaN and the count 0, as
+// the reference's np.min and >= comparison do.
+// ---------------------------------------------------------------------------
+struct VerWork {
+  unsigned min_key;         // least gel dose as an ordered key
+  unsigned nan_gel;         // a gel dose is NaN
+  unsigned long long count;
+};
+
+// float -> unsigned key with the float order (-0 just below +0)
+__device__ __forceinline__ unsigned ver_key(float v) {
+  const unsigned b = __float_as_uint(v);
+  return (b >> 31) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float ver_value(unsigned k) {
+  return __uint_as_float((k >> 31) ? (k & 0x7fffffffu) : ~k);
+}
+
+__global__ void ver_init(VerWork* w) {
+  w->min_key = 0xffffffffu;
+  w->nan_gel = 0u;
+  w->count = 0ull;
+}
+
+__global__ __launch_bounds__(256) void ver_min_gel(const float* __restrict__ d, const unsigned char* __restrict__ gel,
+                                                   long long n, VerWork* w) {
+  __shared__ unsigned s_min[4], s_nan[4];
+  unsigned m = 0xffffffffu, nan = 0u;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    if (!gel[i]) continue;
+    const float v = d[i];
+    if (v != v) nan = 1u;
+    else m = min(m, ver_key(v));
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    m = min(m, (unsigned)__shfl_xor((int)m, o));
+    nan |= (unsigned)__shfl_xor((int)nan, o);
+  }
+  if (__lane_id() == 0) {
+    s_min[threadIdx.x >> 6] = m;
+    s_nan[threadIdx.x >> 6] = nan;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned bm = s_min[0], bn = s_nan[0];
+    for (int i = 1; i < 4; ++i) {
+      bm = min(bm, s_min[i]);
+      bn |= s_nan[i];
+    }
+    if (bm != 0xffffffffu) atomicMin(&w->min_key, bm);
+    if (bn) atomicOr(&w->nan_gel, 1u);
+  }
+}
+
+__global__ __launch_bounds__(256) void ver_count_void(const float* __restrict__ d,
+                                                      const unsigned char* __restrict__ vd, long long n, VerWork* w) {
+  __shared__ unsigned long long s_c[4];
+  if (w->nan_gel || w->min_key == 0xffffffffu) return;   // grid-uniform: nothing counts
+  const float mg = ver_value(w->min_key);
+  unsigned long long c = 0ull;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    if (vd[i] && d[i] >= mg) ++c;
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if (__lane_id() == 0) s_c[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long b = s_c[0] + s_c[1] + s_c[2] + s_c[3];
+    if (b) atomicAdd(&w->count, b);
+  }
+}
+
+__global__ void ver_finish(const VerWork* w, double num, double* out) { *out = (double)w->count / num; }
+
 }  // namespace
 
 }  // namespace vamx
@@ -61,6 +140,28 @@ extern "C" int vamx_scale_clamp(float* v, int64_t n, const float* m, float lo, v
   if (n == 0) return VAMX_OK;
   hipLaunchKernelGGL(scale_clamp, dim3((unsigned)ew_blocks(n)), dim3(256), 0, (hipStream_t)stream, v, (long long)n, m,
                      lo);
+  VAMX_HIP(hipGetLastError());
+  return VAMX_OK;
+}
+
+extern "C" int vamx_ver_workspace_bytes(int64_t* bytes) {
+  if (!bytes) return set_error(VAMX_ERR_ARG, "vamx_ver_workspace_bytes: null argument");
+  *bytes = (int64_t)sizeof(VerWork);
+  return VAMX_OK;
+}
+
+extern "C" int vamx_ver(const float* dose, const uint8_t* gel, const uint8_t* void_mask, int64_t n, int64_t num,
+                        double* ver, void* workspace, void* stream) {
+  if (!dose || !gel || !void_mask || !ver || !workspace) return set_error(VAMX_ERR_ARG, "vamx_ver: null argument");
+  if (n < 1 || num < 1) return set_error(VAMX_ERR_ARG, "vamx_ver: empty volume or no gel / void voxels");
+  if ((uintptr_t)workspace % 8) return set_error(VAMX_ERR_ARG, "vamx_ver: workspace must be 8-byte aligned");
+  hipStream_t s = (hipStream_t)stream;
+  VerWork* w = (VerWork*)workspace;
+  const dim3 grid((unsigned)ew_blocks(n));
+  hipLaunchKernelGGL(ver_init, dim3(1), dim3(1), 0, s, w);
+  hipLaunchKernelGGL(ver_min_gel, grid, dim3(256), 0, s, dose, (const unsigned char*)gel, (long long)n, w);
+  hipLaunchKernelGGL(ver_count_void, grid, dim3(256), 0, s, dose, (const unsigned char*)void_mask, (long long)n, w);
+  hipLaunchKernelGGL(ver_finish, dim3(1), dim3(1), 0, s, (const VerWork*)w, (double)num, ver);
   VAMX_HIP(hipGetLastError());
   return VAMX_OK;
 }

@@ -44,8 +44,7 @@ def calcVER(target, recon, layerwise=False):
             raise ValueError("zero-size array to reduction operation minimum which has no identity")
         inf = float("inf")
         if not layerwise:
-            min_gel = torch.amin(torch.where(gt, recon, inf))
-            return float(torch.count_nonzero(vt & (recon >= min_gel)).item()) / num
+            return float(calcVER_device(target, recon).item())
         gt3 = gt if gt.ndim == 3 else gt[..., None]
         vt3 = vt if vt.ndim == 3 else vt[..., None]
         min_gel = torch.amin(torch.where(gt3, recon, inf), dim=(0, 1))          # per slice
@@ -66,15 +65,34 @@ def calcVER(target, recon, layerwise=False):
     return out
 
 
-def calcVER_device(target, recon):
+_VER_WORK = {}
+
+
+def calcVER_device(target, recon, out=None):
     """calcVER of a device tensor (not layerwise) as a 0-d float64 device
-    tensor, without a host synchronisation: the same counts and division as
-    calcVER."""
+    tensor (written into ``out`` when given), without a host
+    synchronisation: the same counts and division as calcVER, by the
+    library's vamx_ver for a float32 dose."""
     gt, vt, _, _ = device_masks(target, recon.device)
     if target._n_gel == 0:
         raise ValueError("zero-size array to reduction operation minimum which has no identity")
+    out = torch.empty((), dtype=torch.float64, device=recon.device) if out is None else out
+    if recon.dtype == torch.float32 and recon.is_contiguous() and gt.numel() == recon.numel() and \
+            gt.is_contiguous() and vt.is_contiguous():
+        import ctypes
+        from vamtoolbox_amd import _lib
+        lib = _lib.load()
+        ws = _VER_WORK.get(recon.device)
+        if ws is None:
+            nb = ctypes.c_int64()
+            _lib.check(lib.vamx_ver_workspace_bytes(ctypes.byref(nb)), "vamx_ver_workspace_bytes")
+            ws = _VER_WORK[recon.device] = torch.empty(int(nb.value), dtype=torch.uint8, device=recon.device)
+        _lib.check(lib.vamx_ver(recon.data_ptr(), gt.data_ptr(), vt.data_ptr(), recon.numel(), target._num_gv,
+                                out.data_ptr(), ws.data_ptr(), _lib.stream_ptr(recon.device)), "vamx_ver")
+        return out
     min_gel = torch.amin(torch.where(gt, recon, float("inf")))
-    return torch.count_nonzero(vt & (recon >= min_gel)).double() / target._num_gv
+    out.copy_(torch.count_nonzero(vt & (recon >= min_gel)).double() / target._num_gv)
+    return out
 
 
 def _gel_void_values(target, recon):

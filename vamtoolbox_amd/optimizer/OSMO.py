@@ -82,7 +82,7 @@ def minimizeOSMO(target_geo, proj_geo, options):
             _error[k] = metrics.calcVER(target_geo, x)
             iter_times[k] = time.perf_counter() - t0
         else:
-            ver_log[k] = metrics.calcVER_device(target_geo, x)
+            metrics.calcVER_device(target_geo, x, out=ver_log[k])
             ev = torch.cuda.Event(enable_timing=True)
             ev.record()
             events.append(ev)
