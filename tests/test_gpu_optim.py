@@ -659,10 +659,12 @@ def test_ver_kernel_equals_numpy(gpu):
     import torch
     import vamtoolbox_amd as vam
     from tests.test_gpu_optim import _ellipsoids
-    t = _ellipsoids(40, 12, seed=9)
-    tg = vam.geometry.TargetGeometry(target=t)
     rng = np.random.default_rng(3)
-    for case in range(4):
+    # 40 x 40 x 12 voxels: four per step; 33 x 33 x 7 (odd count): one per step
+    for case, (n, nz) in enumerate([(40, 12)] * 4 + [(33, 7)] * 4):
+        t = _ellipsoids(n, nz, seed=9)
+        tg = vam.geometry.TargetGeometry(target=t)
+        case %= 4
         d = rng.random(t.shape).astype(np.float32)
         gel, void = np.asarray(tg.gel_inds), np.asarray(tg.void_inds)
         d[gel] = np.float32(0.5) + np.float32(0.5) * d[gel]   # about half of the void voxels count

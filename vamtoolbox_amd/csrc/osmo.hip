@@ -66,16 +66,32 @@ __global__ void ver_init(VerWork* w) {
   w->count = 0ull;
 }
 
+// VEC: four voxels per step (float4 doses, uchar4 masks; n a multiple of 4)
+template <bool VEC>
 __global__ __launch_bounds__(256) void ver_min_gel(const float* __restrict__ d, const unsigned char* __restrict__ gel,
                                                    long long n, VerWork* w) {
   __shared__ unsigned s_min[4], s_nan[4];
   unsigned m = 0xffffffffu, nan = 0u;
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    if (!gel[i]) continue;
-    const float v = d[i];
+  auto take = [&](bool g, float v) {
+    if (!g) return;
     if (v != v) nan = 1u;
     else m = min(m, ver_key(v));
+  };
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  if (VEC) {
+    const float4* d4 = reinterpret_cast<const float4*>(d);
+    const uchar4* g4 = reinterpret_cast<const uchar4*>(gel);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n / 4; i += stride) {
+      const uchar4 g = g4[i];
+      if (!(g.x | g.y | g.z | g.w)) continue;
+      const float4 v = d4[i];
+      take(g.x, v.x);
+      take(g.y, v.y);
+      take(g.z, v.z);
+      take(g.w, v.w);
+    }
+  } else {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) take(gel[i], d[i]);
   }
   for (int o = 32; o > 0; o >>= 1) {
     m = min(m, (unsigned)__shfl_xor((int)m, o));
@@ -97,6 +113,7 @@ __global__ __launch_bounds__(256) void ver_min_gel(const float* __restrict__ d, 
   }
 }
 
+template <bool VEC>
 __global__ __launch_bounds__(256) void ver_count_void(const float* __restrict__ d,
                                                       const unsigned char* __restrict__ vd, long long n, VerWork* w) {
   __shared__ unsigned long long s_c[4];
@@ -104,8 +121,19 @@ __global__ __launch_bounds__(256) void ver_count_void(const float* __restrict__ 
   const float mg = ver_value(w->min_key);
   unsigned long long c = 0ull;
   const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    if (vd[i] && d[i] >= mg) ++c;
+  if (VEC) {
+    const float4* d4 = reinterpret_cast<const float4*>(d);
+    const uchar4* v4 = reinterpret_cast<const uchar4*>(vd);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n / 4; i += stride) {
+      const uchar4 g = v4[i];
+      if (!(g.x | g.y | g.z | g.w)) continue;
+      const float4 v = d4[i];
+      c += (g.x && v.x >= mg) + (g.y && v.y >= mg) + (g.z && v.z >= mg) + (g.w && v.w >= mg);
+    }
+  } else {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+      if (vd[i] && d[i] >= mg) ++c;
+  }
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
   if (__lane_id() == 0) s_c[threadIdx.x >> 6] = c;
   __syncthreads();
@@ -157,10 +185,18 @@ extern "C" int vamx_ver(const float* dose, const uint8_t* gel, const uint8_t* vo
   if ((uintptr_t)workspace % 8) return set_error(VAMX_ERR_ARG, "vamx_ver: workspace must be 8-byte aligned");
   hipStream_t s = (hipStream_t)stream;
   VerWork* w = (VerWork*)workspace;
-  const dim3 grid((unsigned)ew_blocks(n));
+  const bool vec = n % 4 == 0 && (uintptr_t)dose % 16 == 0 && (uintptr_t)gel % 4 == 0 && (uintptr_t)void_mask % 4 == 0;
+  const dim3 grid((unsigned)ew_blocks(vec ? n / 4 : n));
   hipLaunchKernelGGL(ver_init, dim3(1), dim3(1), 0, s, w);
-  hipLaunchKernelGGL(ver_min_gel, grid, dim3(256), 0, s, dose, (const unsigned char*)gel, (long long)n, w);
-  hipLaunchKernelGGL(ver_count_void, grid, dim3(256), 0, s, dose, (const unsigned char*)void_mask, (long long)n, w);
+  const unsigned char* g8 = (const unsigned char*)gel;
+  const unsigned char* v8 = (const unsigned char*)void_mask;
+  if (vec) {
+    hipLaunchKernelGGL(ver_min_gel<true>, grid, dim3(256), 0, s, dose, g8, (long long)n, w);
+    hipLaunchKernelGGL(ver_count_void<true>, grid, dim3(256), 0, s, dose, v8, (long long)n, w);
+  } else {
+    hipLaunchKernelGGL(ver_min_gel<false>, grid, dim3(256), 0, s, dose, g8, (long long)n, w);
+    hipLaunchKernelGGL(ver_count_void<false>, grid, dim3(256), 0, s, dose, v8, (long long)n, w);
+  }
   hipLaunchKernelGGL(ver_finish, dim3(1), dim3(1), 0, s, (const VerWork*)w, (double)num, ver);
   VAMX_HIP(hipGetLastError());
   return VAMX_OK;
