@@ -66,7 +66,7 @@ __global__ void ver_init(VerWork* w) {
   w->count = 0ull;
 }
 
-// VEC: four voxels per step (float4 doses, uchar4 masks; n a multiple of 4)
+// VEC: sixteen voxels per step (16 mask bytes, four float4 doses; n a multiple of 16)
 template <bool VEC>
 __global__ __launch_bounds__(256) void ver_min_gel(const float* __restrict__ d, const unsigned char* __restrict__ gel,
                                                    long long n, VerWork* w) {
@@ -80,15 +80,19 @@ __global__ __launch_bounds__(256) void ver_min_gel(const float* __restrict__ d, 
   const long long stride = (long long)gridDim.x * blockDim.x;
   if (VEC) {
     const float4* d4 = reinterpret_cast<const float4*>(d);
-    const uchar4* g4 = reinterpret_cast<const uchar4*>(gel);
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n / 4; i += stride) {
-      const uchar4 g = g4[i];
-      if (!(g.x | g.y | g.z | g.w)) continue;
-      const float4 v = d4[i];
-      take(g.x, v.x);
-      take(g.y, v.y);
-      take(g.z, v.z);
-      take(g.w, v.w);
+    const uint4* g16 = reinterpret_cast<const uint4*>(gel);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n / 16; i += stride) {
+      const uint4 gm = g16[i];
+      const unsigned gw[4] = {gm.x, gm.y, gm.z, gm.w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (!gw[q]) continue;
+        const float4 v = d4[4 * i + q];
+        take(gw[q] & 0xffu, v.x);
+        take((gw[q] >> 8) & 0xffu, v.y);
+        take((gw[q] >> 16) & 0xffu, v.z);
+        take(gw[q] >> 24, v.w);
+      }
     }
   } else {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) take(gel[i], d[i]);
@@ -123,12 +127,17 @@ __global__ __launch_bounds__(256) void ver_count_void(const float* __restrict__ 
   const long long stride = (long long)gridDim.x * blockDim.x;
   if (VEC) {
     const float4* d4 = reinterpret_cast<const float4*>(d);
-    const uchar4* v4 = reinterpret_cast<const uchar4*>(vd);
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n / 4; i += stride) {
-      const uchar4 g = v4[i];
-      if (!(g.x | g.y | g.z | g.w)) continue;
-      const float4 v = d4[i];
-      c += (g.x && v.x >= mg) + (g.y && v.y >= mg) + (g.z && v.z >= mg) + (g.w && v.w >= mg);
+    const uint4* v16 = reinterpret_cast<const uint4*>(vd);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n / 16; i += stride) {
+      const uint4 vm = v16[i];
+      const unsigned vw[4] = {vm.x, vm.y, vm.z, vm.w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (!vw[q]) continue;
+        const float4 v = d4[4 * i + q];
+        c += ((vw[q] & 0xffu) && v.x >= mg) + (((vw[q] >> 8) & 0xffu) && v.y >= mg) +
+             (((vw[q] >> 16) & 0xffu) && v.z >= mg) + ((vw[q] >> 24) && v.w >= mg);
+      }
     }
   } else {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
@@ -185,8 +194,9 @@ extern "C" int vamx_ver(const float* dose, const uint8_t* gel, const uint8_t* vo
   if ((uintptr_t)workspace % 8) return set_error(VAMX_ERR_ARG, "vamx_ver: workspace must be 8-byte aligned");
   hipStream_t s = (hipStream_t)stream;
   VerWork* w = (VerWork*)workspace;
-  const bool vec = n % 4 == 0 && (uintptr_t)dose % 16 == 0 && (uintptr_t)gel % 4 == 0 && (uintptr_t)void_mask % 4 == 0;
-  const dim3 grid((unsigned)ew_blocks(vec ? n / 4 : n));
+  const bool vec = n % 16 == 0 && (uintptr_t)dose % 16 == 0 && (uintptr_t)gel % 16 == 0 &&
+                   (uintptr_t)void_mask % 16 == 0;
+  const dim3 grid((unsigned)ew_blocks(vec ? n / 16 : n));
   hipLaunchKernelGGL(ver_init, dim3(1), dim3(1), 0, s, w);
   const unsigned char* g8 = (const unsigned char*)gel;
   const unsigned char* v8 = (const unsigned char*)void_mask;
