@@ -13,7 +13,6 @@ fastest) and the sinogram is P X, again z fastest.  ``forward`` and
 
 from __future__ import annotations
 
-import ctypes
 import logging
 import os
 
