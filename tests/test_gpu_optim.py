@@ -651,6 +651,43 @@ def test_osmo_elementwise_kernels_equal_torch(gpu):
         assert torch.equal(torch.nan_to_num(v, nan=7.0), torch.nan_to_num(ref, nan=7.0)) and bool(torch.isnan(v[17]))
 
 
+def test_segmented_scale_clamp_equals_per_segment_torch(gpu):
+    """vamx_scale_clamp_segments equals torch's amax / div / clamp on every
+    segment alone, bit for bit: uneven and odd segment sizes, an empty
+    segment, a segment holding a NaN (all NaN after), one of negatives only
+    (its maximum negative), and a second call with fewer, smaller segments
+    that reuses the workspace."""
+    import torch
+    from vamtoolbox_amd import _lib
+    lib = _lib.load()
+    st = _lib.stream_ptr(gpu)
+    g = torch.Generator(device=gpu).manual_seed(6)
+    work = torch.empty(16, dtype=torch.int32, device=gpu)
+    for sizes in ((70001, 3, 0, 131072, 1, 5000, 257, 9999), (1000, 17, 4)):
+        n = sum(sizes)
+        x = torch.rand(n, device=gpu, generator=g) * 3.0 - 0.5
+        off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+        planted = len(sizes) == 8
+        if planted:
+            x[off[3] + 77] = float("nan")             # segment 3
+            x[off[6]:off[7]] = -x[off[6]:off[7]].abs() - 0.1   # segment 6: negatives only
+        seg = torch.as_tensor(off, device=gpu)
+        for lo in (float("-inf"), 0.0, 0.05):
+            v = x.clone()
+            _lib.check(lib.vamx_scale_clamp_segments(v.data_ptr(), n, seg.data_ptr(), len(sizes), max(sizes), lo,
+                                                     work.data_ptr(), st), "scale_clamp_segments")
+            ref = x.clone()
+            for s in range(len(sizes)):
+                r = ref[off[s]:off[s + 1]]
+                if r.numel():
+                    r.copy_(torch.clamp(r / torch.amax(r), min=lo))
+            torch.cuda.synchronize()
+            assert torch.equal(torch.nan_to_num(v, nan=7.0), torch.nan_to_num(ref, nan=7.0)), (sizes, lo)
+            assert torch.equal(torch.isnan(v), torch.isnan(ref))
+            if planted:
+                assert bool(torch.isnan(v[off[3]:off[4]]).all())
+
+
 def test_ver_kernel_equals_numpy(gpu):
     """calcVER on a device dose (vamx_ver: the least gel dose, then the count
     of void voxels at or above it) equals the NumPy path (metrics.py:47-87),

@@ -190,6 +190,12 @@ class OSMOBatch:
             g_.copy_(torch.as_tensor(np.asarray(t.gel_inds).reshape(g_.shape)))
             v_.copy_(torch.as_tensor(np.asarray(t.void_inds).reshape(v_.shape)))
         self.gel, self.void = gel, void
+        self._lib = _lib.load()
+        # segment tables of the flat buffers for the per-problem maxima
+        self._vol_seg = torch.as_tensor(np.append(B.vol_off, B.n_vol), dtype=torch.int64, device=dev)
+        self._sino_seg = torch.as_tensor(np.append(B.sino_off, B.n_sino), dtype=torch.int64, device=dev)
+        self._max_vol, self._max_sino = int(max(B.vol_size)), int(max(B.sino_size))
+        self._seg_work = torch.empty(B.k, dtype=torch.int32, device=dev)
         # initialisation (OSMO.py:128-140), per problem
         self.x_model = B.empty_vol()
         for t, xm in zip(self.target_geos, B.vol_views(self.x_model)):
@@ -199,36 +205,40 @@ class OSMOBatch:
         self.b = B.forward(self.x_model)
         self.b.clamp_(min=0.0)
         self.x = B.backward(self.b)
-        self._normalise(self.x, B.vol_views(self.x), clamp_min=None)
+        self._normalise_vol(self.x, float("-inf"))
         self.x_views = B.vol_views(self.x)
         self.b_views = B.sino_views(self.b)
         self.graph = None
 
-    @staticmethod
-    def _normalise(flat, views, clamp_min):
-        for v in views:
-            v.div_(torch.amax(v))
-        if clamp_min is not None:
-            flat.clamp_(min=clamp_min)
+    def _normalise(self, flat, seg, max_seg, lo):
+        """Every problem's v / max(v), then v[v < lo] = lo (OSMO.py:60-69)."""
+        # the current stream at each call: a capture runs step() on its own
+        st = _lib.stream_ptr(self.device)
+        _lib.check(self._lib.vamx_scale_clamp_segments(flat.data_ptr(), flat.numel(), seg.data_ptr(), self.batch.k,
+                                                       max_seg, lo, self._seg_work.data_ptr(), st),
+                   "vamx_scale_clamp_segments")
+
+    def _normalise_vol(self, flat, lo):
+        self._normalise(flat, self._vol_seg, self._max_vol, lo)
 
     def _project(self):
         B = self.batch
         B.forward(self.x_model, out=self.b)
-        for v in self.b_views:
-            v.div_(torch.amax(v))
-        self.b.clamp_(min=self.min_proj_val)      # b < inhibition -> inhibition (OSMO.py:63-66)
+        # b / max(b), b < inhibition -> inhibition (OSMO.py:60-66)
+        self._normalise(self.b, self._sino_seg, self._max_sino, self.min_proj_val)
         B.backward(self.b, out=self.x)
-        for v in self.x_views:
-            v.div_(torch.amax(v))
-        self.x.clamp_(min=0.0)
+        self._normalise_vol(self.x, 0.0)
+
+    def _update_model(self, d, mask, is_gel):
+        _lib.check(self._lib.vamx_osmo_model_update(self.x.data_ptr(), self.x_model.data_ptr(), mask.data_ptr(),
+                                                    self.x.numel(), d, int(is_gel), _lib.stream_ptr(self.device)),
+                   "vamx_osmo_model_update")
 
     def step(self):
         """One OSMO iteration for every problem (OSMO.py:36-109)."""
-        void_diff = torch.clamp(self.x - self.d_l, min=0.0)
-        self.x_model.sub_(torch.where(self.void, void_diff, torch.zeros((), device=self.device)))
+        self._update_model(self.d_l, self.void, False)    # x_model -= max(x - d_l, 0) on void
         self._project()
-        gel_diff = torch.clamp(self.d_h - self.x, min=0.0)
-        self.x_model.add_(torch.where(self.gel, gel_diff, torch.zeros((), device=self.device)))
+        self._update_model(self.d_h, self.gel, True)      # x_model += max(d_h - x, 0) on gel
         self._project()
 
     def capture(self):

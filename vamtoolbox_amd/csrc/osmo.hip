@@ -39,6 +39,65 @@ __global__ __launch_bounds__(256) void scale_clamp(float* __restrict__ v, long l
 long long ew_blocks(long long n) { return std::min<long long>(4096, std::max<long long>(1, (n + 255) / 256)); }
 
 // ---------------------------------------------------------------------------
+// The same normalisation for k arrays packed one after another in one buffer
+// (segment s is v[off[s], off[s+1]), each clipped to [0, n)), as the batched
+// OSMO runs OSMO.py:60-69 on every problem: one grid pass for all the maxima
+// and one for all the divisions.  The maximum is exact (atomicMax of ordered
+// keys) and a NaN anywhere in a segment makes its maximum NaN, as np.max and
+// torch.amax do.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned max_key(float v) {
+  if (v != v) return 0xffffffffu;   // every NaN above +inf
+  const unsigned b = __float_as_uint(v);
+  return (b >> 31) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float max_value(unsigned k) {
+  return __uint_as_float((k >> 31) ? (k & 0x7fffffffu) : ~k);
+}
+
+__device__ __forceinline__ void seg_range(const long long* off, int s, long long n, long long& b, long long& e) {
+  b = min(max(off[s], 0ll), n);
+  e = max(b, min(off[s + 1], n));
+}
+
+__global__ void seg_max_init(unsigned* key, int k) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < k) key[s] = 0u;   // below every key of a float
+}
+
+// grid (chunks, k): blockIdx.y is the segment
+__global__ __launch_bounds__(256) void seg_max(const float* __restrict__ v, long long n,
+                                               const long long* __restrict__ off, unsigned* __restrict__ key) {
+  __shared__ unsigned s_m[4];
+  long long b, e;
+  seg_range(off, blockIdx.y, n, b, e);
+  unsigned m = 0u;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = b + (long long)blockIdx.x * blockDim.x + threadIdx.x; i < e; i += stride)
+    m = max(m, max_key(v[i]));
+  for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o));
+  if (__lane_id() == 0) s_m[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned bm = max(max(s_m[0], s_m[1]), max(s_m[2], s_m[3]));
+    if (bm) atomicMax(&key[blockIdx.y], bm);
+  }
+}
+
+__global__ __launch_bounds__(256) void seg_scale_clamp(float* __restrict__ v, long long n,
+                                                       const long long* __restrict__ off,
+                                                       const unsigned* __restrict__ key, float lo) {
+  long long b, e;
+  seg_range(off, blockIdx.y, n, b, e);
+  const float mv = max_value(key[blockIdx.y]);
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = b + (long long)blockIdx.x * blockDim.x + threadIdx.x; i < e; i += stride) {
+    const float s = v[i] / mv;
+    v[i] = s < lo ? lo : s;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // The volumetric error rate (metrics.py:47-87): #(void voxels with dose >=
 // the least gel dose) / (#gel + #void), as two grid passes over the dose with
 // integer atomics (order-independent, so the value does not depend on the
@@ -177,6 +236,27 @@ extern "C" int vamx_scale_clamp(float* v, int64_t n, const float* m, float lo, v
   if (n == 0) return VAMX_OK;
   hipLaunchKernelGGL(scale_clamp, dim3((unsigned)ew_blocks(n)), dim3(256), 0, (hipStream_t)stream, v, (long long)n, m,
                      lo);
+  VAMX_HIP(hipGetLastError());
+  return VAMX_OK;
+}
+
+extern "C" int vamx_scale_clamp_segments(float* v, int64_t n, const int64_t* seg_off, int32_t k, int64_t max_seg,
+                                         float lo, void* workspace, void* stream) {
+  if (!v || !seg_off || !workspace) return set_error(VAMX_ERR_ARG, "vamx_scale_clamp_segments: null argument");
+  if (n < 0 || k < 1 || k > 65535 || max_seg < 0)
+    return set_error(VAMX_ERR_ARG, "vamx_scale_clamp_segments: need n >= 0, 1 <= k <= 65535, max_seg >= 0");
+  if ((uintptr_t)workspace % 4 || (uintptr_t)seg_off % 8)
+    return set_error(VAMX_ERR_ARG, "vamx_scale_clamp_segments: misaligned workspace or offsets");
+  if (n == 0) return VAMX_OK;
+  hipStream_t s = (hipStream_t)stream;
+  unsigned* key = (unsigned*)workspace;
+  const long long* off = (const long long*)seg_off;
+  // blocks per segment cover the largest one; a segment with fewer elements
+  // leaves the rest of its blocks idle
+  const dim3 grid((unsigned)std::min<long long>(256, std::max<long long>(1, (max_seg + 1023) / 1024)), (unsigned)k);
+  hipLaunchKernelGGL(seg_max_init, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, s, key, (int)k);
+  hipLaunchKernelGGL(seg_max, grid, dim3(256), 0, s, (const float*)v, (long long)n, off, key);
+  hipLaunchKernelGGL(seg_scale_clamp, grid, dim3(256), 0, s, v, (long long)n, off, (const unsigned*)key, lo);
   VAMX_HIP(hipGetLastError());
   return VAMX_OK;
 }
