@@ -295,7 +295,7 @@ int create_batch(vamx_plan* const* plans, int n, vamx_batch** out) {
 // kernel path of problem i for these buffers: wave z width (1, 2, 4; 3 = z
 // and z + 64), or -1 / -4 for the per-lane kernels with 1 / 4 z per thread
 int problem_path(vamx_plan* p, const float* in, float* out, bool forward) {
-  int wz = par_wave_zv(p, in, out, forward ? p->par.fp_zv : p->par.bp_zv);
+  int wz = forward ? par_fp_wave_zv(p, in, out) : par_wave_zv(p, in, out, p->par.bp_zv);
   if (par_wave_split(p, wz)) return 3;
   if (wz) return wz;
   return par_use_vec4(p, in, out) ? -4 : -1;

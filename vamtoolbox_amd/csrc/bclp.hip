@@ -441,7 +441,7 @@ int bclp_update_impl(vamx_plan* plan, const vamx_bclp_params* prm, float* g, con
       rc = par_slice_minmax(plan, operand, s);
       if (rc) return rc;
     }
-    const int wz = par_wave_zv(plan, g, operand, gd.fp_zv);
+    const int wz = par_fp_wave_zv(plan, operand, g);
     if (wz && gd.fp_pair && gd.pair_ok) {
       const long long threads = fp_pair_tiled(gd) * wave_zchunks(gd.nz, wz) * 64;
       const dim3 grid(blocks_for(threads, 256)), block(256);

@@ -1479,6 +1479,7 @@ int par_slice_minmax(vamx_plan* p, const float* vol, hipStream_t s);
 int par_forward_launch(vamx_plan* p, const float* vol, float* sino, hipStream_t s, bool warp_clip);
 bool par_use_vec4(const vamx_plan* p, const void* a, const void* b);
 int par_wave_zv(const vamx_plan* p, const void* a, const void* b, int requested);
+int par_fp_wave_zv(const vamx_plan* p, const void* vol, const void* sino);
 bool par_wave_split(const vamx_plan* p, int zv);
 int wave_zv_choice(int nz, uintptr_t align_bits, int requested);
 void par_pair_units(const std::vector<double>& cs, int na, int n, int c, double D, std::vector<int>& units,

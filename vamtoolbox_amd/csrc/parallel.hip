@@ -427,6 +427,19 @@ int vamx::wave_zv_choice(int nz, uintptr_t align_bits, int requested) {
   return best_util >= 0.5 ? best : 0;
 }
 
+int vamx::par_fp_wave_zv(const vamx_plan* p, const void* vol, const void* sino) {
+  // the forward's wavefront z width, 0 for the per-lane kernel.  At one z
+  // per lane a wavefront issues one 4-byte tap load per lane, and the forward
+  // is bound by load issue, not bytes; the per-lane kernel with four z per
+  // thread (several rays per wavefront, 16-byte tap loads) is then faster
+  // up to nz = 56.  At 256^2 x 360 with absorption, nz 32 / 40 / 48 measured
+  // 0.66 / 0.82 / 0.93 ms per-lane against 1.23 / 1.25 / 1.22 ms for the
+  // wavefront kernel, and they tie at nz = 64.
+  const int wz = par_wave_zv(p, vol, sino, p->par.fp_zv);
+  if (wz == 1 && p->par.nz <= 56 && par_use_vec4(p, vol, sino)) return 0;
+  return wz;
+}
+
 bool vamx::par_wave_split(const vamx_plan* p, int zv) {
   // the split z layout (ZV = 2, lanes own z and z + 64) where the contiguous
   // choice is one z per lane: half the wavefronts at the same lane
@@ -653,7 +666,7 @@ int par_forward_launch(vamx_plan* p, const float* vol, float* sino, hipStream_t 
     int rc = par_slice_minmax(p, vol, s);
     if (rc) return rc;
   }
-  int wz = par_wave_zv(p, vol, sino, g.fp_zv);
+  int wz = par_fp_wave_zv(p, vol, sino);
   const bool split = par_wave_split(p, wz);
   if (split) wz = 2;
   if (wz && g.fp_pair && g.pair_ok) {
