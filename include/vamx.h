@@ -373,13 +373,12 @@ int vamx_osmo_model_update(const float* x, float* x_model, const uint8_t* mask, 
 int vamx_scale_clamp(float* v, int64_t n, const float* m, float lo, void* stream);
 /* vamx_scale_clamp on k arrays packed in one buffer, each with its own
  * maximum: segment s is v[seg_off[s], seg_off[s+1]) (device int64 offsets,
- * k + 1 of them, each clipped to [0, n)); v[i] = v[i] / max(segment), then
- * v[i] < lo -> lo.  A NaN in a segment makes its maximum NaN.  max_seg (the
- * largest segment's length) only sizes the grid.  workspace: k uint32,
- * 4-byte aligned.  The batched OSMO's per-problem normalisations
+ * k + 1 of them, non-decreasing, off[k] = n; each range is clipped to
+ * [0, n)); v[i] = v[i] / max(segment), then v[i] < lo -> lo.  A NaN in a
+ * segment makes its maximum NaN.  workspace: k uint32, 4-byte aligned.  The batched OSMO's per-problem normalisations
  * (OSMO.py:60-69 for every problem); no reference counterpart as one call. */
-int vamx_scale_clamp_segments(float* v, int64_t n, const int64_t* seg_off, int32_t k, int64_t max_seg, float lo,
-                              void* workspace, void* stream);
+int vamx_scale_clamp_segments(float* v, int64_t n, const int64_t* seg_off, int32_t k, float lo, void* workspace,
+                              void* stream);
 
 /* calcVER (vamtoolbox/metrics.py:47-87, not layerwise) of a device float32
  * dose of n voxels with device uint8 gel / void masks: *ver (device double)

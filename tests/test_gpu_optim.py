@@ -656,7 +656,7 @@ def test_segmented_scale_clamp_equals_per_segment_torch(gpu):
     segment alone, bit for bit: uneven and odd segment sizes, an empty
     segment, a segment holding a NaN (all NaN after), one of negatives only
     (its maximum negative), and a second call with fewer, smaller segments
-    that reuses the workspace."""
+    that reuses the workspace; 16-byte aligned and misaligned buffers."""
     import torch
     from vamtoolbox_amd import _lib
     lib = _lib.load()
@@ -672,10 +672,12 @@ def test_segmented_scale_clamp_equals_per_segment_torch(gpu):
             x[off[3] + 77] = float("nan")             # segment 3
             x[off[6]:off[7]] = -x[off[6]:off[7]].abs() - 0.1   # segment 6: negatives only
         seg = torch.as_tensor(off, device=gpu)
-        for lo in (float("-inf"), 0.0, 0.05):
-            v = x.clone()
-            _lib.check(lib.vamx_scale_clamp_segments(v.data_ptr(), n, seg.data_ptr(), len(sizes), max(sizes), lo,
-                                                     work.data_ptr(), st), "scale_clamp_segments")
+        for lo, shift in ((float("-inf"), 0), (0.0, 0), (0.05, 0), (0.05, 1)):
+            # shift 1: a buffer 4 bytes past a 16-byte boundary
+            v = torch.empty(n + shift, device=gpu)[shift:]
+            v.copy_(x)
+            _lib.check(lib.vamx_scale_clamp_segments(v.data_ptr(), n, seg.data_ptr(), len(sizes), lo, work.data_ptr(),
+                                                     st), "scale_clamp_segments")
             ref = x.clone()
             for s in range(len(sizes)):
                 r = ref[off[s]:off[s + 1]]

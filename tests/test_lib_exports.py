@@ -83,14 +83,14 @@ def test_osmo_elementwise_argument_errors_without_gpu_work():
     assert lib.vamx_scale_clamp(fake, 10, None, 0.0, None) == -1
     assert lib.vamx_scale_clamp(fake, -3, fake, 0.0, None) == -1
     assert lib.vamx_scale_clamp(fake, 0, fake, 0.0, None) == 0
-    # vamx_scale_clamp_segments(v, n, seg_off, k, max_seg, lo, workspace, stream)
-    assert lib.vamx_scale_clamp_segments(fake, 10, None, 2, 5, 0.0, fake, None) == -1
-    assert lib.vamx_scale_clamp_segments(fake, 10, fake, 0, 5, 0.0, fake, None) == -1
-    assert lib.vamx_scale_clamp_segments(fake, 10, fake, 70000, 5, 0.0, fake, None) == -1
-    assert lib.vamx_scale_clamp_segments(fake, -1, fake, 2, 5, 0.0, fake, None) == -1
-    assert lib.vamx_scale_clamp_segments(fake, 10, ctypes.c_void_p(260), 2, 5, 0.0, fake, None) == -1
-    assert lib.vamx_scale_clamp_segments(fake, 10, fake, 2, 5, 0.0, ctypes.c_void_p(258), None) == -1
-    assert lib.vamx_scale_clamp_segments(fake, 0, fake, 2, 5, 0.0, fake, None) == 0
+    # vamx_scale_clamp_segments(v, n, seg_off, k, lo, workspace, stream)
+    assert lib.vamx_scale_clamp_segments(fake, 10, None, 2, 0.0, fake, None) == -1
+    assert lib.vamx_scale_clamp_segments(fake, 10, fake, 0, 0.0, fake, None) == -1
+    assert lib.vamx_scale_clamp_segments(fake, 10, fake, 70000, 0.0, fake, None) == -1
+    assert lib.vamx_scale_clamp_segments(fake, -1, fake, 2, 0.0, fake, None) == -1
+    assert lib.vamx_scale_clamp_segments(fake, 10, ctypes.c_void_p(260), 2, 0.0, fake, None) == -1
+    assert lib.vamx_scale_clamp_segments(fake, 10, fake, 2, 0.0, ctypes.c_void_p(258), None) == -1
+    assert lib.vamx_scale_clamp_segments(fake, 0, fake, 2, 0.0, fake, None) == 0
 
 
 def test_ver_argument_errors_without_gpu_work():

@@ -180,8 +180,8 @@ SIGNATURES = {
     "vamx_osmo_model_update": (c_int32, [c_void_p, c_void_p, c_void_p, ctypes.c_int64, ctypes.c_float, c_int32,
                                          c_void_p]),
     "vamx_scale_clamp": (c_int32, [c_void_p, ctypes.c_int64, c_void_p, ctypes.c_float, c_void_p]),
-    "vamx_scale_clamp_segments": (c_int32, [c_void_p, ctypes.c_int64, c_void_p, c_int32, ctypes.c_int64,
-                                            ctypes.c_float, c_void_p, c_void_p]),
+    "vamx_scale_clamp_segments": (c_int32, [c_void_p, ctypes.c_int64, c_void_p, c_int32, ctypes.c_float, c_void_p,
+                                            c_void_p]),
     "vamx_ver_workspace_bytes": (c_int32, [ctypes.POINTER(ctypes.c_int64)]),
     "vamx_ver": (c_int32, [c_void_p, c_void_p, c_void_p, ctypes.c_int64, ctypes.c_int64, c_void_p, c_void_p,
                            c_void_p]),

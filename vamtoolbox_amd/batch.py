@@ -194,7 +194,6 @@ class OSMOBatch:
         # segment tables of the flat buffers for the per-problem maxima
         self._vol_seg = torch.as_tensor(np.append(B.vol_off, B.n_vol), dtype=torch.int64, device=dev)
         self._sino_seg = torch.as_tensor(np.append(B.sino_off, B.n_sino), dtype=torch.int64, device=dev)
-        self._max_vol, self._max_sino = int(max(B.vol_size)), int(max(B.sino_size))
         self._seg_work = torch.empty(B.k, dtype=torch.int32, device=dev)
         # initialisation (OSMO.py:128-140), per problem
         self.x_model = B.empty_vol()
@@ -210,22 +209,22 @@ class OSMOBatch:
         self.b_views = B.sino_views(self.b)
         self.graph = None
 
-    def _normalise(self, flat, seg, max_seg, lo):
+    def _normalise(self, flat, seg, lo):
         """Every problem's v / max(v), then v[v < lo] = lo (OSMO.py:60-69)."""
         # the current stream at each call: a capture runs step() on its own
         st = _lib.stream_ptr(self.device)
         _lib.check(self._lib.vamx_scale_clamp_segments(flat.data_ptr(), flat.numel(), seg.data_ptr(), self.batch.k,
-                                                       max_seg, lo, self._seg_work.data_ptr(), st),
+                                                       lo, self._seg_work.data_ptr(), st),
                    "vamx_scale_clamp_segments")
 
     def _normalise_vol(self, flat, lo):
-        self._normalise(flat, self._vol_seg, self._max_vol, lo)
+        self._normalise(flat, self._vol_seg, lo)
 
     def _project(self):
         B = self.batch
         B.forward(self.x_model, out=self.b)
         # b / max(b), b < inhibition -> inhibition (OSMO.py:60-66)
-        self._normalise(self.b, self._sino_seg, self._max_sino, self.min_proj_val)
+        self._normalise(self.b, self._sino_seg, self.min_proj_val)
         B.backward(self.b, out=self.x)
         self._normalise_vol(self.x, 0.0)
 

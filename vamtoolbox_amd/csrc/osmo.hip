@@ -55,9 +55,26 @@ __device__ __forceinline__ float max_value(unsigned k) {
   return __uint_as_float((k >> 31) ? (k & 0x7fffffffu) : ~k);
 }
 
+// One block per chunk of kSegChunk elements of the whole buffer, so every
+// block has the same work whatever the segment sizes; a chunk that straddles
+// segment ends walks them in order (block-uniform loop).
+constexpr int kSegChunk = 8192;
+
+// segment s's range, clipped to [0, n)
 __device__ __forceinline__ void seg_range(const long long* off, int s, long long n, long long& b, long long& e) {
   b = min(max(off[s], 0ll), n);
   e = max(b, min(off[s + 1], n));
+}
+
+// the last segment whose start is <= pos (offsets non-decreasing)
+__device__ __forceinline__ int seg_find(const long long* off, int k, long long pos) {
+  int lo = 0, hi = k - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= pos) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
 }
 
 __global__ void seg_max_init(unsigned* key, int k) {
@@ -65,35 +82,51 @@ __global__ void seg_max_init(unsigned* key, int k) {
   if (s < k) key[s] = 0u;   // below every key of a float
 }
 
-// grid (chunks, k): blockIdx.y is the segment
 __global__ __launch_bounds__(256) void seg_max(const float* __restrict__ v, long long n,
-                                               const long long* __restrict__ off, unsigned* __restrict__ key) {
+                                               const long long* __restrict__ off, int k,
+                                               unsigned* __restrict__ key) {
   __shared__ unsigned s_m[4];
-  long long b, e;
-  seg_range(off, blockIdx.y, n, b, e);
-  unsigned m = 0u;
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long i = b + (long long)blockIdx.x * blockDim.x + threadIdx.x; i < e; i += stride)
-    m = max(m, max_key(v[i]));
-  for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o));
-  if (__lane_id() == 0) s_m[threadIdx.x >> 6] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned bm = max(max(s_m[0], s_m[1]), max(s_m[2], s_m[3]));
-    if (bm) atomicMax(&key[blockIdx.y], bm);
+  long long pos = (long long)blockIdx.x * kSegChunk;
+  const long long cend = min(pos + kSegChunk, n);
+  while (pos < cend) {
+    const int s = seg_find(off, k, pos);
+    long long b, e;
+    seg_range(off, s, n, b, e);
+    if (e <= pos) break;   // pos in no segment: offsets not as documented
+    e = min(e, cend);
+    unsigned m = 0u;
+#pragma unroll 4
+    for (long long i = pos + threadIdx.x; i < e; i += 256) m = max(m, max_key(v[i]));
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o));
+    if (__lane_id() == 0) s_m[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const unsigned bm = max(max(s_m[0], s_m[1]), max(s_m[2], s_m[3]));
+      if (bm) atomicMax(&key[s], bm);
+    }
+    __syncthreads();   // s_m is reused by the next segment of this chunk
+    pos = e;
   }
 }
 
 __global__ __launch_bounds__(256) void seg_scale_clamp(float* __restrict__ v, long long n,
-                                                       const long long* __restrict__ off,
+                                                       const long long* __restrict__ off, int k,
                                                        const unsigned* __restrict__ key, float lo) {
-  long long b, e;
-  seg_range(off, blockIdx.y, n, b, e);
-  const float mv = max_value(key[blockIdx.y]);
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long i = b + (long long)blockIdx.x * blockDim.x + threadIdx.x; i < e; i += stride) {
-    const float s = v[i] / mv;
-    v[i] = s < lo ? lo : s;
+  long long pos = (long long)blockIdx.x * kSegChunk;
+  const long long cend = min(pos + kSegChunk, n);
+  while (pos < cend) {
+    const int s = seg_find(off, k, pos);
+    long long b, e;
+    seg_range(off, s, n, b, e);
+    if (e <= pos) break;
+    e = min(e, cend);
+    const float mv = max_value(key[s]);
+#pragma unroll 4
+    for (long long i = pos + threadIdx.x; i < e; i += 256) {
+      const float q = v[i] / mv;
+      v[i] = q < lo ? lo : q;
+    }
+    pos = e;
   }
 }
 
@@ -240,23 +273,23 @@ extern "C" int vamx_scale_clamp(float* v, int64_t n, const float* m, float lo, v
   return VAMX_OK;
 }
 
-extern "C" int vamx_scale_clamp_segments(float* v, int64_t n, const int64_t* seg_off, int32_t k, int64_t max_seg,
-                                         float lo, void* workspace, void* stream) {
+extern "C" int vamx_scale_clamp_segments(float* v, int64_t n, const int64_t* seg_off, int32_t k, float lo,
+                                         void* workspace, void* stream) {
   if (!v || !seg_off || !workspace) return set_error(VAMX_ERR_ARG, "vamx_scale_clamp_segments: null argument");
-  if (n < 0 || k < 1 || k > 65535 || max_seg < 0)
-    return set_error(VAMX_ERR_ARG, "vamx_scale_clamp_segments: need n >= 0, 1 <= k <= 65535, max_seg >= 0");
+  if (n < 0 || k < 1 || k > 65535)
+    return set_error(VAMX_ERR_ARG, "vamx_scale_clamp_segments: need n >= 0 and 1 <= k <= 65535");
   if ((uintptr_t)workspace % 4 || (uintptr_t)seg_off % 8)
     return set_error(VAMX_ERR_ARG, "vamx_scale_clamp_segments: misaligned workspace or offsets");
   if (n == 0) return VAMX_OK;
   hipStream_t s = (hipStream_t)stream;
   unsigned* key = (unsigned*)workspace;
   const long long* off = (const long long*)seg_off;
-  // blocks per segment cover the largest one; a segment with fewer elements
-  // leaves the rest of its blocks idle
-  const dim3 grid((unsigned)std::min<long long>(256, std::max<long long>(1, (max_seg + 1023) / 1024)), (unsigned)k);
+  const long long chunks = (n + kSegChunk - 1) / kSegChunk;
+  if (chunks > 0x7fffffffll) return set_error(VAMX_ERR_ARG, "vamx_scale_clamp_segments: array too large");
+  const dim3 grid((unsigned)chunks);
   hipLaunchKernelGGL(seg_max_init, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, s, key, (int)k);
-  hipLaunchKernelGGL(seg_max, grid, dim3(256), 0, s, (const float*)v, (long long)n, off, key);
-  hipLaunchKernelGGL(seg_scale_clamp, grid, dim3(256), 0, s, v, (long long)n, off, (const unsigned*)key, lo);
+  hipLaunchKernelGGL(seg_max, grid, dim3(256), 0, s, (const float*)v, (long long)n, off, (int)k, key);
+  hipLaunchKernelGGL(seg_scale_clamp, grid, dim3(256), 0, s, v, (long long)n, off, (int)k, (const unsigned*)key, lo);
   VAMX_HIP(hipGetLastError());
   return VAMX_OK;
 }
