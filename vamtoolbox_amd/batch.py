@@ -14,8 +14,10 @@ large ones leave.
 ``minimizeOSMOBatched`` runs the OSMO iteration of
 vamtoolbox/optimizer/OSMO.py:9-178 for every problem (the same steps as
 ``vamtoolbox_amd.optimizer.OSMO.minimizeOSMO`` on each problem alone) with
-the element-wise steps on the flat buffers and the projections batched; the
-iteration is captured once in a HIP graph and replayed.
+the projections batched and the element-wise steps as single launches over
+the flat buffers (``vamx_osmo_model_update`` for the model updates,
+``vamx_scale_clamp_segments`` for every problem's max normalisation at
+once); the iteration is captured once in a HIP graph and replayed.
 """
 
 from __future__ import annotations
@@ -158,9 +160,10 @@ class OSMOBatch:
     """State of the batched OSMO iteration (OSMO.py:9-178 for every problem).
 
     ``step()`` enqueues one iteration (void step then gel step, each one
-    batched forward and one batched back projection with the per-problem max
-    normalisations) without host synchronisation; ``run(n)`` replays it,
-    from a captured HIP graph when ``hip_graph`` is set.
+    model update, one batched forward and one batched back projection with
+    the per-problem max normalisations) on the current stream without host
+    synchronisation; ``capture()`` records it in a HIP graph that
+    ``graph.replay()`` runs again.
     """
 
     def __init__(self, target_geos, proj_geos, options, device=None, autotune=False):
