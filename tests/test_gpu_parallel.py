@@ -169,6 +169,34 @@ def test_wave_kernels_parity(gpu, case, positive):
     assert rel_l2(fw, fl) <= 1e-6 and rel_l2(bw, bl) <= 1e-6
 
 
+@pytest.mark.parametrize("nz, kw", [(6, dict(mu=0.02)), (34, dict(det_offset=0.5)), (18, {})])
+@pytest.mark.parametrize("positive", [False, True])
+def test_per_lane_forward_two_z_per_thread(gpu, nz, kw, positive):
+    """The per-lane forward with two z per thread (even nz, not a multiple of
+    4, 8-byte aligned buffers) against the oracle, and bitwise equal to the
+    one-z-per-thread kernel that a buffer 4 bytes off an 8-byte boundary
+    takes; warp clip inactive (signed data) and active (positive slices)."""
+    import torch
+    import oracle.d1 as d1
+    from vamtoolbox_amd.parallel import ParallelPlan
+    n, ang = 23, np.arange(0, 360, 9.0)
+    print("nz", nz, kw, "positive" if positive else "signed", flush=True)
+    rng = np.random.default_rng(700 + nz)
+    vol = rng.uniform(0.2, 1.0, (n, n, nz)) if positive else rng.uniform(-1, 1, (n, n, nz))
+    lane = ParallelPlan(n, nz, ang, device=gpu, per_lane_geometry=True, **kw)
+    tv = torch.tensor(vol, dtype=torch.float32, device=gpu)
+    f2 = lane.forward(tv)
+    vb = torch.empty(tv.numel() + 1, device=gpu)[1:].view(lane.vol_shape)
+    vb.copy_(tv)
+    sb = torch.empty(f2.numel() + 1, device=gpu)[1:].view(lane.sino_shape)
+    lane.forward(vb, out=sb)
+    assert f2.data_ptr() % 8 == 0 and sb.data_ptr() % 8 == 4
+    assert torch.equal(f2, sb)
+    e = rel_l2(f2.cpu().numpy(), d1.forward(vol, ang, **kw))
+    print("rel L2 forward %.2e" % e)
+    assert e <= TOL
+
+
 @pytest.mark.parametrize("na", [1, 15, 16, 17, 47])
 @pytest.mark.parametrize("nz", [40, 64, 128, 256])
 def test_angle_batch_boundaries(gpu, na, nz):

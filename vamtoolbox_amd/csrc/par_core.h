@@ -1478,6 +1478,7 @@ __device__ __forceinline__ void fpT_quad_wave(const ParDev& g, const float* __re
 int par_slice_minmax(vamx_plan* p, const float* vol, hipStream_t s);
 int par_forward_launch(vamx_plan* p, const float* vol, float* sino, hipStream_t s, bool warp_clip);
 bool par_use_vec4(const vamx_plan* p, const void* a, const void* b);
+bool par_use_vec2(const vamx_plan* p, const void* a, const void* b);
 int par_wave_zv(const vamx_plan* p, const void* a, const void* b, int requested);
 int par_fp_wave_zv(const vamx_plan* p, const void* vol, const void* sino);
 bool par_wave_split(const vamx_plan* p, int zv);

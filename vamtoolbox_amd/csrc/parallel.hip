@@ -449,6 +449,10 @@ bool vamx::par_wave_split(const vamx_plan* p, int zv) {
   return ch % 2 == 0;
 }
 
+bool vamx::par_use_vec2(const vamx_plan* p, const void* a, const void* b) {
+  return (p->par.nz % 2 == 0) && !p->par.incl && ((uintptr_t)a % 8 == 0) && ((uintptr_t)b % 8 == 0);
+}
+
 bool vamx::par_use_vec4(const vamx_plan* p, const void* a, const void* b) {
   return (p->par.nz % 4 == 0) && !p->par.incl && ((uintptr_t)a % 16 == 0) && ((uintptr_t)b % 16 == 0);
 }
@@ -712,7 +716,8 @@ int par_forward_launch(vamx_plan* p, const float* vol, float* sino, hipStream_t 
     return VAMX_OK;
   }
   const bool v4 = par_use_vec4(p, vol, sino);
-  const int zv = v4 ? 4 : 1;
+  const bool v2 = !v4 && par_use_vec2(p, vol, sino);
+  const int zv = v4 ? 4 : v2 ? 2 : 1;
   const long long total = (long long)g.na * g.n * (g.nz / zv);
   const dim3 grid(blocks_for(total, 256)), block(256);
   if (v4) {
@@ -720,6 +725,11 @@ int par_forward_launch(vamx_plan* p, const float* vol, float* sino, hipStream_t 
       hipLaunchKernelGGL((par_fp_kernel<4, true>), grid, block, 0, s, g, vol, sino, p->d_minmax);
     else
       hipLaunchKernelGGL((par_fp_kernel<4, false>), grid, block, 0, s, g, vol, sino, p->d_minmax);
+  } else if (v2) {
+    if (warp_clip)
+      hipLaunchKernelGGL((par_fp_kernel<2, true>), grid, block, 0, s, g, vol, sino, p->d_minmax);
+    else
+      hipLaunchKernelGGL((par_fp_kernel<2, false>), grid, block, 0, s, g, vol, sino, p->d_minmax);
   } else {
     if (warp_clip)
       hipLaunchKernelGGL((par_fp_kernel<1, true>), grid, block, 0, s, g, vol, sino, p->d_minmax);
