@@ -171,11 +171,12 @@ def test_wave_kernels_parity(gpu, case, positive):
 
 @pytest.mark.parametrize("nz, kw", [(6, dict(mu=0.02)), (34, dict(det_offset=0.5)), (18, {})])
 @pytest.mark.parametrize("positive", [False, True])
-def test_per_lane_forward_two_z_per_thread(gpu, nz, kw, positive):
-    """The per-lane forward with two z per thread (even nz, not a multiple of
-    4, 8-byte aligned buffers) against the oracle, and bitwise equal to the
-    one-z-per-thread kernel that a buffer 4 bytes off an 8-byte boundary
-    takes; warp clip inactive (signed data) and active (positive slices)."""
+def test_per_lane_two_z_per_thread(gpu, nz, kw, positive):
+    """The per-lane forward and back projection with two z per thread (even
+    nz, not a multiple of 4, 8-byte aligned buffers) against the oracle, and
+    bitwise equal to the one-z-per-thread kernels that buffers 4 bytes off an
+    8-byte boundary take; warp clip inactive (signed data) and active
+    (positive slices)."""
     import torch
     import oracle.d1 as d1
     from vamtoolbox_amd.parallel import ParallelPlan
@@ -192,9 +193,19 @@ def test_per_lane_forward_two_z_per_thread(gpu, nz, kw, positive):
     lane.forward(vb, out=sb)
     assert f2.data_ptr() % 8 == 0 and sb.data_ptr() % 8 == 4
     assert torch.equal(f2, sb)
-    e = rel_l2(f2.cpu().numpy(), d1.forward(vol, ang, **kw))
-    print("rel L2 forward %.2e" % e)
-    assert e <= TOL
+    sino = rng.uniform(0, 1, (n, len(ang), nz))
+    ts = torch.tensor(sino, dtype=torch.float32, device=gpu)
+    b2 = lane.backward(ts)
+    tb = torch.empty(ts.numel() + 1, device=gpu)[1:].view(lane.sino_shape)
+    tb.copy_(ts)
+    bb = torch.empty(b2.numel() + 1, device=gpu)[1:].view(lane.vol_shape)
+    lane.backward(tb, out=bb)
+    assert b2.data_ptr() % 8 == 0 and bb.data_ptr() % 8 == 4
+    assert torch.equal(b2, bb)
+    e1 = rel_l2(f2.cpu().numpy(), d1.forward(vol, ang, **kw))
+    e2 = rel_l2(b2.cpu().numpy(), d1.backward(sino, ang, **kw))
+    print("rel L2 forward %.2e backward %.2e" % (e1, e2))
+    assert e1 <= TOL and e2 <= TOL
 
 
 @pytest.mark.parametrize("na", [1, 15, 16, 17, 47])

@@ -768,7 +768,8 @@ int par_backward_launch(vamx_plan* p, const float* sino, float* vol, hipStream_t
     return VAMX_OK;
   }
   const bool v4 = par_use_vec4(p, sino, vol);
-  const int zv = v4 ? 4 : 1;
+  const bool v2 = !v4 && par_use_vec2(p, sino, vol);
+  const int zv = v4 ? 4 : v2 ? 2 : 1;
   const long long total = (long long)g.n * g.n * (g.nz / zv);
   const dim3 grid(blocks_for(total, 256)), block(256);
   if (v4) {
@@ -776,6 +777,11 @@ int par_backward_launch(vamx_plan* p, const float* sino, float* vol, hipStream_t
       hipLaunchKernelGGL((par_bp_kernel<4, true>), grid, block, 0, s, g, sino, vol);
     else
       hipLaunchKernelGGL((par_bp_kernel<4, false>), grid, block, 0, s, g, sino, vol);
+  } else if (v2) {
+    if (clipc)
+      hipLaunchKernelGGL((par_bp_kernel<2, true>), grid, block, 0, s, g, sino, vol);
+    else
+      hipLaunchKernelGGL((par_bp_kernel<2, false>), grid, block, 0, s, g, sino, vol);
   } else {
     if (clipc)
       hipLaunchKernelGGL((par_bp_kernel<1, true>), grid, block, 0, s, g, sino, vol);
